@@ -1,0 +1,187 @@
+"""Benchmark of the ZDT hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): one Float32
+spectrogram of Nf=2^20 channels x Nt=16 time steps (Exponential(1) noise plus
+16 drifting tones, synthetic, seeded), drift rates -4:1/16:4 (Nr=129, Nl=144).
+One step = zdtfdr!(dest, ws, spectrogram) = input! + one ZDT batch writing the
+2^20 x 129 Float32 FDR, with the spectrogram already resident in HBM.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+
+Prints ONE JSON line.  value = FDR cells/s = Nf*Nr / (seconds per step).
+The roofline object covers the kernel with the largest share of the step
+time, timed live with HIP events on the workspace stream (library profiling
+hook); its algorithmic bytes per launch come from the library.  The
+cpu_baseline object times the FP32 CPU restatement of the reference
+(oracle.zdt_oracle.RefWorkspace32) on one batch of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "frequencydriftratetransforms.jl_amd"
+sys.path.insert(0, str(PKG))
+
+HBM_PEAK_GBS = 8000.0
+NF, NT = 1 << 20, 16
+R0, DR, NR = -4.0, 0.0625, 129
+
+
+def synthetic_spectrogram(Nf: int, Nt: int, seed: int = 1234) -> np.ndarray:
+    """Exponential(1) noise plus 16 tones (amplitude U(10,50), rate U(-4,4))."""
+    rng = np.random.default_rng(seed)
+    s = rng.exponential(1.0, size=(Nt, Nf)).astype(np.float32)  # row t = time step
+    trng = np.random.default_rng(seed + 7919)
+    c0 = trng.uniform(0, Nf, 16)
+    rate = trng.uniform(-4, 4, 16)
+    amp = trng.uniform(10, 50, 16)
+    for c, r, a in zip(c0, rate, amp):
+        for t in range(Nt):
+            s[t, int(round(c + r * t)) % Nf] += np.float32(a)
+    return s  # (Nt, Nf) row-major == (Nf, Nt) column-major
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/traffic_<kernel>_cfg2.txt), or None."""
+    p = ROOT / "profiles" / f"traffic_{kernel}_cfg2.txt"
+    if not p.exists():
+        return None
+    vals = {}
+    for line in p.read_text().splitlines():
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    try:
+        return (vals["FETCH_KIB_PER_DISPATCH"] + vals["WRITE_KIB_PER_DISPATCH"]) * 1024.0 * vals.get("COPY_SCALE", 1.0)
+    except KeyError:
+        return None
+
+
+def cpu_baseline(seed: int = 1234) -> dict:
+    """One batch of the FP32 CPU restatement of the reference, 1 thread."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    sys.path.insert(0, str(ROOT))
+    from oracle import zdt_oracle as Z  # checker / CPU baseline only
+    s = np.asfortranarray(synthetic_spectrogram(NF, NT, seed).T)
+    ws = Z.RefWorkspace32(s, R0, DR, NR)
+    t0 = time.perf_counter()
+    ws.input(s)
+    ws.zdtfdr()
+    dt = time.perf_counter() - t0
+    return {"value": NF * NR / dt, "unit": "FDR cells/s", "cores": 1, "kind": "port",
+            "sample": f"one zdtfdr! batch (input!+preprocess!+convolve!+postprocess!+output!) at "
+                      f"2^20x16x129 by the numpy/scipy FP32 restatement, {dt:.2f} s"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--generic", action="store_true", help="force the unfused kernels")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every GPU box in this setup has one; "
+              "the path shards by spectrogram and drift-rate batch (see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import fdrt_hip as F
+    from fdrt_hip import _lib
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    spec_rows = torch.from_numpy(synthetic_spectrogram(NF, NT)).to(dev)
+    spec = spec_rows.t()  # (Nf, Nt) column-major view
+    flags = _lib.ZDT_FLAG_PROFILE | (_lib.ZDT_FLAG_GENERIC if a.generic else 0)
+    ws = F.ZDTWorkspace(spec, R0, DR, NR, generic=a.generic)
+    # a second handle with the profiling hook on the same stream
+    L = _lib.lib()
+    h = C.c_void_p()
+    arr, n = _lib.int_array((2, 3, 5))
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(L.zdt_create(spec.data_ptr(), NF, NT, R0, DR, NR, arr, n, flags, stream, C.byref(h)))
+    dest = F.create_fdr(spec, NR)
+
+    def step():
+        _lib.check(L.zdt_input(h, spec.data_ptr()))
+        _lib.check(L.zdt_run(h, R0, _lib.ZDT_WINDOW_RECT, None, dest.data_ptr()))
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(L.zdt_profile_reset(h))
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+
+    cnt = C.c_int()
+    _lib.check(L.zdt_profile_read(h, -1, C.byref(cnt), None, 0, None, None, None))
+    kern = []
+    for i in range(cnt.value):
+        name = C.create_string_buffer(64)
+        ms = C.c_double()
+        nl = C.c_int64()
+        by = C.c_double()
+        _lib.check(L.zdt_profile_read(h, i, None, name, 64, C.byref(ms), C.byref(nl), C.byref(by)))
+        kern.append((name.value.decode(), ms.value, nl.value, by.value))
+    kern.sort(key=lambda x: -x[1])
+    dname, dms, dn, dbytes = kern[0]
+    avg_s = dms / dn / 1e3
+    achieved = dbytes / avg_s / 1e9
+    L.zdt_destroy(h)
+
+    # sanity: the result is the drift-line sum (column sums equal the total power)
+    colsum = dest.double().sum(dim=0)
+    total = spec.double().sum()
+    if not torch.allclose(colsum, total.expand_as(colsum), rtol=1e-5):
+        print("bench.py: result check failed", file=sys.stderr)
+        return 1
+
+    out = {
+        "metric": "FDR cells/s (Nf x Nrates) at 2^20ch x 16t x 129 rates, FP32 ZDT",
+        "value": NF * NR / dt,
+        "unit": "FDR cells/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Exponential(1) noise + 16 drifting tones, seed 1234)",
+        "config": {"workload": "zdtfdr! input+1 batch, Nf=2^20, Nt=16, rates -4:1/16:4 (Nr=129, Nl=144)",
+                   "Nf": NF, "Nt": NT, "Nr": NR, "r0": R0, "dr": DR,
+                   "path": "generic" if a.generic else "fused"},
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dname),
+                     "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": dbytes},
+        "kernels": [{"name": k, "ms_per_step": ms / a.steps, "launches_per_step": n_ / a.steps,
+                     "alg_GBps": (by / (ms / n_ / 1e3) / 1e9) if ms > 0 else None} for k, ms, n_, by in kern],
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,236 @@
+// FDR statistics, normalization and proto-hit extraction (src/fdrutils.jl).
+//
+// fdrstats (:39-50) needs the mean of column 1 and the minimum over columns
+// of the corrected standard deviation: colstats produces per-column shifted
+// FP64 sums S1 = sum(x - c), S2 = sum((x - c)^2) with c = the column's first
+// element (the shift keeps S2 - S1^2/N free of cancellation), reduced in a
+// fixed order so the threshold is bit-reproducible run to run.
+//
+// findprotohits (:148-169) is findall(>=(thr)) in column-major order.  The
+// matrix is cut into tiles of HIT_TILE consecutive linear elements; a count
+// pass writes the number of hits per tile, one block scans the counts, and the
+// write pass re-reads only the tiles that hold hits, compacting them in order
+// with a block-level scan.  Hits are sparse at useful thresholds, so the data
+// is read about once.
+#include <hip/hip_runtime.h>
+
+#include "zdt_kernels.h"
+
+namespace zdt {
+
+constexpr int STAT_THREADS = 256;
+constexpr int STAT_ROWS = 16384;  // rows per partial-sum block
+
+__global__ __launch_bounds__(STAT_THREADS) void colstats_partial(const float* __restrict__ fdr, int64_t Nf,
+                                                                 int64_t ld, double* __restrict__ part,
+                                                                 int nchunk) {
+  const int chunk = blockIdx.x;
+  const int64_t j = blockIdx.y;
+  const float* col = fdr + j * ld;
+  const double c = static_cast<double>(col[0]);
+  const int64_t r0 = static_cast<int64_t>(chunk) * STAT_ROWS;
+  int64_t r1 = r0 + STAT_ROWS;
+  if (r1 > Nf) r1 = Nf;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += STAT_THREADS) {
+    const double d = static_cast<double>(col[i]) - c;
+    s1 += d;
+    s2 += d * d;
+  }
+  __shared__ double sh1[STAT_THREADS], sh2[STAT_THREADS];
+  sh1[threadIdx.x] = s1;
+  sh2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = STAT_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sh1[threadIdx.x] += sh1[threadIdx.x + w];
+      sh2[threadIdx.x] += sh2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[(j * nchunk + chunk) * 2 + 0] = sh1[0];
+    part[(j * nchunk + chunk) * 2 + 1] = sh2[0];
+  }
+}
+
+// per column: mean and corrected variance from the partial sums (fixed order)
+__global__ __launch_bounds__(256) void colstats_final(const float* __restrict__ fdr, int64_t Nf, int64_t Nr,
+                                                      int64_t ld, const double* __restrict__ part, int nchunk,
+                                                      double* __restrict__ colmean,
+                                                      double* __restrict__ colvar) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= Nr) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    s1 += part[(j * nchunk + c) * 2 + 0];
+    s2 += part[(j * nchunk + c) * 2 + 1];
+  }
+  const double n = static_cast<double>(Nf);
+  const double shift = static_cast<double>(fdr[j * ld]);
+  colmean[j] = shift + s1 / n;
+  colvar[j] = Nf > 1 ? (s2 - s1 * s1 / n) / (n - 1.0) : 0.0 / 0.0;
+}
+
+int colstats_nchunk(int64_t Nf) { return static_cast<int>((Nf + STAT_ROWS - 1) / STAT_ROWS); }
+
+int launch_colstats2(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double* part, double* colmean,
+                     double* colvar, hipStream_t st) {
+  const int nchunk = colstats_nchunk(Nf);
+  dim3 grid(nchunk, static_cast<unsigned>(Nr));
+  colstats_partial<<<grid, STAT_THREADS, 0, st>>>(fdr, Nf, ld, part, nchunk);
+  ZDT_CHECK_LAUNCH("colstats_partial");
+  colstats_final<<<static_cast<unsigned>((Nr + 255) / 256), 256, 0, st>>>(fdr, Nf, Nr, ld, part, nchunk,
+                                                                          colmean, colvar);
+  ZDT_CHECK_LAUNCH("colstats_final");
+  return ZDT_OK;
+}
+
+// ---------------------------------------------------------------- normalize
+
+__global__ __launch_bounds__(256) void normalize_kernel(float* __restrict__ fdr, int64_t Nf, int64_t Nr,
+                                                        int64_t ld, double m, double inv_s) {
+  const int64_t total = Nf * Nr;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = g % Nf, j = g / Nf;
+    float* p = fdr + i + j * ld;
+    *p = static_cast<float>((static_cast<double>(*p) - m) * inv_s);
+  }
+}
+
+int launch_normalize(float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double m, double s, hipStream_t st) {
+  int64_t g = (Nf * Nr + 255) / 256;
+  if (g > (1 << 20)) g = 1 << 20;
+  if (g < 1) g = 1;
+  normalize_kernel<<<static_cast<unsigned>(g), 256, 0, st>>>(fdr, Nf, Nr, ld, m, 1.0 / s);
+  ZDT_CHECK_LAUNCH("normalize_kernel");
+  return ZDT_OK;
+}
+
+// ---------------------------------------------------------------- proto-hits
+
+constexpr int HIT_THREADS = 256;
+constexpr int HIT_PER_THREAD = 16;
+constexpr int64_t HIT_TILE = HIT_THREADS * HIT_PER_THREAD;
+
+int64_t hits_tiles(int64_t Nf, int64_t Nr) { return (Nf * Nr + HIT_TILE - 1) / HIT_TILE; }
+
+__device__ __forceinline__ float load_lin(const float* fdr, int64_t Nf, int64_t ld, int64_t g) {
+  if (ld == Nf) return fdr[g];
+  return fdr[(g % Nf) + (g / Nf) * ld];
+}
+
+__global__ __launch_bounds__(HIT_THREADS) void hits_count(const float* __restrict__ fdr, int64_t Nf, int64_t n,
+                                                          int64_t ld, double thr, int64_t* __restrict__ counts) {
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * HIT_TILE;
+  int c = 0;
+#pragma unroll 4
+  for (int i = 0; i < HIT_PER_THREAD; ++i) {
+    const int64_t g = base + static_cast<int64_t>(i) * HIT_THREADS + threadIdx.x;
+    if (g < n && static_cast<double>(load_lin(fdr, Nf, ld, g)) >= thr) ++c;
+  }
+  __shared__ int sh[HIT_THREADS];
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = HIT_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[tile] = sh[0];
+}
+
+// exclusive scan of counts[0..n) in place, total in *total (one block)
+__global__ __launch_bounds__(1024) void hits_scan(int64_t* __restrict__ counts, int64_t n,
+                                                  int64_t* __restrict__ total) {
+  __shared__ int64_t sh[1024];
+  int64_t carry = 0;
+  for (int64_t base = 0; base < n; base += 1024) {
+    const int64_t g = base + threadIdx.x;
+    const int64_t v = g < n ? counts[g] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int64_t add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int64_t incl = sh[threadIdx.x];
+    if (g < n) counts[g] = carry + incl - v;
+    const int64_t blocksum = sh[1023];
+    __syncthreads();
+    carry += blocksum;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restrict__ fdr, int64_t Nf, int64_t n,
+                                                          int64_t ld, double thr,
+                                                          const int64_t* __restrict__ offsets, int64_t ntiles,
+                                                          int64_t total, int64_t base_index,
+                                                          int64_t* __restrict__ hits, int64_t cap) {
+  const int64_t tile = blockIdx.x;
+  const int64_t start = offsets[tile];
+  const int64_t end = (tile + 1 < ntiles) ? offsets[tile + 1] : total;
+  if (end == start || start >= cap) return;  // uniform: no hits here (or out of room)
+  // thread handles HIT_PER_THREAD consecutive elements, so a block scan of
+  // the per-thread counts gives each hit its ordered slot
+  const int64_t base = tile * HIT_TILE + static_cast<int64_t>(threadIdx.x) * HIT_PER_THREAD;
+  unsigned mask = 0;
+#pragma unroll
+  for (int i = 0; i < HIT_PER_THREAD; ++i) {
+    const int64_t g = base + i;
+    if (g < n && static_cast<double>(load_lin(fdr, Nf, ld, g)) >= thr) mask |= 1u << i;
+  }
+  __shared__ int sh[HIT_THREADS];
+  const int c = __popc(mask);
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 1; off < HIT_THREADS; off <<= 1) {
+    int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int64_t pos = start + sh[threadIdx.x] - c;
+  while (mask) {
+    const int i = __ffs(mask) - 1;
+    mask &= mask - 1;
+    if (pos < cap) hits[pos] = base_index + base + i;
+    ++pos;
+  }
+}
+
+int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, int64_t* counts,
+                      hipStream_t st) {
+  const int64_t nt = hits_tiles(Nf, Nr);
+  if (nt == 0) return ZDT_OK;
+  hits_count<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, counts);
+  ZDT_CHECK_LAUNCH("hits_count");
+  return ZDT_OK;
+}
+
+int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, hipStream_t st) {
+  hits_scan<<<1, 1024, 0, st>>>(counts, n, total);
+  ZDT_CHECK_LAUNCH("hits_scan");
+  return ZDT_OK;
+}
+
+int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr,
+                           const int64_t* offsets, int64_t ntiles_all, int64_t tile0, const int64_t* total_dev,
+                           int64_t total, int64_t base_index, int64_t* hits, int64_t cap, hipStream_t st) {
+  (void)total_dev;
+  const int64_t nt = hits_tiles(Nf, Nr);
+  if (nt == 0) return ZDT_OK;
+  // offsets of this matrix's tiles start at tile0 of the global scan; the end
+  // of its last tile is the next matrix's first offset (or the total)
+  hits_write<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, offsets + tile0,
+                                                                ntiles_all - tile0, total, base_index, hits,
+                                                                cap);
+  ZDT_CHECK_LAUNCH("hits_write");
+  return ZDT_OK;
+}
+
+}  // namespace zdt

@@ -1,0 +1,624 @@
+// Generic batched mixed-radix FFT for gfx950 (FP32 and FP64).
+//
+// This engine serves every transform whose size or layout the fused ZDT path
+// does not cover: the step functions' length-Nl FFT along dim 2 of the Nk x Nl
+// work matrix (src/zdtfdr.jl:122-123,186,250-252), the chirp-kernel spectrum V
+// (computed in FP64, src/zdtfdr.jl:173-187), and the frequency-axis r2c/c2r of
+// the generic path (src/zdtfdr.jl:119-120,211,330) for any Nf whose prime
+// factors are in {2,3,5,7,11,13}.
+//
+// Algorithm: Stockham auto-sort, one kernel per radix pass, in the
+// formulation of Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors" (SC'08): pass with radix R over the
+// current sub-transform length Ns reads x[j + r*N/R], applies the twiddles
+// W_{Ns*R}^{r*(j mod Ns)}, does a length-R DFT and writes
+// y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].  Twiddles come from a per-N FP64-derived
+// table W_N^m.  The inverse is conj(forward(conj(x))).  The thread order is
+// chosen so that consecutive lanes touch consecutive addresses: along j when
+// the transform axis is contiguous, along the batch otherwise.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "fft_generic.h"
+
+namespace zdt {
+
+// ---------------------------------------------------------------- butterflies
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mul_neg_i(cplx<T> a) {  // a * (-i)
+  return {a.y, -a.x};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft2(cplx<T>& a, cplx<T>& b) {
+  cplx<T> t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cplx<T>* v) {
+  cplx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  cplx<T> t2 = cadd(v[1], v[3]), t3 = mul_neg_i(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+
+// length-2H DFT from two length-H DFTs (even/odd split) with W_{2H}^k
+template <typename T, int H>
+__device__ __forceinline__ void combine_even_odd(const cplx<T>* e, const cplx<T>* o, cplx<T>* out,
+                                                 const double* cw, const double* sw) {
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    cplx<T> w = {static_cast<T>(cw[k]), static_cast<T>(-sw[k])};
+    cplx<T> ow = cmul(o[k], w);
+    out[k] = cadd(e[k], ow);
+    out[k + H] = csub(e[k], ow);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(cplx<T>* v) {
+  constexpr double c8[4] = {1, 0.70710678118654757, 0, -0.70710678118654757};
+  constexpr double s8[4] = {0, 0.70710678118654757, 1, 0.70710678118654757};
+  cplx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft4(e);
+  dft4(o);
+  combine_even_odd<T, 4>(e, o, v, c8, s8);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft16(cplx<T>* v) {
+  constexpr double c16[8] = {1, 0.92387953251128674, 0.70710678118654757, 0.38268343236508978,
+                             0, -0.38268343236508978, -0.70710678118654757, -0.92387953251128674};
+  constexpr double s16[8] = {0, 0.38268343236508978, 0.70710678118654757, 0.92387953251128674,
+                             1, 0.92387953251128674, 0.70710678118654757, 0.38268343236508978};
+  cplx<T> e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    e[i] = v[2 * i];
+    o[i] = v[2 * i + 1];
+  }
+  dft8(e);
+  dft8(o);
+  combine_even_odd<T, 8>(e, o, v, c16, s16);
+}
+
+template <int R>
+struct OddTab;
+template <>
+struct OddTab<3> {
+  static constexpr double c[3] = {1, -0.5, -0.5};
+  static constexpr double s[3] = {0, 0.86602540378443865, -0.86602540378443865};
+};
+template <>
+struct OddTab<5> {
+  static constexpr double c[5] = {1, 0.30901699437494745, -0.80901699437494734, -0.80901699437494756,
+                                  0.30901699437494723};
+  static constexpr double s[5] = {0, 0.95105651629515353, 0.58778525229247325, -0.58778525229247303,
+                                  -0.95105651629515364};
+};
+template <>
+struct OddTab<7> {
+  static constexpr double c[7] = {1, 0.62348980185873359, -0.22252093395631434, -0.90096886790241903,
+                                  -0.90096886790241915, -0.22252093395631459, 0.62348980185873337};
+  static constexpr double s[7] = {0, 0.7818314824680298, 0.97492791218182362, 0.43388373911755823,
+                                  -0.43388373911755801, -0.97492791218182362, -0.78183148246802991};
+};
+template <>
+struct OddTab<11> {
+  static constexpr double c[11] = {1, 0.84125353283118121, 0.41541501300188644, -0.142314838273285,
+                                   -0.65486073394528499, -0.95949297361449737, -0.95949297361449748,
+                                   -0.65486073394528521, -0.14231483827328523, 0.41541501300188605,
+                                   0.84125353283118121};
+  static constexpr double s[11] = {0, 0.54064081745559756, 0.90963199535451833, 0.9898214418809328,
+                                   0.75574957435425827, 0.28173255684142967, -0.28173255684142939,
+                                   -0.75574957435425816, -0.98982144188093268, -0.90963199535451855,
+                                   -0.54064081745559744};
+};
+template <>
+struct OddTab<13> {
+  static constexpr double c[13] = {1, 0.88545602565320991, 0.56806474673115592, 0.12053668025532301,
+                                   -0.35460488704253545, -0.74851074817110119, -0.97094181742605201,
+                                   -0.97094181742605212, -0.7485107481711013, -0.3546048870425359,
+                                   0.1205366802553232, 0.56806474673115481, 0.88545602565321002};
+  static constexpr double s[13] = {0, 0.46472317204376851, 0.82298386589365635, 0.99270887409805397,
+                                   0.93501624268541483, 0.66312265824079519, 0.23931566428755768,
+                                   -0.23931566428755743, -0.66312265824079497, -0.93501624268541472,
+                                   -0.99270887409805397, -0.82298386589365702, -0.4647231720437684};
+};
+
+// odd-length DFT in the symmetric form: for k in 1..(R-1)/2
+//   a_k = x0 + sum_m cos(2 pi m k/R) (x_m + x_{R-m}),  b_k = sum_m sin(...) (x_m - x_{R-m})
+//   y_k = a_k - i b_k,  y_{R-k} = a_k + i b_k
+template <typename T, int R>
+__device__ __forceinline__ void dft_odd(cplx<T>* v) {
+  constexpr int H = (R - 1) / 2;
+  cplx<T> p[H], m[H];
+  cplx<T> y0 = v[0];
+#pragma unroll
+  for (int i = 1; i <= H; ++i) {
+    p[i - 1] = cadd(v[i], v[R - i]);
+    m[i - 1] = csub(v[i], v[R - i]);
+    y0 = cadd(y0, p[i - 1]);
+  }
+  cplx<T> out[R];
+  out[0] = y0;
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    cplx<T> a = v[0], b = {0, 0};
+#pragma unroll
+    for (int i = 1; i <= H; ++i) {
+      const int idx = (i * k) % R;
+      const T c = static_cast<T>(OddTab<R>::c[idx]);
+      const T s = static_cast<T>(OddTab<R>::s[idx]);
+      a = {a.x + c * p[i - 1].x, a.y + c * p[i - 1].y};
+      b = {b.x + s * m[i - 1].x, b.y + s * m[i - 1].y};
+    }
+    // -i*b = (b.y, -b.x)
+    out[k] = {a.x + b.y, a.y - b.x};
+    out[R - k] = {a.x - b.y, a.y + b.x};
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) v[i] = out[i];
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft(cplx<T>* v) {
+  if constexpr (R == 2) {
+    dft2(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4(v);
+  } else if constexpr (R == 8) {
+    dft8(v);
+  } else if constexpr (R == 16) {
+    dft16(v);
+  } else {
+    dft_odd<T, R>(v);
+  }
+}
+
+// ---------------------------------------------------------------- pass kernel
+
+struct PassArgs {
+  int64_t N, NR, Ns, twstep;  // twstep = N/(Ns*R)
+  int64_t ies, ibs, oes, obs, batch;
+  double scale;
+  int inverse, jmajor;
+};
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void stockham_pass(const cplx<T>* __restrict__ in,
+                                                     cplx<T>* __restrict__ out,
+                                                     const cplx<T>* __restrict__ tw, PassArgs a) {
+  const int64_t total = a.NR * a.batch;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int64_t j, b;
+    if (a.jmajor) {
+      j = g % a.NR;
+      b = g / a.NR;
+    } else {
+      b = g % a.batch;
+      j = g / a.batch;
+    }
+    const int64_t k = j % a.Ns;
+    cplx<T> v[R];
+    const cplx<T>* src = in + b * a.ibs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      v[r] = src[(j + r * a.NR) * a.ies];
+      if (a.inverse) v[r].y = -v[r].y;
+    }
+    if (a.Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * k * a.twstep]);
+    }
+    dft<T, R>(v);
+    const T sc = static_cast<T>(a.scale);
+    const int64_t idxD = (j / a.Ns) * a.Ns * R + k;
+    cplx<T>* dst = out + b * a.obs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cplx<T> y = v[r];
+      if (a.inverse) y.y = -y.y;
+      if (a.scale != 1.0) y = {y.x * sc, y.y * sc};
+      dst[(idxD + r * a.Ns) * a.oes] = y;
+    }
+  }
+}
+
+static unsigned grid_for(int64_t total, int threads) {
+  int64_t g = (total + threads - 1) / threads;
+  if (g > (1 << 20)) g = 1 << 20;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+template <typename T>
+static int launch_pass(int R, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, const PassArgs& a,
+                       hipStream_t st) {
+  const unsigned grid = grid_for(a.NR * a.batch, 256);
+  static const char* names[17] = {nullptr, nullptr, "fft_pass_r2", "fft_pass_r3", "fft_pass_r4", "fft_pass_r5",
+                                  nullptr, "fft_pass_r7", "fft_pass_r8", nullptr, nullptr, "fft_pass_r11",
+                                  nullptr, "fft_pass_r13", nullptr, nullptr, "fft_pass_r16"};
+  ProfScope ps(st, (R >= 2 && R <= 16 && names[R]) ? names[R] : "fft_pass",
+               2.0 * static_cast<double>(a.N * a.batch) * sizeof(cplx<T>));
+#define ZDT_PASS_CASE(RR)                                                  \
+  case RR:                                                                 \
+    stockham_pass<T, RR><<<grid, 256, 0, st>>>(in, out, tw, a);            \
+    break;
+  switch (R) {
+    ZDT_PASS_CASE(2)
+    ZDT_PASS_CASE(3)
+    ZDT_PASS_CASE(4)
+    ZDT_PASS_CASE(5)
+    ZDT_PASS_CASE(7)
+    ZDT_PASS_CASE(8)
+    ZDT_PASS_CASE(11)
+    ZDT_PASS_CASE(13)
+    ZDT_PASS_CASE(16)
+    default:
+      return fail(ZDT_ENOTSUP, "internal: radix %d not implemented", R);
+  }
+#undef ZDT_PASS_CASE
+  ZDT_CHECK_LAUNCH("stockham_pass");
+  return ZDT_OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_copy(const cplx<T>* __restrict__ in, int64_t ibs,
+                                                  cplx<T>* __restrict__ out, int64_t obs, int64_t batch,
+                                                  T scale) {
+  for (int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < batch;
+       b += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const cplx<T> v = in[b * ibs];
+    out[b * obs] = {v.x * scale, v.y * scale};
+  }
+}
+
+// ---------------------------------------------------------------- plans
+
+bool fft_factorize(int64_t N, std::vector<int>* radices) {
+  radices->clear();
+  if (N < 1) return false;
+  int64_t n = N;
+  while (n % 16 == 0) {
+    radices->push_back(16);
+    n /= 16;
+  }
+  for (int r : {8, 4, 2, 3, 5, 7, 11, 13}) {
+    while (n % r == 0) {
+      radices->push_back(r);
+      n /= r;
+    }
+  }
+  return n == 1;
+}
+
+namespace {
+struct PlanKey {
+  int device;
+  int64_t N;
+  bool dbl;
+  bool operator<(const PlanKey& o) const {
+    return std::tie(device, N, dbl) < std::tie(o.device, o.N, o.dbl);
+  }
+};
+std::mutex g_plan_mu;
+std::map<PlanKey, FftPlan*> g_plans;
+}  // namespace
+
+int fft_plan_get(int64_t N, bool dbl, const FftPlan** out) {
+  std::vector<int> radices;
+  if (!fft_factorize(N, &radices))
+    return fail(ZDT_ENOTSUP, "FFT length %lld has a prime factor outside {2,3,5,7,11,13}",
+                static_cast<long long>(N));
+  int dev = 0;
+  ZDT_TRY_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  PlanKey key{dev, N, dbl};
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) {
+    *out = it->second;
+    return ZDT_OK;
+  }
+  FftPlan* p = new FftPlan();
+  p->N = N;
+  p->dbl = dbl;
+  p->radices = radices;
+  // twiddle table W_N^m = exp(-2 pi i m / N), m < N, from an exact reduction
+  const size_t esz = dbl ? sizeof(c64) : sizeof(c32);
+  std::vector<unsigned char> host(static_cast<size_t>(N) * esz);
+  for (int64_t m = 0; m < N; ++m) {
+    // angle = 2 pi m/N; use the symmetric quarter to keep cos/sin exact at
+    // multiples of pi/2
+    const long double a = 2.0L * 3.14159265358979323846264338327950288L * static_cast<long double>(m) /
+                          static_cast<long double>(N);
+    double c = static_cast<double>(cosl(a));
+    double s = -static_cast<double>(sinl(a));
+    if (4 * m == N || 4 * m == 3 * N) c = 0.0;
+    if (2 * m == N) s = 0.0;
+    if (dbl) {
+      reinterpret_cast<c64*>(host.data())[m] = {c, s};
+    } else {
+      reinterpret_cast<c32*>(host.data())[m] = {static_cast<float>(c), static_cast<float>(s)};
+    }
+  }
+  hipError_t e = hipMalloc(&p->tw, host.size());
+  if (e != hipSuccess) {
+    delete p;
+    return fail(ZDT_ENOMEM, "hipMalloc of FFT twiddles (%zu bytes) failed", host.size());
+  }
+  e = hipMemcpy(p->tw, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p->tw);
+    delete p;
+    return hip_fail(e, "hipMemcpy(twiddles)");
+  }
+  g_plans[key] = p;
+  *out = p;
+  return ZDT_OK;
+}
+
+template <typename T>
+static int c2c_impl(const FftPlan* p, const void* in, int64_t ies, int64_t ibs, void* out, int64_t oes,
+                    int64_t obs, int64_t batch, bool inverse, double scale, void* s0, void* s1,
+                    hipStream_t st) {
+  const int np = static_cast<int>(p->radices.size());
+  const int64_t N = p->N;
+  const cplx<T>* tw = static_cast<const cplx<T>*>(p->tw);
+  if (N == 1) {
+    // the length-1 DFT is the identity (times scale)
+    scale_copy<T><<<grid_for(batch, 256), 256, 0, st>>>(static_cast<const cplx<T>*>(in), ibs,
+                                                        static_cast<cplx<T>*>(out), obs, batch,
+                                                        static_cast<T>(scale));
+    ZDT_CHECK_LAUNCH("scale_copy");
+    return ZDT_OK;
+  }
+  int64_t Ns = 1;
+  const void* src = in;
+  int64_t ses = ies, sbs = ibs;
+  for (int pi = 0; pi < np; ++pi) {
+    const int R = p->radices[pi];
+    const bool last = (pi == np - 1);
+    void* dst = last ? out : ((pi & 1) ? s1 : s0);
+    const int64_t des = last ? oes : 1;
+    const int64_t dbs = last ? obs : N;
+    PassArgs a;
+    a.N = N;
+    a.NR = N / R;
+    a.Ns = Ns;
+    a.twstep = N / (Ns * R);
+    a.ies = ses;
+    a.ibs = sbs;
+    a.oes = des;
+    a.obs = dbs;
+    a.batch = batch;
+    a.scale = last ? scale : 1.0;
+    a.inverse = inverse ? 1 : 0;
+    // consecutive lanes along the contiguous index
+    a.jmajor = (ses == 1 || batch == 1) ? 1 : 0;
+    ZDT_TRY(launch_pass<T>(R, static_cast<const cplx<T>*>(src), static_cast<cplx<T>*>(dst), tw, a, st));
+    Ns *= R;
+    src = dst;
+    ses = des;
+    sbs = dbs;
+  }
+  return ZDT_OK;
+}
+
+int fft_c2c(const FftPlan* p, const void* in, int64_t ies, int64_t ibs, void* out, int64_t oes,
+            int64_t obs, int64_t batch, bool inverse, double scale, void* s0, void* s1, hipStream_t st) {
+  if (p->dbl)
+    return c2c_impl<double>(p, in, ies, ibs, out, oes, obs, batch, inverse, scale, s0, s1, st);
+  return c2c_impl<float>(p, in, ies, ibs, out, oes, obs, batch, inverse, scale, s0, s1, st);
+}
+
+// ---------------------------------------------------------------- real transforms
+
+// r2c post-processing for even N = 2M.  Z = FFT_M(x[2n] + i x[2n+1]).
+// X[k] = (Z[k] + conj Z[M-k])/2 - i W_N^k (Z[k] - conj Z[M-k])/2, k in [0, M].
+__global__ __launch_bounds__(256) void r2c_post(const c32* __restrict__ Z, c32* __restrict__ X,
+                                                const c32* __restrict__ twN, int64_t M, int64_t ldx,
+                                                int64_t batch) {
+  const int64_t half = M / 2 + 1;  // k in [0, M/2]
+  const int64_t total = half * batch;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t k = g % half, b = g / half;
+    const c32* z = Z + b * M;
+    c32* x = X + b * ldx;
+    if (k == 0) {
+      const c32 z0 = z[0];
+      x[0] = {z0.x + z0.y, 0.0f};
+      x[M] = {z0.x - z0.y, 0.0f};
+      continue;
+    }
+    const int64_t km = M - k;
+    const c32 a = z[k], bb = z[km];
+    {
+      // E = (a + conj bb)/2, O = (a - conj bb)/(2i)
+      const c32 E = {0.5f * (a.x + bb.x), 0.5f * (a.y - bb.y)};
+      const c32 O = {0.5f * (a.y + bb.y), -0.5f * (a.x - bb.x)};
+      x[k] = cadd(E, cmul(twN[k], O));
+    }
+    if (km != k) {
+      const c32 E = {0.5f * (bb.x + a.x), 0.5f * (bb.y - a.y)};
+      const c32 O = {0.5f * (bb.y + a.y), -0.5f * (bb.x - a.x)};
+      x[km] = cadd(E, cmul(twN[km], O));
+    }
+  }
+}
+
+// c2r pre-processing for even N = 2M.  The imaginary parts of X[0] and X[M]
+// are ignored (the c2r convention of FFTW/cuFFT).  Z'[k] = (E + i O) * scale
+// with E = (X[k] + conj X[M-k])/2, O = W_N^{-k} (X[k] - conj X[M-k])/2.
+__global__ __launch_bounds__(256) void c2r_pre(const c32* __restrict__ X, c32* __restrict__ Z,
+                                               const c32* __restrict__ twN, int64_t M, int64_t ldx,
+                                               int64_t batch, float scale) {
+  const int64_t half = M / 2 + 1;
+  const int64_t total = half * batch;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t k = g % half, b = g / half;
+    const c32* x = X + b * ldx;
+    c32* z = Z + b * M;
+    if (k == 0) {
+      const float a = x[0].x, c = x[M].x;
+      z[0] = {0.5f * (a + c) * scale, 0.5f * (a - c) * scale};
+      continue;
+    }
+    const int64_t km = M - k;
+    const c32 p = x[k], q = x[km];
+    {
+      const c32 E = {0.5f * (p.x + q.x), 0.5f * (p.y - q.y)};
+      const c32 D = {0.5f * (p.x - q.x), 0.5f * (p.y + q.y)};
+      const c32 O = cmulc(D, twN[k]);  // D * conj(W^k) = W^{-k} D
+      z[k] = {(E.x - O.y) * scale, (E.y + O.x) * scale};
+    }
+    if (km != k) {
+      const c32 E = {0.5f * (q.x + p.x), 0.5f * (q.y - p.y)};
+      const c32 D = {0.5f * (q.x - p.x), 0.5f * (q.y + p.y)};
+      const c32 O = cmulc(D, twN[km]);
+      z[km] = {(E.x - O.y) * scale, (E.y + O.x) * scale};
+    }
+  }
+}
+
+// odd N helpers
+__global__ __launch_bounds__(256) void real_to_complex(const float* __restrict__ in, int64_t ldi,
+                                                       c32* __restrict__ out, int64_t N, int64_t batch) {
+  const int64_t total = N * batch;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t n = g % N, b = g / N;
+    out[b * N + n] = {in[b * ldi + n], 0.0f};
+  }
+}
+
+__global__ __launch_bounds__(256) void copy_rows(const c32* __restrict__ in, int64_t ldi,
+                                                 c32* __restrict__ out, int64_t ldo, int64_t rows,
+                                                 int64_t batch) {
+  const int64_t total = rows * batch;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t n = g % rows, b = g / rows;
+    out[b * ldo + n] = in[b * ldi + n];
+  }
+}
+
+// full Hermitian spectrum of length N (odd) from Nk = (N+1)/2 bins; Im X[0] ignored
+__global__ __launch_bounds__(256) void hermitian_extend(const c32* __restrict__ X, int64_t ldx,
+                                                        c32* __restrict__ full, int64_t N,
+                                                        int64_t batch) {
+  const int64_t total = N * batch;
+  const int64_t Nk = N / 2 + 1;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t n = g % N, b = g / N;
+    c32 v;
+    if (n == 0) {
+      v = {X[b * ldx].x, 0.0f};
+    } else if (n < Nk) {
+      v = X[b * ldx + n];
+    } else {
+      v = cconj(X[b * ldx + (N - n)]);
+    }
+    full[b * N + n] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void complex_real_part(const c32* __restrict__ in, float* __restrict__ out,
+                                                         int64_t ldo, int64_t N, int64_t batch) {
+  const int64_t total = N * batch;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t n = g % N, b = g / N;
+    out[b * ldo + n] = in[b * N + n].x;
+  }
+}
+
+size_t fft_real_scratch_elems(int64_t N, int64_t batch) {
+  const int64_t n = (N % 2 == 0) ? N / 2 : N;
+  return static_cast<size_t>(n) * static_cast<size_t>(batch);
+}
+
+int fft_r2c(int64_t N, const float* in, int64_t ldi, c32* out, int64_t ldo, int64_t batch, void* s0,
+            void* s1, void* s2, hipStream_t st) {
+  if (N % 2 == 0) {
+    const int64_t M = N / 2;
+    if (ldi % 2 != 0) return fail(ZDT_EINVAL, "r2c: column stride must be even for even N");
+    const FftPlan *pM = nullptr, *pN = nullptr;
+    ZDT_TRY(fft_plan_get(N, false, &pN));
+    c32* Z = static_cast<c32*>(s2);
+    if (M == 1) {
+      // length-1 complex FFT of the packed pair is the pair itself
+      ZDT_TRY_HIP(hipMemcpy2DAsync(Z, sizeof(c32), in, ldi * sizeof(float), sizeof(c32), batch,
+                                   hipMemcpyDeviceToDevice, st));
+    } else {
+      ZDT_TRY(fft_plan_get(M, false, &pM));
+      ZDT_TRY(fft_c2c(pM, in, 1, ldi / 2, Z, 1, M, batch, false, 1.0, s0, s1, st));
+    }
+    ProfScope ps(st, "r2c_post", 16.0 * static_cast<double>(M * batch));
+    r2c_post<<<grid_for((M / 2 + 1) * batch, 256), 256, 0, st>>>(Z, out, static_cast<const c32*>(pN->tw),
+                                                                   M, ldo, batch);
+    ZDT_CHECK_LAUNCH("r2c_post");
+    return ZDT_OK;
+  }
+  const FftPlan* p = nullptr;
+  ZDT_TRY(fft_plan_get(N, false, &p));
+  c32* full = static_cast<c32*>(s2);
+  real_to_complex<<<grid_for(N * batch, 256), 256, 0, st>>>(in, ldi, full, N, batch);
+  ZDT_CHECK_LAUNCH("real_to_complex");
+  // N odd and > 1: FFT into s0/s1 chain, final result into s0 region is not
+  // allowed (scratch), so transform in place on `full`.
+  ZDT_TRY(fft_c2c(p, full, 1, N, full, 1, N, batch, false, 1.0, s0, s1, st));
+  copy_rows<<<grid_for((N / 2 + 1) * batch, 256), 256, 0, st>>>(full, N, out, ldo, N / 2 + 1, batch);
+  ZDT_CHECK_LAUNCH("copy_rows");
+  return ZDT_OK;
+}
+
+int fft_c2r(int64_t N, const c32* in, int64_t ldi, float* out, int64_t ldo, int64_t batch, double scale,
+            void* s0, void* s1, void* s2, hipStream_t st) {
+  if (N % 2 == 0) {
+    const int64_t M = N / 2;
+    if (ldo % 2 != 0) return fail(ZDT_EINVAL, "c2r: column stride must be even for even N");
+    const FftPlan *pM = nullptr, *pN = nullptr;
+    ZDT_TRY(fft_plan_get(N, false, &pN));
+    c32* Z = static_cast<c32*>(s2);
+    {
+      ProfScope ps(st, "c2r_pre", 16.0 * static_cast<double>(M * batch));
+      c2r_pre<<<grid_for((M / 2 + 1) * batch, 256), 256, 0, st>>>(
+          in, Z, static_cast<const c32*>(pN->tw), M, ldi, batch, static_cast<float>(scale / M));
+      ZDT_CHECK_LAUNCH("c2r_pre");
+    }
+    if (M == 1) {
+      ZDT_TRY_HIP(hipMemcpy2DAsync(out, ldo * sizeof(float), Z, sizeof(c32), sizeof(c32), batch,
+                                   hipMemcpyDeviceToDevice, st));
+    } else {
+      ZDT_TRY(fft_plan_get(M, false, &pM));
+      ZDT_TRY(fft_c2c(pM, Z, 1, M, out, 1, ldo / 2, batch, true, 1.0, s0, s1, st));
+    }
+    return ZDT_OK;
+  }
+  const FftPlan* p = nullptr;
+  ZDT_TRY(fft_plan_get(N, false, &p));
+  c32* full = static_cast<c32*>(s2);
+  hermitian_extend<<<grid_for(N * batch, 256), 256, 0, st>>>(in, ldi, full, N, batch);
+  ZDT_CHECK_LAUNCH("hermitian_extend");
+  ZDT_TRY(fft_c2c(p, full, 1, N, full, 1, N, batch, true, scale / static_cast<double>(N), s0, s1, st));
+  complex_real_part<<<grid_for(N * batch, 256), 256, 0, st>>>(full, out, ldo, N, batch);
+  ZDT_CHECK_LAUNCH("complex_real_part");
+  return ZDT_OK;
+}
+
+}  // namespace zdt

@@ -1,0 +1,22 @@
+// Fused ZDT path for power-of-two Nf (see zdt_fast.hip).
+#pragma once
+
+#include "zdt_internal.h"
+
+namespace zdt {
+
+struct FastPlan;
+
+bool fast_supported(int64_t Nf, int64_t Nt, int64_t Nr);
+int64_t fast_bytes(int64_t Nf, int64_t Nt, int64_t Nr);
+int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, FastPlan** out);
+void fast_destroy(FastPlan* p);
+// input!: spectrogram (device Nf x Nt) -> the plan's input-spectrum buffer
+int fast_input(FastPlan* p, const float* spec, hipStream_t st);
+// the input spectrum F in natural Nk x Nt layout (for the step functions)
+int fast_unpack_F(FastPlan* p, c32* F, hipStream_t st);
+// one batch; dest NULL keeps the result for fast_output
+int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest, double scale, hipStream_t st);
+int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st);
+
+}  // namespace zdt

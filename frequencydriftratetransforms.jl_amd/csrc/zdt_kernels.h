@@ -1,0 +1,50 @@
+// Launchers of the generic ZDT kernels, the step-function kernels and the
+// FDR statistics / proto-hit kernels.
+#pragma once
+
+#include "zdt_internal.h"
+
+namespace zdt {
+
+struct CztArgs {
+  const c32* F;
+  int64_t ldF;  // Nk x Nt
+  c32* Ys;
+  int64_t ldY;  // Nk x Nr
+  int64_t Nk, Nf;
+  int Nt, Nr;
+  double hd;  // dr/(2 Nf): chirp(k,l) = exp(-2 pi i k l^2 hd)
+  double h0;  // r0/Nf:     pre(k,t) = exp(2 pi i (k t^2 hd + k t h0))
+  int wkind;
+  const float* wcustom;
+  float scale;
+};
+
+int launch_czt_direct(const CztArgs& a, hipStream_t st);
+
+int launch_step_prephase(const c32* F, int64_t ldF, c32* Y, int64_t ldY, int64_t Nk, int Nt, int Nl,
+                         double hd, double h0, hipStream_t st);
+int launch_step_vgen(c64* V, int64_t Nk, int Nt, int Nr, int Nl, double hd, hipStream_t st);
+int launch_c64_to_c32(const c64* in, c32* out, int64_t n, hipStream_t st);
+int launch_step_vmul(c32* Y, const c32* V, int64_t n, hipStream_t st);
+int launch_step_postphase(c32* Y, int64_t ldY, int64_t Nk, int64_t Nf, int Nr, double hd, int wkind,
+                          const float* wcustom, float scale, hipStream_t st);
+
+// FDR statistics: per-column mean and corrected variance (FP64) of an
+// Nf x Nr matrix with column stride ld.  part needs 2*Nr*colstats_nchunk(Nf)
+// doubles of scratch; colmean/colvar receive Nr doubles each.
+int colstats_nchunk(int64_t Nf);
+int launch_colstats2(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double* part, double* colmean,
+                     double* colvar, hipStream_t st);
+// x = (x - m)/s in place
+int launch_normalize(float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double m, double s, hipStream_t st);
+// proto-hits: count per tile, exclusive scan, ordered write (see fdr_stats.hip)
+int64_t hits_tiles(int64_t Nf, int64_t Nr);
+int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, int64_t* counts,
+                      hipStream_t st);
+int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, hipStream_t st);
+int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr,
+                           const int64_t* offsets, int64_t ntiles_all, int64_t tile0, const int64_t* total_dev,
+                           int64_t total, int64_t base_index, int64_t* hits, int64_t cap, hipStream_t st);
+
+}  // namespace zdt

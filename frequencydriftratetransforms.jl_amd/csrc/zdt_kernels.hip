@@ -1,0 +1,267 @@
+// ZDT kernels of the generic path and the step functions.
+//
+// The chirp-Z transform per frequency bin k (src/zdtfdr.jl:196-321) is
+//   Ys[k,j] = post(k,j) w(k) sum_t F[k,t] pre(k,t) chirp(k, j-t),   j < Nr
+// with chirp(k,l) = exp(-i pi k l^2 dr/Nf)                       (:140-142)
+//      pre(k,t)   = exp( i pi k t (t dr + 2 r0)/Nf)              (:196-198)
+//      post(k,j)  = exp( i pi k j^2 dr/Nf) = conj(chirp(k,j))    (:268-270)
+// The reference evaluates the convolution over t as a length-Nl circular
+// convolution through FFTs (:249-253); the identity chirp(k,l) = chirp(k,-l)
+// makes the high lags of V (:182-184) the negative lags, so the result equals
+// the linear convolution above for any Nl >= Nt+Nr-1.  czt_direct evaluates
+// that convolution directly (Nt complex MACs per output) which for Nt <= 64
+// costs less than the two length-Nl FFTs, needs no V, and is exact up to
+// float rounding of the products.  The step functions (preprocess!,
+// convolve!, postprocess!) keep the reference's FFT data flow on the
+// materialized Nk x Nl matrix Y for API parity.
+#include <hip/hip_runtime.h>
+
+#include "zdt_kernels.h"
+
+namespace zdt {
+
+// exp(2 pi i rev) in double for any rev (reduced first)
+__device__ __forceinline__ c64 cis64(double rev) {
+  const double f = rev - __builtin_rint(rev);
+  double s, c;
+  sincospi(2.0 * f, &s, &c);
+  return {c, s};
+}
+
+// exp(2 pi i (n1*h1 + n2*h2)) in float, each product reduced mod 1 in double
+__device__ __forceinline__ c32 cis_sum2(double n1, double h1, double n2, double h2) {
+  double p1 = n1 * h1;
+  p1 -= __builtin_rint(p1);
+  double p2 = n2 * h2;
+  p2 -= __builtin_rint(p2);
+  double p = p1 + p2;
+  p -= __builtin_rint(p);
+  return cis_rev(static_cast<float>(p));
+}
+
+__device__ __forceinline__ float window_value(int wkind, const float* wcustom, int64_t k, int64_t Nf) {
+  if (wkind == ZDT_WINDOW_HAMMING) {
+    // 0.53836 + 0.46164 cospi(2k/Nf)  (src/zdtfdr.jl:303-305)
+    double p = static_cast<double>(k) / static_cast<double>(Nf);
+    p -= __builtin_rint(p);
+    const c32 c = cis_rev(static_cast<float>(p));
+    return 0.53836f + 0.46164f * c.x;
+  }
+  if (wkind == ZDT_WINDOW_CUSTOM) return wcustom[k];
+  return 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// generic fused CZT: F (Nk x Nt) -> Ys (Nk x Nr), one thread per bin k.
+//
+// Time steps are taken TC at a time (chunks processed last to first so that
+// the chunk starting at t=0 finishes each output); the chirp values at the
+// TC lags an output needs live in a register ring indexed at compile time.
+// chirp(k,l) for successive l comes from an FP64 recurrence
+//   chirp(l+1) = chirp(l) g(l),  g(l+1) = g(l) h,  h = exp(-4 pi i k dr/(2Nf))
+// started from exactly evaluated values, so the phases stay at FP64 accuracy
+// however large k*l^2*dr/Nf grows.
+// ---------------------------------------------------------------------------
+template <int TC>
+__global__ __launch_bounds__(256) void czt_direct(CztArgs a) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= a.Nk) return;
+  const double kd = static_cast<double>(k);
+  const float w = window_value(a.wkind, a.wcustom, k, a.Nf) * a.scale;
+  const c64 h = cis64(-2.0 * kd * a.hd);
+  const int nchunks = (a.Nt + TC - 1) / TC;
+  for (int c = nchunks - 1; c >= 0; --c) {
+    const int c0 = c * TC;
+    c32 y[TC];
+#pragma unroll
+    for (int u = 0; u < TC; ++u) {
+      const int t = c0 + u;
+      if (t < a.Nt) {
+        const double td = static_cast<double>(t);
+        const c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
+        y[u] = cmul(f, cis_sum2(kd * td * td, a.hd, kd * td, a.h0));
+      } else {
+        y[u] = {0.0f, 0.0f};
+      }
+    }
+    // ring slot of lag L is (L + c0) mod TC; fill lags -c0-TC+1 .. -c0-1
+    const double L0 = static_cast<double>(-c0 - TC + 1);
+    c64 ch = cis64(-(kd * L0 * L0) * a.hd);
+    c64 g = cis64(-(kd * (2.0 * L0 + 1.0)) * a.hd);
+    c32 ring[TC];
+#pragma unroll
+    for (int i = 0; i < TC - 1; ++i) {
+      ring[(i + 1) % TC] = {static_cast<float>(ch.x), static_cast<float>(ch.y)};
+      ch = cmul(ch, g);
+      g = cmul(g, h);
+    }
+    // now ch = chirp(k, -c0)
+    const bool first = (c == nchunks - 1);
+    const bool final = (c == 0);
+    for (int J = 0; J < a.Nr; J += TC) {
+#pragma unroll
+      for (int m = 0; m < TC; ++m) {
+        const int j = J + m;
+        if (j < a.Nr) {
+          ring[m] = {static_cast<float>(ch.x), static_cast<float>(ch.y)};  // lag j - c0
+          ch = cmul(ch, g);
+          g = cmul(g, h);
+          c32 acc = {0.0f, 0.0f};
+#pragma unroll
+          for (int u = 0; u < TC; ++u) acc = cfma(acc, y[u], ring[(m - u + TC) % TC]);
+          c32* dst = a.Ys + k + static_cast<int64_t>(j) * a.ldY;
+          if (!first) acc = cadd(acc, *dst);
+          if (final) {
+            // c0 == 0: ring[m] = chirp(k, j), post(k,j) = conj(chirp(k,j))
+            acc = cmulc(acc, ring[m]);
+            acc = {acc.x * w, acc.y * w};
+          }
+          *dst = acc;
+        }
+      }
+    }
+  }
+}
+
+int launch_czt_direct(const CztArgs& a, hipStream_t st) {
+  const unsigned grid = static_cast<unsigned>((a.Nk + 255) / 256);
+  const int nchunks = (a.Nt + 15) / 16;
+  ProfScope ps(st, "czt_direct",
+               8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr * (2.0 * nchunks - 1.0)));
+  if (a.Nt <= 2) {
+    czt_direct<2><<<grid, 256, 0, st>>>(a);
+  } else if (a.Nt <= 4) {
+    czt_direct<4><<<grid, 256, 0, st>>>(a);
+  } else if (a.Nt <= 8) {
+    czt_direct<8><<<grid, 256, 0, st>>>(a);
+  } else {
+    czt_direct<16><<<grid, 256, 0, st>>>(a);
+  }
+  ZDT_CHECK_LAUNCH("czt_direct");
+  return ZDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step functions (materialized Y, src/zdtfdr.jl:221-321)
+// ---------------------------------------------------------------------------
+
+// preprocess!: Y[:, :Nt] = F .* prephase, Y[:, Nt:] = 0   (src/zdtfdr.jl:221-235)
+__global__ __launch_bounds__(256) void step_prephase(const c32* __restrict__ F, int64_t ldF,
+                                                     c32* __restrict__ Y, int64_t ldY, int64_t Nk,
+                                                     int Nt, int Nl, double hd, double h0) {
+  const int64_t total = Nk * Nl;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t k = g % Nk;
+    const int t = static_cast<int>(g / Nk);
+    c32 v = {0.0f, 0.0f};
+    if (t < Nt) {
+      const double kd = static_cast<double>(k), td = static_cast<double>(t);
+      v = cmul(F[k + t * ldF], cis_sum2(kd * td * td, hd, kd * td, h0));
+    }
+    Y[k + t * ldY] = v;
+  }
+}
+
+// chirp kernel before its FFT, in FP64 (src/zdtfdr.jl:173-185):
+// V0[k,l] = chirp(k,l) for l < Nr, chirp(k, Nl-l) for l >= Nl-Nt+1, else 0
+__global__ __launch_bounds__(256) void step_vgen(c64* __restrict__ V, int64_t Nk, int Nt, int Nr, int Nl,
+                                                 double hd) {
+  const int64_t total = Nk * Nl;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t k = g % Nk;
+    const int l = static_cast<int>(g / Nk);
+    c64 v = {0.0, 0.0};
+    int lag = -1;
+    if (l < Nr) {
+      lag = l;
+    } else if (l >= Nl - Nt + 1) {
+      lag = Nl - l;
+    }
+    if (lag >= 0) {
+      const double kd = static_cast<double>(k), ld = static_cast<double>(lag);
+      v = cis64(-(kd * ld * ld) * hd);
+    }
+    V[g] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void c64_to_c32(const c64* __restrict__ in, c32* __restrict__ out,
+                                                  int64_t n) {
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < n;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    out[g] = {static_cast<float>(in[g].x), static_cast<float>(in[g].y)};
+  }
+}
+
+// Y .*= V  (src/zdtfdr.jl:251)
+__global__ __launch_bounds__(256) void step_vmul(c32* __restrict__ Y, const c32* __restrict__ V, int64_t n) {
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < n;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    Y[g] = cmul(Y[g], V[g]);
+  }
+}
+
+// postprocess!: Ys .*= postphase .* w  (src/zdtfdr.jl:297-321)
+__global__ __launch_bounds__(256) void step_postphase(c32* __restrict__ Y, int64_t ldY, int64_t Nk,
+                                                      int64_t Nf, int Nr, double hd, int wkind,
+                                                      const float* __restrict__ wcustom, float scale) {
+  const int64_t total = Nk * Nr;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t k = g % Nk;
+    const int j = static_cast<int>(g / Nk);
+    const double kd = static_cast<double>(k), jd = static_cast<double>(j);
+    double p = (kd * jd * jd) * hd;
+    p -= __builtin_rint(p);
+    const c32 post = cis_rev(static_cast<float>(p));
+    const float w = window_value(wkind, wcustom, k, Nf) * scale;
+    c32 v = cmul(Y[k + j * ldY], post);
+    Y[k + j * ldY] = {v.x * w, v.y * w};
+  }
+}
+
+static unsigned grid_for(int64_t total) {
+  int64_t g = (total + 255) / 256;
+  if (g > (1 << 20)) g = 1 << 20;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+int launch_step_prephase(const c32* F, int64_t ldF, c32* Y, int64_t ldY, int64_t Nk, int Nt, int Nl,
+                         double hd, double h0, hipStream_t st) {
+  ProfScope ps(st, "step_prephase", 8.0 * static_cast<double>(Nk) * (Nt + Nl));
+  step_prephase<<<grid_for(Nk * Nl), 256, 0, st>>>(F, ldF, Y, ldY, Nk, Nt, Nl, hd, h0);
+  ZDT_CHECK_LAUNCH("step_prephase");
+  return ZDT_OK;
+}
+
+int launch_step_vgen(c64* V, int64_t Nk, int Nt, int Nr, int Nl, double hd, hipStream_t st) {
+  step_vgen<<<grid_for(Nk * Nl), 256, 0, st>>>(V, Nk, Nt, Nr, Nl, hd);
+  ZDT_CHECK_LAUNCH("step_vgen");
+  return ZDT_OK;
+}
+
+int launch_c64_to_c32(const c64* in, c32* out, int64_t n, hipStream_t st) {
+  c64_to_c32<<<grid_for(n), 256, 0, st>>>(in, out, n);
+  ZDT_CHECK_LAUNCH("c64_to_c32");
+  return ZDT_OK;
+}
+
+int launch_step_vmul(c32* Y, const c32* V, int64_t n, hipStream_t st) {
+  ProfScope ps(st, "step_vmul", 24.0 * static_cast<double>(n));
+  step_vmul<<<grid_for(n), 256, 0, st>>>(Y, V, n);
+  ZDT_CHECK_LAUNCH("step_vmul");
+  return ZDT_OK;
+}
+
+int launch_step_postphase(c32* Y, int64_t ldY, int64_t Nk, int64_t Nf, int Nr, double hd, int wkind,
+                          const float* wcustom, float scale, hipStream_t st) {
+  ProfScope ps(st, "step_postphase", 16.0 * static_cast<double>(Nk * Nr));
+  step_postphase<<<grid_for(Nk * Nr), 256, 0, st>>>(Y, ldY, Nk, Nf, Nr, hd, wkind, wcustom, scale);
+  ZDT_CHECK_LAUNCH("step_postphase");
+  return ZDT_OK;
+}
+
+}  // namespace zdt

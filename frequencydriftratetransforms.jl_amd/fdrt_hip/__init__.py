@@ -1,0 +1,23 @@
+"""fdrt_hip: Python host mirror of FrequencyDriftRateTransforms.jl's ZDT path
+over libzdt_hip.so (MI355X / gfx950).
+
+Names follow the reference's exports (src/FrequencyDriftRateTransforms.jl:3-24)
+restricted to the ZDT path; a trailing underscore stands for Julia's ``!``.
+The same C-ABI is bound from Julia by ``julia/ZDTHip.jl``.
+"""
+import torch  # noqa: F401  (loads the HIP runtime libzdt_hip.so binds to)
+
+from ._lib import ZDTError, lib
+from .fdrutils import (create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_, fdrstats,
+                       findprotohits)
+from .zdt import (ZDTWorkspace, convolve_, fdrsynchronize, input_, output_, postprocess_,
+                  preprocess_, zdtfdr, zdtfdr_)
+from .zdtutils import (RateRange, batchrates, calcNl, driftrates, estimate_memory, growNr,
+                       workspace_bytes)
+
+__all__ = [
+    "ZDTWorkspace", "input_", "output_", "preprocess_", "convolve_", "postprocess_",
+    "zdtfdr", "zdtfdr_", "calcNl", "growNr", "estimate_memory", "driftrates", "batchrates",
+    "RateRange", "workspace_bytes", "create_fdr", "fdrstats", "fdrsynchronize",
+    "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "ZDTError", "lib",
+]

@@ -1,0 +1,191 @@
+/*
+ * zdt.h -- C-ABI of libzdt_hip.so, the MI355X (gfx950) implementation of the
+ * Chirp-Z De-Doppler Transform (ZDT) path of FrequencyDriftRateTransforms.jl.
+ *
+ * Each entry point replaces one method of the reference package; the citation
+ * next to it names the reference function (file:line, relative to the
+ * reference repository root).  All matrices are column-major, frequency
+ * fastest, exactly like the Julia arrays: a spectrogram is Nf x Nt Float32,
+ * an FDR matrix is Nf x Nr Float32.  Pointers named spec/dest/fdr/window/
+ * hits are DEVICE pointers owned by the caller; the workspace owns every
+ * other device buffer.  Every call returns ZDT_OK (0) or an error code, and
+ * zdt_last_error() returns the message of the calling thread's last error
+ * (the Julia host throws ErrorException with it).
+ *
+ * Output scale: the FDR is normalized (inverse transforms carry 1/N), i.e.
+ * each element is the Fourier-interpolated drift-line sum; this is what
+ * src/zdtfdr.jl:120,123,330 computes with AbstractFFTs' scaled plans and what
+ * ext/CUDAFrequencyDriftRateTransformsExt.jl:101 reproduces.  Passing
+ * ZDT_FLAG_UNNORMALIZED returns Nf*Nl times that (the convention of
+ * test/runtests.jl:31).
+ *
+ * Threading: a workspace is bound to one HIP stream and is not re-entrant
+ * (ext/CUDAFrequencyDriftRateTransformsExt.jl:42-45); distinct workspaces may
+ * be used concurrently from different threads.
+ */
+#ifndef ZDT_H
+#define ZDT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ZDT_OK 0
+#define ZDT_EINVAL 1   /* bad argument (shape, pointer, window kind, ...)   */
+#define ZDT_ENOMEM 2   /* device allocation failed                          */
+#define ZDT_EHIP 3     /* a HIP runtime call failed                         */
+#define ZDT_ENOTSUP 4  /* size or factor the library does not implement     */
+#define ZDT_ETRUNC 5   /* more proto-hits than the caller's capacity        */
+#define ZDT_ESTATE 6   /* call order violated (e.g. output before a run)    */
+
+/* window kinds for postprocess (src/zdtfdr.jl:297-321) */
+#define ZDT_WINDOW_RECT 0    /* (n,N)->1, the default                       */
+#define ZDT_WINDOW_HAMMING 1 /* 0.53836 + 0.46164*cospi(2n/N)               */
+#define ZDT_WINDOW_CUSTOM 2  /* device vector of Nf/2+1 Float32 values w(k) */
+
+/* workspace flags */
+#define ZDT_FLAG_UNNORMALIZED 0x1 /* scale the FDR by Nf*Nl                  */
+#define ZDT_FLAG_GENERIC 0x2      /* force the generic (unfused) kernels     */
+#define ZDT_FLAG_PROFILE 0x4      /* time every kernel launch (HIP events)    */
+
+/* buffers exposed by zdt_buffer */
+#define ZDT_BUF_F 0  /* Nk x Nt ComplexF32: input FFT (natural layout)        */
+#define ZDT_BUF_Y 1  /* Nk x Nl ComplexF32: CZT work matrix (step functions)  */
+#define ZDT_BUF_V 2  /* Nk x Nl ComplexF32: FFT of the chirp kernel           */
+
+typedef struct zdt_ws zdt_ws;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* zdt_last_error(void);
+
+/* Library ABI version (incremented on incompatible changes). */
+int zdt_version(void);
+
+/* ---------------------------------------------------------------- sizing */
+
+/* calcNl(Nt, Nr, factors) = nextprod(factors, Nt+Nr-1)  -- src/zdtutils.jl:7-9.
+ * factors may be NULL (then (2,3,5)).  Returns -1 on bad input. */
+int64_t zdt_calc_nl(int64_t Nt, int64_t Nr, const int* factors, int nfactors);
+
+/* growNr(Nt, Nr, factors) = calcNl - Nt + 1  -- src/zdtutils.jl:17-19 */
+int64_t zdt_grow_nr(int64_t Nt, int64_t Nr, const int* factors, int nfactors);
+
+/* estimate_memory(Nf, Nt, Nr, Ni, No; factors)  -- src/zdtutils.jl:35-53
+ * (the reference's formula, in bytes). */
+int64_t zdt_estimate_memory(int64_t Nf, int64_t Nt, int64_t Nr, int64_t Ni,
+                            int64_t No, const int* factors, int nfactors);
+
+/* Device bytes this library would allocate for a workspace of that shape
+ * (fused path; the step functions add 2*Nk*Nl*8 + scratch on first use). */
+int64_t zdt_workspace_bytes(int64_t Nf, int64_t Nt, int64_t Nr,
+                            const int* factors, int nfactors, int flags);
+
+/* ------------------------------------------------------------- workspace */
+
+/* ZDTWorkspace(spectrogram, r0, dr, Nr, factors)  -- src/zdtfdr.jl:37-75.
+ * Computes Nl, allocates the device buffers, precomputes the chirp kernel and,
+ * when spec is not NULL, runs input! on it.  stream is a hipStream_t (NULL =
+ * the default stream).  The rates are r0 + j*dr, j < Nr, with r0/dr taken as
+ * Float32 (src/zdtfdr.jl:13-14). */
+int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr,
+               int64_t Nr, const int* factors, int nfactors, int flags,
+               void* stream, zdt_ws** out);
+
+/* Frees every device buffer of the workspace (the Julia finalizer). */
+int zdt_destroy(zdt_ws* ws);
+
+/* Shape of a workspace (Base.show, src/zdtfdr.jl:89-99). Any pointer may be NULL. */
+int zdt_info(const zdt_ws* ws, int64_t* Nf, int64_t* Nt, float* r0, float* dr,
+             int64_t* Nr, int64_t* Nl);
+
+/* Device bytes held by the workspace (Base.sizeof, src/zdtfdr.jl:101-103). */
+int zdt_sizeof(const zdt_ws* ws, int64_t* bytes);
+
+/* Rebind the workspace to another stream (update_stream,
+ * ext/CUDAFrequencyDriftRateTransformsExt.jl:99). */
+int zdt_set_stream(zdt_ws* ws, void* stream);
+
+/* fdrsynchronize(::Type{<:CuArray})  -- ext/CUDAFrequencyDriftRateTransformsExt.jl:26-28 */
+int zdt_sync(zdt_ws* ws);
+
+/* ------------------------------------------------------------ transform */
+
+/* input!(workspace, spectrogram)  -- src/zdtfdr.jl:209-212.
+ * spec: device Nf x Nt Float32, column stride Nf. */
+int zdt_input(zdt_ws* ws, const float* spec);
+
+/* zdtfdr!([w,] dest, workspace; r0)  -- src/zdtfdr.jl:349-396.
+ * One ZDT batch at rates r0 + j*dr from the current input: preprocess!,
+ * convolve!, postprocess! and, when dest is not NULL, output! into dest
+ * (device Nf x Nr Float32).  With dest NULL the post-processed spectra stay in
+ * the workspace for a later zdt_output (the no-dest method, :382-396).
+ * window_kind is one of ZDT_WINDOW_*; window is used only for CUSTOM. */
+int zdt_run(zdt_ws* ws, float r0, int window_kind, const float* window, float* dest);
+
+/* zdtfdr!(w, dests, workspace; r0=r0s)  -- src/zdtfdr.jl:349-362: nb batches
+ * from one input and one chirp kernel, dests[b] receiving the rates
+ * r0s[b] + j*dr.  dests is a HOST array of nb device pointers. */
+int zdt_run_batches(zdt_ws* ws, const float* r0s, int nb, int window_kind,
+                    const float* window, float* const* dests);
+
+/* output!(dest, workspace)  -- src/zdtfdr.jl:328-331 (CUDA: ext:94-103).
+ * Inverse real FFT of the post-processed spectra of the last zdt_run with a
+ * NULL dest (or of the step functions) into dest. */
+int zdt_output(zdt_ws* ws, float* dest);
+
+/* Step functions with the reference's materialized data flow (Y is Nk x Nl):
+ * preprocess!(workspace, r0)  -- src/zdtfdr.jl:221-239
+ * convolve!(workspace)        -- src/zdtfdr.jl:249-253
+ * postprocess!(w, workspace)  -- src/zdtfdr.jl:297-321 */
+int zdt_preprocess(zdt_ws* ws, float r0);
+int zdt_convolve(zdt_ws* ws);
+int zdt_postprocess(zdt_ws* ws, int window_kind, const float* window);
+
+/* Device view of a workspace buffer (ZDT_BUF_*): pointer, rows, columns.
+ * Y and V exist once a step function ran; F exists after zdt_input. */
+int zdt_buffer(zdt_ws* ws, int which, void** ptr, int64_t* rows, int64_t* cols);
+
+/* ------------------------------------------------------------ profiling
+ * No counterpart in the reference: a measurement hook for benchmarks.  With
+ * ZDT_FLAG_PROFILE every kernel the workspace launches is bracketed by a HIP
+ * event pair on the workspace stream.  zdt_profile_read synchronizes and
+ * aggregates the launches recorded since the last zdt_profile_reset by kernel
+ * name: entry idx (0-based, < *count) yields the name, the summed duration in
+ * milliseconds, the launch count and the algorithmic HBM bytes per launch. */
+int zdt_profile_reset(zdt_ws* ws);
+int zdt_profile_read(zdt_ws* ws, int idx, int* count, char* name, int name_len,
+                     double* total_ms, int64_t* launches, double* bytes_per_launch);
+
+/* ---------------------------------------------------------- statistics */
+
+/* fdrstats(fdr) / fdrstats(fdrs)  -- src/fdrutils.jl:39-50.
+ * m = mean of column 1 of fdrs[0]; s = minimum over all columns of all
+ * matrices of the corrected (N-1) standard deviation.  fdrs: HOST array of nb
+ * device pointers to Nf x Nr matrices with column stride ld. FP64 accumulation. */
+int zdt_fdrstats(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr,
+                 int64_t ld, void* stream, double* m, double* s);
+
+/* fdrnormalize!(fdr[s], m, s)  -- src/fdrutils.jl:70-90: x = (x - m)/s in place. */
+int zdt_fdrnormalize(float* const* fdrs, int nb, int64_t Nf, int64_t Nr,
+                     int64_t ld, double m, double s, void* stream);
+
+/* findprotohits(fdr[s], threshold)  -- src/fdrutils.jl:148-169.
+ * Every element >= threshold, in column-major order of the hcat'd batches, is
+ * written to hits (device) as the 0-based linear index i + Nf*(b*Nr + j)
+ * (the host turns it into the 1-based CartesianIndex (i+1, b*Nr+j+1)).  At
+ * most cap indices are written; *nhits receives the total count, and the call
+ * returns ZDT_ETRUNC when it exceeds cap.  hits may be NULL with cap 0 to count.
+ * fdrdenormalize (snr*s + m, src/fdrutils.jl:104-119) is done by the host. */
+int zdt_findprotohits(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr,
+                      int64_t ld, double threshold, int64_t* hits, int64_t cap,
+                      int64_t* nhits, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZDT_H */
