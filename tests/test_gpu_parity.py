@@ -32,7 +32,9 @@ def say(msg):
 
 
 def normwise(a, b):
-    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / np.max(np.abs(b)))
+    a = np.asarray(a)
+    a = a.astype(np.complex128) if np.iscomplexobj(a) else a.astype(np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
 def host(t):
