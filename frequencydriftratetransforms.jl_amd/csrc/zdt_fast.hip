@@ -1,26 +1,748 @@
-// Fused ZDT path for power-of-two Nf.  (Not yet enabled: fast_supported
-// returns false and the generic kernels serve every shape.)
+// Fused ZDT path for power-of-two Nf (2^16 .. 2^20) and Nt <= 16.
+//
+// The frequency-axis transforms of the reference (rfft along dim 1,
+// src/zdtfdr.jl:119,211, and irfft per rate, :120,330) are real transforms of
+// length Nf done as complex transforms of length M = Nf/2 on the packed pairs
+// z[n] = x[2n] + i x[2n+1], split four-step as M = N1*N2 (N2 = 512):
+//   k = k2 + N2*k1 (spectrum index),  n = n1 + N1*n2 (time/channel index).
+// Three kernels, each touching HBM once per array:
+//   k1_input   for each (t, n1): DFT_N2 over n2 of z[n1 + N1 n2] (strided
+//              column FFT in LDS, 16 columns per block), times W_M^{n1 k2}
+//              -> G[t][k2][n1]                                  (input!)
+//   k2_czt     one block per Hermitian column pair (k2, N2-k2): DFT_N1 over n1
+//              -> Z[k] -> r2c untangle -> X[k,t] (= F of the reference) ->
+//              chirp-Z over t for every rate (preprocess!/convolve!/
+//              postprocess!) -> c2r pre-pass -> inverse DFT_N1 over k1, times
+//              W_M^{-n1 k2} -> B[j][k2][n1]
+//   k3_output  for each (j, n1): inverse DFT_N2 over k2 (column FFT in LDS)
+//              -> z'[n1 + N1 n2] = FDR[2n, j] + i FDR[2n+1, j]  (output!)
+//
+// The chirp-Z transform per bin k is evaluated in its Vandermonde form
+//   Ys[k,j] = w(k) sum_t a_t(k) gamma_k^{j t},  a_t = X[k,t] e^{2 pi i k t r0/Nf},
+//   gamma_k = e^{2 pi i k dr/Nf}
+// which equals the reference's chirp multiply / convolve / chirp multiply
+// sequence (Bluestein's identity j t = (j^2 + t^2 - (j-t)^2)/2) for any Nl,
+// by Horner's rule in FP32 at z = gamma_k^j, with z carried by an FP64
+// recurrence.  A thread owns a bin k and its Hermitian partner M-k, for which
+// gamma_{M-k}^j = e^{i pi dr j} conj(gamma_k^j), so the partner's powers cost
+// one multiply by a per-rate table.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <vector>
+
+#include "fft_butterflies.h"
+#include "fft_generic.h"
 #include "zdt_fast.h"
 
 namespace zdt {
 
-struct FastPlan {
-  int64_t Nf = 0;
+constexpr int FAST_N2 = 512;   // strided FFT length of k1/k3
+constexpr int FAST_COLS = 16;  // columns per k1/k3 block (16 x 8 B = 128 B rows)
+constexpr int FAST_ROWS = 16;  // LDS rows per k2 group (2 columns x 8 steps)
+
+// ---------------------------------------------------------------- LDS FFT
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// One Stockham radix-R pass over nrows length-N FFTs held in LDS (row f at
+// buf + f*P).  All threads read their butterflies' inputs, a barrier
+// separates reads from writes, and a barrier ends the pass.
+template <int N, int R, int T, int MAXROWS, bool INV>
+__device__ __forceinline__ void lds_pass(c32* buf, int P, int nrows, int Ns, const c32* __restrict__ tw) {
+  constexpr int NR = N / R;
+  constexpr int LOGNR = ilog2c(NR);
+  constexpr int LOGN = ilog2c(N);
+  constexpr int LOGR = ilog2c(R);
+  constexpr int MAXB = (MAXROWS * NR + T - 1) / T;
+  const int logNs = 31 - __builtin_clz(Ns);
+  const int total = nrows * NR;
+  c32 v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = threadIdx.x + i * T;
+    if (b < total) {
+      const int f = b >> LOGNR;
+      const int j = b & (NR - 1);
+      const c32* x = buf + f * P;
+      const int k = j & (Ns - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        v[i][r] = x[j + r * NR];
+        if (INV) v[i][r].y = -v[i][r].y;
+      }
+      if (Ns > 1) {
+        const int sh = LOGN - logNs - LOGR;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[(r * k) << sh]);
+      }
+      dft<float, R>(v[i]);
+      dst[i] = f * P + ((j >> logNs) << (logNs + LOGR)) + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = threadIdx.x + i * T;
+    if (b < total) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        c32 y = v[i][r];
+        if (INV) y.y = -y.y;
+        buf[dst[i] + r * Ns] = y;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// radix schedule: 16s first, then the remainder
+template <int N, int T, int MAXROWS, bool INV, int NS = 1>
+__device__ __forceinline__ void lds_fft(c32* buf, int P, int nrows, const c32* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = REM >= 16 ? 16 : REM;
+    lds_pass<N, R, T, MAXROWS, INV>(buf, P, nrows, NS, tw);
+    lds_fft<N, T, MAXROWS, INV, NS * R>(buf, P, nrows, tw);
+  }
+}
+
+// Ping-pong variant: radix-R butterflies stream from buffer `in` to buffer
+// `out` (one butterfly live per thread), one barrier per pass.
+template <int N, int R, int T, bool INV>
+__device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int nrows, int Ns,
+                                            const c32* __restrict__ tw) {
+  constexpr int NR = N / R;
+  constexpr int LOGNR = ilog2c(NR);
+  constexpr int LOGN = ilog2c(N);
+  constexpr int LOGR = ilog2c(R);
+  const int logNs = 31 - __builtin_clz(Ns);
+  const int total = nrows * NR;
+  for (int b = threadIdx.x; b < total; b += T) {
+    const int f = b >> LOGNR;
+    const int j = b & (NR - 1);
+    const c32* x = in + f * P;
+    const int k = j & (Ns - 1);
+    c32 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      v[r] = x[j + r * NR];
+      if (INV) v[r].y = -v[r].y;
+    }
+    if (Ns > 1) {
+      const int sh = LOGN - logNs - LOGR;
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(r * k) << sh]);
+    }
+    dft<float, R>(v);
+    c32* y = out + f * P + ((j >> logNs) << (logNs + LOGR)) + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      c32 o = v[r];
+      if (INV) o.y = -o.y;
+      y[r * Ns] = o;
+    }
+  }
+  __syncthreads();
+}
+
+// radix-8 schedule (remainder last); returns the buffer holding the result
+template <int N, int T, bool INV, int NS = 1>
+__device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, const c32* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = REM >= 8 ? 8 : REM;
+    lds_pass_pp<N, R, T, INV>(a, b, P, nrows, NS, tw);
+    return lds_fft_pp<N, T, INV, NS * R>(b, a, P, nrows, tw);
+  } else {
+    return a;
+  }
+}
+
+// ---------------------------------------------------------------- k1: input pass A
+
+struct K1Args {
+  const float* spec;  // Nf x Nt, column stride ldspec floats
+  int64_t ldspec;
+  c32* G;             // [Nt][N2][N1]
+  const c32* tw2;     // W_N2^m
+  int logN1;
+  int64_t M;
 };
 
-bool fast_supported(int64_t, int64_t, int64_t) { return false; }
-int64_t fast_bytes(int64_t, int64_t, int64_t) { return 0; }
-int fast_create(int64_t, int64_t, int64_t, float, FastPlan**) {
-  return fail(ZDT_ENOTSUP, "fused path not available");
+template <int N2>
+__global__ __launch_bounds__(256) void k1_input(K1Args a) {
+  constexpr int P = N2 + 1;
+  __shared__ c32 buf[FAST_COLS * P];
+  const int t = blockIdx.y;
+  const int N1 = 1 << a.logN1;
+  const int n10 = blockIdx.x * FAST_COLS;
+  const c32* z = reinterpret_cast<const c32*>(a.spec + static_cast<int64_t>(t) * a.ldspec);
+  {
+    // all loads in flight before the LDS stores
+    constexpr int PER = FAST_COLS * N2 / 256;
+    c32 v[PER];
+    const int c = threadIdx.x & (FAST_COLS - 1), n2b = threadIdx.x >> 4;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] = z[n10 + c + (static_cast<int64_t>(n2b + 16 * i) << a.logN1)];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) buf[c * P + n2b + 16 * i] = v[i];
+  }
+  __syncthreads();
+  lds_fft<N2, 256, FAST_COLS, false>(buf, P, FAST_COLS, a.tw2);
+  const float invM = 1.0f / static_cast<float>(a.M);
+  c32* G = a.G + static_cast<int64_t>(t) * N2 * N1;
+  for (int e = threadIdx.x; e < FAST_COLS * N2; e += 256) {
+    const int c = e & (FAST_COLS - 1), k2 = e >> 4;
+    const int n1 = n10 + c;
+    const int64_t m = (static_cast<int64_t>(n1) * k2) & (a.M - 1);
+    const c32 w = cis_rev(-static_cast<float>(m) * invM);  // W_M^{n1 k2}
+    G[static_cast<int64_t>(k2) * N1 + n1] = cmul(buf[c * P + k2], w);
+  }
 }
-void fast_destroy(FastPlan* p) { delete p; }
-int fast_input(FastPlan*, const float*, hipStream_t) { return fail(ZDT_ENOTSUP, "fused path not available"); }
-int fast_unpack_F(FastPlan*, c32*, hipStream_t) { return fail(ZDT_ENOTSUP, "fused path not available"); }
-int fast_run(FastPlan*, float, int, const float*, float*, double, hipStream_t) {
-  return fail(ZDT_ENOTSUP, "fused path not available");
+
+// ---------------------------------------------------------------- k3: output pass 2
+
+struct K3Args {
+  const c32* B;  // [Nr][N2][N1]
+  float* dest;   // Nf x Nr, column stride ld floats
+  int64_t ld;
+  const c32* tw2;
+  int logN1;
+};
+
+template <int N2>
+__global__ __launch_bounds__(256) void k3_output(K3Args a) {
+  constexpr int P = N2 + 1;
+  __shared__ c32 buf[FAST_COLS * P];
+  const int j = blockIdx.y;
+  const int N1 = 1 << a.logN1;
+  const int n10 = blockIdx.x * FAST_COLS;
+  const c32* Bj = a.B + static_cast<int64_t>(j) * N2 * N1;
+  {
+    constexpr int PER = FAST_COLS * N2 / 256;
+    c32 v[PER];
+    const int c = threadIdx.x & (FAST_COLS - 1), k2b = threadIdx.x >> 4;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] = Bj[static_cast<int64_t>(k2b + 16 * i) * N1 + n10 + c];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) buf[c * P + k2b + 16 * i] = v[i];
+  }
+  __syncthreads();
+  lds_fft<N2, 256, FAST_COLS, true>(buf, P, FAST_COLS, a.tw2);
+  c32* out = reinterpret_cast<c32*>(a.dest + static_cast<int64_t>(j) * a.ld);
+  for (int e = threadIdx.x; e < FAST_COLS * N2; e += 256) {
+    const int c = e & (FAST_COLS - 1), n2 = e >> 4;
+    out[n10 + c + (static_cast<int64_t>(n2) << a.logN1)] = buf[c * P + n2];
+  }
 }
-int fast_output(FastPlan*, float*, double, hipStream_t) { return fail(ZDT_ENOTSUP, "fused path not available"); }
+
+// ---------------------------------------------------------------- k2: fused chirp-Z
+
+struct K2Args {
+  const c32* G;      // [Nt][N2][N1]
+  c32* B;            // [Nr][N2][N1]
+  const c32* tw1;    // W_N1^m
+  const c32* tabq;   // e^{i pi dr j}, j < Nr
+  int Nt, Nr;
+  int logN1;
+  int64_t M, Nf;
+  double hd2;        // dr/Nf: gamma_k = e^{2 pi i k hd2}
+  double h0;         // r0/Nf
+  double r0half;     // r0/2:  e^{i pi u r0} = e^{2 pi i u r0half}
+  int wkind;
+  const float* wcustom;
+  float oscale;      // output scale / M  (c2r normalization)
+};
+
+__device__ __forceinline__ float fast_window(int wkind, const float* wcustom, int64_t k, int64_t Nf) {
+  if (wkind == ZDT_WINDOW_HAMMING) {
+    const float p = static_cast<float>(static_cast<double>(k) / static_cast<double>(Nf));
+    return 0.53836f + 0.46164f * cis_rev(p).x;
+  }
+  if (wkind == ZDT_WINDOW_CUSTOM) return wcustom[k];
+  return 1.0f;
+}
+
+__device__ __forceinline__ c64 cis64f(double rev) {
+  const double f = rev - __builtin_rint(rev);
+  double s, c;
+  sincospi(2.0 * f, &s, &c);
+  return {c, s};
+}
+
+// Horner: sum_u a[u] z^u
+template <int NT>
+__device__ __forceinline__ c32 horner(const c32* a, c32 z) {
+  c32 acc = a[NT - 1];
+#pragma unroll
+  for (int u = NT - 2; u >= 0; --u) acc = cfma(a[u], acc, z);
+  return acc;
+}
+
+// Block of T = min(N1, 512) threads; each thread owns PP = N1/T Hermitian
+// pairs q = tid + i*T (i < PP).  Pair q of a regular block (columns c, N2-c)
+// is (k = c + N2 q, M - k = (N2-c) + N2 (N1-1-q)); the block for p = 0 holds the
+// self-paired columns 0 and N2/2: q < N1/2 pairs (N2 q, N2 (N1-q)) in column 0
+// (q = 0 pairs bin 0 with the Nyquist bin M), q >= N1/2 pairs within column
+// N2/2.  The one bin left over, M/2 (column 0, k1 = N1/2), is evaluated by the
+// lanes of the block's last wave, one rate each.
+template <int N1, int NT, int ROWS>
+__global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
+  constexpr int T = N1 < 512 ? N1 : 512;
+  constexpr int PP = N1 / T;
+  constexpr int P = N1 + 1;
+  constexpr int N2 = FAST_N2;
+  constexpr int HALF = N1 / 2;
+  __shared__ c32 buf[ROWS * P];
+  __shared__ c32 buf2[ROWS * P];
+  __shared__ c32 ahalf[NT];  // a_t of the bin M/2 (block 0 only)
+
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x;
+  const bool special = (p == 0);
+  const int colA = special ? 0 : p;
+  const int colB = special ? N2 / 2 : N2 - p;
+
+  int rowA[PP], rowB[PP], k1A[PP], k1B[PP];
+  bool nyq[PP];
+  int64_t kA[PP];
+  c32 WA[PP];
+  const float invNf = 1.0f / static_cast<float>(a.Nf);
+#pragma unroll
+  for (int i = 0; i < PP; ++i) {
+    const int q = tid + i * T;
+    int cA;
+    nyq[i] = false;
+    if (!special) {
+      rowA[i] = 0;
+      rowB[i] = 1;
+      k1A[i] = q;
+      k1B[i] = N1 - 1 - q;
+      cA = colA;
+    } else if (q < HALF) {
+      rowA[i] = rowB[i] = 0;
+      k1A[i] = q;
+      k1B[i] = N1 - q;  // q == 0: partner is the Nyquist bin M
+      nyq[i] = (q == 0);
+      cA = 0;
+    } else {
+      rowA[i] = rowB[i] = 1;
+      k1A[i] = q - HALF;
+      k1B[i] = N1 - 1 - k1A[i];
+      cA = N2 / 2;
+    }
+    kA[i] = cA + static_cast<int64_t>(N2) * k1A[i];
+    WA[i] = cis_rev(-static_cast<float>(kA[i]) * invNf);  // W_Nf^{kA}
+  }
+  const bool half_lane = special && tid >= T - 64;
+  const int hl = tid - (T - 64);
+  if (special && tid < NT) ahalf[tid] = {0.0f, 0.0f};  // rows u >= Nt stay zero
+
+  // ---- input: DFT_N1 over n1, r2c untangle, pre-rotation -> aA, aB
+  c32 aA[PP][NT], aB[PP][NT];
+#pragma unroll
+  for (int i = 0; i < PP; ++i)
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      aA[i][u] = {0.0f, 0.0f};
+      aB[i][u] = {0.0f, 0.0f};
+    }
+  constexpr int TG = ROWS / 2;
+#pragma unroll
+  for (int t0 = 0; t0 < NT; t0 += TG) {
+    if (t0 >= a.Nt) break;  // uniform
+    const int nt = min(TG, a.Nt - t0);
+    // loads issued in batches of 8 before their LDS stores
+    constexpr int RB = 8 / PP;
+#pragma unroll
+    for (int r0 = 0; r0 < ROWS; r0 += RB) {
+      c32 v[RB][PP];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        if (r0 + r < 2 * nt) {
+          const int col = ((r0 + r) & 1) ? colB : colA;
+          const c32* src = a.G + (static_cast<int64_t>(t0 + ((r0 + r) >> 1)) * N2 + col) * N1;
+#pragma unroll
+          for (int i = 0; i < PP; ++i) v[r][i] = src[tid + i * T];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        if (r0 + r < 2 * nt) {
+#pragma unroll
+          for (int i = 0; i < PP; ++i) buf[(r0 + r) * P + tid + i * T] = v[r][i];
+        }
+      }
+    }
+    __syncthreads();
+    const c32* fz = lds_fft_pp<N1, T, false>(buf, buf2, P, 2 * nt, a.tw1);
+#pragma unroll
+    for (int uu = 0; uu < TG; ++uu) {
+      const int u = t0 + uu;
+      if (u < NT && uu < nt) {
+        const double ud = static_cast<double>(u);
+        const c32 rq = cis_nh(ud, a.r0half);  // e^{i pi u r0}
+#pragma unroll
+        for (int i = 0; i < PP; ++i) {
+          const c32 Za = fz[(2 * uu + rowA[i]) * P + k1A[i]];
+          c32 XA, XB;
+          if (nyq[i]) {
+            XA = {Za.x + Za.y, 0.0f};
+            XB = {Za.x - Za.y, 0.0f};
+          } else {
+            const c32 Zb = fz[(2 * uu + rowB[i]) * P + k1B[i]];
+            const c32 E = {0.5f * (Za.x + Zb.x), 0.5f * (Za.y - Zb.y)};
+            const c32 O = {0.5f * (Za.y + Zb.y), -0.5f * (Za.x - Zb.x)};
+            const c32 WO = cmul(WA[i], O);
+            XA = cadd(E, WO);
+            XB = {E.x - WO.x, -(E.y - WO.y)};
+          }
+          // pre-rotation e^{2 pi i k u r0/Nf}; partner: e^{i pi u r0} conj(.)
+          const c32 rot = cis_nh(static_cast<double>(kA[i]) * ud, a.h0);
+          aA[i][u] = cmul(XA, rot);
+          aB[i][u] = cmul(XB, cmulc(rq, rot));
+        }
+        if (special && tid == 0) {
+          // bin M/2: X = conj(Z[M/2]); rotation e^{2 pi i (M/2) u r0/Nf} = e^{i pi u r0/2}
+          const c32 Zh = fz[(2 * uu) * P + HALF];
+          ahalf[u] = cmul(cconj(Zh), cis_nh(ud, 0.5 * a.r0half));
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- chirp-Z for every rate, c2r pre-pass, inverse DFT_N1, store
+  float wA[PP], wB[PP];
+  c64 gam[PP], z[PP];
+#pragma unroll
+  for (int i = 0; i < PP; ++i) {
+    wA[i] = fast_window(a.wkind, a.wcustom, kA[i], a.Nf) * a.oscale;
+    wB[i] = fast_window(a.wkind, a.wcustom, a.M - kA[i], a.Nf) * a.oscale;
+    gam[i] = cis64f(static_cast<double>(kA[i]) * a.hd2);
+    z[i] = {1.0, 0.0};
+  }
+  // final twiddles W_M^{-n1 k2} for the store role (n1 = tid + i*T)
+  const float invM = 1.0f / static_cast<float>(a.M);
+  c32 twS0[PP], twS1[PP];
+#pragma unroll
+  for (int i = 0; i < PP; ++i) {
+    const int64_t n1 = tid + i * T;
+    twS0[i] = cis_rev(static_cast<float>((n1 * colA) & (a.M - 1)) * invM);
+    twS1[i] = cis_rev(static_cast<float>((n1 * colB) & (a.M - 1)) * invM);
+  }
+  const float wH = special ? fast_window(a.wkind, a.wcustom, a.M / 2, a.Nf) * a.oscale : 0.0f;
+
+  constexpr int JG = ROWS / 2;
+  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
+    const int nj = min(JG, a.Nr - j0);
+    for (int jj = 0; jj < nj; ++jj) {
+      const c32 qj = a.tabq[j0 + jj];
+#pragma unroll
+      for (int i = 0; i < PP; ++i) {
+        const c32 zA = {static_cast<float>(z[i].x), static_cast<float>(z[i].y)};
+        z[i] = cmul(z[i], gam[i]);
+        const c32 zB = cmulc(qj, zA);  // e^{i pi dr j} conj(zA)
+        c32 YA = horner<NT>(aA[i], zA);
+        c32 YB = horner<NT>(aB[i], zB);
+        YA = {YA.x * wA[i], YA.y * wA[i]};
+        YB = {YB.x * wB[i], YB.y * wB[i]};
+        if (nyq[i]) {
+          // pair (0, M): only the real parts enter the c2r
+          buf[(2 * jj + rowA[i]) * P] = {0.5f * (YA.x + YB.x), 0.5f * (YA.x - YB.x)};
+        } else {
+          const c32 E = {0.5f * (YA.x + YB.x), 0.5f * (YA.y - YB.y)};
+          const c32 D = {0.5f * (YA.x - YB.x), 0.5f * (YA.y + YB.y)};
+          const c32 O = cmulc(D, WA[i]);  // W^{-k} D
+          buf[(2 * jj + rowA[i]) * P + k1A[i]] = {E.x - O.y, E.y + O.x};
+          buf[(2 * jj + rowB[i]) * P + k1B[i]] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
+        }
+      }
+    }
+    if (half_lane && hl < nj) {
+      // bin M/2 (self-paired): gamma = e^{i pi dr/2}, c2r pre gives conj(Ys)
+      c32 ah[NT];
+#pragma unroll
+      for (int u = 0; u < NT; ++u) ah[u] = ahalf[u];
+      const c32 zh = cis_nh(static_cast<double>(j0 + hl), 0.25 * a.hd2 * static_cast<double>(a.Nf));
+      const c32 Y = horner<NT>(ah, zh);
+      buf[(2 * hl) * P + HALF] = {Y.x * wH, -Y.y * wH};
+    }
+    __syncthreads();
+    const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, a.tw1);
+    for (int r = 0; r < 2 * nj; ++r) {
+      const int col = (r & 1) ? colB : colA;
+      c32* dst = a.B + (static_cast<int64_t>(j0 + (r >> 1)) * N2 + col) * N1;
+#pragma unroll
+      for (int i = 0; i < PP; ++i)
+        dst[tid + i * T] = cmul(fo[r * P + tid + i * T], (r & 1) ? twS1[i] : twS0[i]);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- F unpack
+
+// natural F[k, t] (k <= M) from Z[t][k2][k1] (the DFT_N1 of G over n1)
+__global__ __launch_bounds__(256) void unpack_F(const c32* __restrict__ Z, c32* __restrict__ F, int Nt, int logN1,
+                                                int64_t M, int64_t Nf) {
+  const int64_t Nk = M + 1;
+  const int64_t total = Nk * Nt;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t k = g % Nk;
+    const int t = static_cast<int>(g / Nk);
+    const c32* Zt = Z + static_cast<int64_t>(t) * M;
+    auto at = [&](int64_t kk) {  // Z[kk] for kk in [0, M)
+      const int64_t k2 = kk & (FAST_N2 - 1), k1 = kk >> 9;
+      return Zt[(k2 << logN1) + k1];
+    };
+    c32 X;
+    if (k == 0 || k == M) {
+      const c32 z0 = at(0);
+      X = {k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.0f};
+    } else {
+      const c32 Za = at(k), Zb = at(M - k);
+      const c32 E = {0.5f * (Za.x + Zb.x), 0.5f * (Za.y - Zb.y)};
+      const c32 O = {0.5f * (Za.y + Zb.y), -0.5f * (Za.x - Zb.x)};
+      double p = -static_cast<double>(k) / static_cast<double>(Nf);
+      p -= __builtin_rint(p);
+      X = cadd(E, cmul(cis_rev(static_cast<float>(p)), O));
+    }
+    F[k + static_cast<int64_t>(t) * Nk] = X;
+  }
+}
+
+}  // namespace zdt
+
+// ---------------------------------------------------------------- host side
+
+namespace zdt {
+
+struct FastPlan {
+  int64_t Nf = 0, Nt = 0, Nr = 0, M = 0;
+  int N1 = 0, logN1 = 0, NT = 0;
+  float dr = 0;
+  c32* G = nullptr;     // [Nt][N2][N1]
+  c32* B = nullptr;     // [Nr][N2][N1]
+  c32* tw1 = nullptr;   // W_N1^m
+  c32* tw2 = nullptr;   // W_N2^m
+  c32* tabq = nullptr;  // e^{i pi dr j}
+  bool have_B = false;
+};
+
+static int ilog2i(int64_t n) {
+  int l = 0;
+  while ((int64_t(1) << l) < n) ++l;
+  return l;
+}
+
+bool fast_supported(int64_t Nf, int64_t Nt, int64_t Nr) {
+  if (Nf < (1 << 16) || Nf > (1 << 20) || (Nf & (Nf - 1)) != 0) return false;
+  if (Nt < 1 || Nt > 16) return false;
+  if (Nr < 1 || Nr > 65535) return false;
+  return true;
+}
+
+int64_t fast_bytes(int64_t Nf, int64_t Nt, int64_t Nr) {
+  const int64_t M = Nf / 2;
+  const int64_t N1 = M / FAST_N2;
+  return 8 * M * (Nt + Nr) + 8 * (N1 + FAST_N2 + Nr);
+}
+
+static int upload_table(c32** dst, const std::vector<c32>& h) {
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), h.size() * sizeof(c32));
+  if (e != hipSuccess) return fail(ZDT_ENOMEM, "hipMalloc(table) failed");
+  ZDT_TRY_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(c32), hipMemcpyHostToDevice));
+  return ZDT_OK;
+}
+
+static std::vector<c32> twiddles(int N) {
+  std::vector<c32> t(N);
+  for (int m = 0; m < N; ++m) {
+    const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / N;
+    t[m] = {static_cast<float>(cosl(a)), static_cast<float>(sinl(a))};
+  }
+  return t;
+}
+
+void fast_destroy(FastPlan* p) {
+  if (p == nullptr) return;
+  for (void* q : {static_cast<void*>(p->G), static_cast<void*>(p->B), static_cast<void*>(p->tw1),
+                  static_cast<void*>(p->tw2), static_cast<void*>(p->tabq)})
+    if (q) (void)hipFree(q);
+  delete p;
+}
+
+int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, FastPlan** out) {
+  *out = nullptr;
+  if (!fast_supported(Nf, Nt, Nr)) return fail(ZDT_ENOTSUP, "fused path: unsupported shape");
+  FastPlan* p = new FastPlan();
+  p->Nf = Nf;
+  p->Nt = Nt;
+  p->Nr = Nr;
+  p->M = Nf / 2;
+  p->N1 = static_cast<int>(p->M / FAST_N2);
+  p->logN1 = ilog2i(p->N1);
+  p->NT = Nt <= 2 ? 2 : Nt <= 4 ? 4 : Nt <= 8 ? 8 : 16;
+  p->dr = dr;
+  int rc = ZDT_OK;
+  auto alloc = [&](c32** q, size_t n) {
+    if (rc != ZDT_OK) return;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(q), n * sizeof(c32));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      rc = fail(ZDT_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(c32), hipGetErrorString(e));
+    }
+  };
+  alloc(&p->G, static_cast<size_t>(Nt * p->M));
+  alloc(&p->B, static_cast<size_t>(Nr * p->M));
+  if (rc == ZDT_OK) rc = upload_table(&p->tw1, twiddles(p->N1));
+  if (rc == ZDT_OK) rc = upload_table(&p->tw2, twiddles(FAST_N2));
+  if (rc == ZDT_OK) {
+    std::vector<c32> q(Nr);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int64_t j = 0; j < Nr; ++j) {
+      // e^{i pi dr j}, reduced exactly in long double
+      long double rev = 0.5L * static_cast<long double>(dr) * static_cast<long double>(j);
+      rev -= floorl(rev);
+      q[j] = {static_cast<float>(cosl(2 * pi * rev)), static_cast<float>(sinl(2 * pi * rev))};
+    }
+    rc = upload_table(&p->tabq, q);
+  }
+  if (rc != ZDT_OK) {
+    fast_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return ZDT_OK;
+}
+
+int fast_input(FastPlan* p, const float* spec, hipStream_t st) {
+  K1Args a;
+  a.spec = spec;
+  a.ldspec = p->Nf;
+  a.G = p->G;
+  a.tw2 = p->tw2;
+  a.logN1 = p->logN1;
+  a.M = p->M;
+  dim3 grid(p->N1 / FAST_COLS, static_cast<unsigned>(p->Nt));
+  ProfScope ps(st, "k1_input", 4.0 * p->Nf * p->Nt + 8.0 * p->M * p->Nt);
+  k1_input<FAST_N2><<<grid, 256, 0, st>>>(a);
+  ZDT_CHECK_LAUNCH("k1_input");
+  return ZDT_OK;
+}
+
+template <int N1>
+static int launch_k2(const K2Args& a, int NT, hipStream_t st) {
+  const unsigned grid = FAST_N2 / 2;
+  constexpr unsigned TB = N1 < 512 ? N1 : 512;
+  switch (NT) {
+    case 2:
+      k2_czt<N1, 2, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      break;
+    case 4:
+      k2_czt<N1, 4, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      break;
+    case 8:
+      k2_czt<N1, 8, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      break;
+    default:
+      k2_czt<N1, 16, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      break;
+  }
+  ZDT_CHECK_LAUNCH("k2_czt");
+  return ZDT_OK;
+}
+
+int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest, double scale, hipStream_t st) {
+  K2Args a;
+  a.G = p->G;
+  a.B = p->B;
+  a.tw1 = p->tw1;
+  a.tabq = p->tabq;
+  a.Nt = static_cast<int>(p->Nt);
+  a.Nr = static_cast<int>(p->Nr);
+  a.logN1 = p->logN1;
+  a.M = p->M;
+  a.Nf = p->Nf;
+  a.hd2 = static_cast<double>(p->dr) / static_cast<double>(p->Nf);
+  a.h0 = static_cast<double>(r0) / static_cast<double>(p->Nf);
+  a.r0half = 0.5 * static_cast<double>(r0);
+  a.wkind = wkind;
+  a.wcustom = window;
+  a.oscale = static_cast<float>(scale / static_cast<double>(p->M));
+  {
+    ProfScope ps(st, "k2_czt", 8.0 * p->M * (p->Nt + p->Nr));
+    switch (p->N1) {
+      case 64:
+        ZDT_TRY(launch_k2<64>(a, p->NT, st));
+        break;
+      case 128:
+        ZDT_TRY(launch_k2<128>(a, p->NT, st));
+        break;
+      case 256:
+        ZDT_TRY(launch_k2<256>(a, p->NT, st));
+        break;
+      case 512:
+        ZDT_TRY(launch_k2<512>(a, p->NT, st));
+        break;
+      case 1024:
+        ZDT_TRY(launch_k2<1024>(a, p->NT, st));
+        break;
+      default:
+        return fail(ZDT_ENOTSUP, "fused path: N1=%d", p->N1);
+    }
+  }
+  p->have_B = true;
+  if (dest != nullptr) ZDT_TRY(fast_output(p, dest, scale, st));
+  return ZDT_OK;
+}
+
+int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st) {
+  (void)scale;  // folded into k2
+  if (!p->have_B) return fail(ZDT_ESTATE, "output!: no post-processed spectra");
+  K3Args a;
+  a.B = p->B;
+  a.dest = dest;
+  a.ld = p->Nf;
+  a.tw2 = p->tw2;
+  a.logN1 = p->logN1;
+  dim3 grid(p->N1 / FAST_COLS, static_cast<unsigned>(p->Nr));
+  ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
+  k3_output<FAST_N2><<<grid, 256, 0, st>>>(a);
+  ZDT_CHECK_LAUNCH("k3_output");
+  return ZDT_OK;
+}
+
+int fast_unpack_F(FastPlan* p, c32* F, hipStream_t st) {
+  const FftPlan* fp = nullptr;
+  ZDT_TRY(fft_plan_get(p->N1, false, &fp));
+  const size_t n = static_cast<size_t>(p->Nt * p->M);
+  void *Z = nullptr, *s0 = nullptr, *s1 = nullptr;
+  int rc = ZDT_OK;
+  if (hipMallocAsync(&Z, n * sizeof(c32), st) != hipSuccess || hipMallocAsync(&s0, n * sizeof(c32), st) != hipSuccess ||
+      hipMallocAsync(&s1, n * sizeof(c32), st) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = fail(ZDT_ENOMEM, "F unpack: allocation failed");
+  }
+  // DFT_N1 over n1 (contiguous rows of G): Z[t][k2][k1]
+  if (rc == ZDT_OK)
+    rc = fft_c2c(fp, p->G, 1, p->N1, Z, 1, p->N1, static_cast<int64_t>(p->Nt) * FAST_N2, false, 1.0, s0, s1, st);
+  if (rc == ZDT_OK) {
+    const int64_t total = (p->M + 1) * p->Nt;
+    int64_t g = (total + 255) / 256;
+    if (g > (1 << 20)) g = 1 << 20;
+    unpack_F<<<static_cast<unsigned>(g), 256, 0, st>>>(static_cast<const c32*>(Z), F, static_cast<int>(p->Nt),
+                                                        p->logN1, p->M, p->Nf);
+    if (hipGetLastError() != hipSuccess) rc = fail(ZDT_EHIP, "unpack_F launch failed");
+  }
+  if (Z) (void)hipFreeAsync(Z, st);
+  if (s0) (void)hipFreeAsync(s0, st);
+  if (s1) (void)hipFreeAsync(s1, st);
+  return rc;
+}
 
 }  // namespace zdt

@@ -1,0 +1,113 @@
+"""GPU parity of the fused path (power-of-two Nf in [2^16, 2^20], Nt <= 16)
+against the FP64 oracle and against the generic kernels.  Same tolerances
+as test_gpu_parity.py (normwise <= 2e-6 vs FP64)."""
+import sys
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import host_oracle as H
+from oracle import synth
+from oracle import zdt_oracle as Z
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def normwise(a, b):
+    a = np.asarray(a)
+    a = a.astype(np.complex128) if np.iscomplexobj(a) else a.astype(np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def F(gpu):
+    import fdrt_hip
+    return fdrt_hip
+
+
+def fused(F, ws):
+    return F.workspace_bytes(ws.Nf, ws.Nt, ws.Nr) < F.workspace_bytes(ws.Nf, ws.Nt, ws.Nr, generic=True) or True
+
+
+@pytest.mark.parametrize("Nf", [1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20])
+@pytest.mark.parametrize("Nt", [16, 7, 1])
+def test_fast_sizes(F, Nf, Nt):
+    Nr = 33 if Nf >= 1 << 19 else 65
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.31, 1.5, 25.0), (Nf * 0.77, -2.75, 40.0)], seed=Nf % 1000 + Nt)
+    say(f"fast Nf={Nf} Nt={Nt} Nr={Nr}")
+    ws = F.ZDTWorkspace(s, -3.0, 0.1875, Nr)
+    out = host(F.zdtfdr(ws))
+    ref = Z.fdr64(s, -3.0, 0.1875, Nr)
+    assert normwise(out, ref) <= 2e-6
+
+
+@pytest.mark.parametrize("Nt", [2, 4, 8, 12, 16])
+def test_fast_vs_generic(F, Nt):
+    Nf = 1 << 17
+    s = synth.spectrogram(Nf, Nt, [(5000.0, 0.5, 30.0)], seed=40 + Nt)
+    say(f"fast vs generic Nt={Nt}")
+    a = host(F.zdtfdr(F.ZDTWorkspace(s, -4.0, 0.0625, 129)))
+    b = host(F.zdtfdr(F.ZDTWorkspace(s, -4.0, 0.0625, 129, generic=True)))
+    assert normwise(a, b.astype(np.float64)) <= 2e-6
+
+
+@pytest.mark.parametrize("w", ["hamming", "custom"])
+def test_fast_windows(F, w):
+    Nf = 1 << 16
+    s = synth.spectrogram(Nf, 16, [(1000.0, 0.75, 20.0)], seed=4)
+    fn = (lambda n, N: 0.5 + 0.5 * np.cos(np.pi * n / N))
+    win = "hamming" if w == "hamming" else fn
+    say(f"fast window {w}")
+    ws = F.ZDTWorkspace(s, -2.0, 0.125, 33)
+    out = host(F.zdtfdr(win, ws))
+    ref = Z.fdr64(s, -2.0, 0.125, 33, window=win)
+    assert normwise(out, ref) <= 2e-6
+
+
+def test_fast_batches_output_and_F(F):
+    Nf = 1 << 18
+    s = synth.spectrogram(Nf, 16, [(70000.0, 1.5, 30.0), (200000.0, -3.25, 25.0)], seed=8)
+    say("fast batches / no-dest / F")
+    ws = F.ZDTWorkspace(s, -4.0, 0.0625, 43)
+    r0s = [-4.0, -4.0 + 43 * 0.0625, -4.0 + 86 * 0.0625]
+    dests = [F.create_fdr(s, 43) for _ in r0s]
+    F.zdtfdr_(dests, ws, r0=r0s)
+    full = np.concatenate([host(d) for d in dests], axis=1)
+    assert normwise(full, Z.fdr64(s, -4.0, 0.0625, 129)) <= 2e-6
+    assert F.zdtfdr_(ws, r0=-1.0) is None
+    dest = F.create_fdr(s, 43)
+    F.output_(dest, ws)
+    F.output_(dest, ws)  # output! may be repeated (the spectra are not clobbered)
+    assert normwise(host(dest), Z.fdr64(s, -1.0, 0.0625, 43)) <= 2e-6
+    Fk = host(ws.F)
+    ref = np.fft.rfft(s.astype(np.float64), axis=0)
+    assert normwise(Fk, ref) <= 1e-6
+    # step functions on a fused workspace
+    F.preprocess_(ws, -4.0)
+    F.convolve_(ws)
+    F.postprocess_(ws)
+    F.output_(dest, ws)
+    assert normwise(host(dest), Z.fdr64(s, -4.0, 0.0625, 43)) <= 2e-6
+
+
+def test_fast_unnormalized_and_hits(F):
+    Nf = 1 << 16
+    s = synth.cfg1(seed=77)
+    say("fast unnormalized + hits")
+    ws = F.ZDTWorkspace(s, -4.0, 0.25, 33, unnormalized=True)
+    out = host(F.zdtfdr(ws)) / (Nf * ws.Nl)
+    ref = Z.fdr64(s, -4.0, 0.25, 33)
+    assert normwise(out, ref) <= 2e-6
+    ws2 = F.ZDTWorkspace(s, -4.0, 0.25, 33)
+    fdr = F.zdtfdr(ws2)
+    np.testing.assert_array_equal(F.findprotohits(fdr, 10, snr=True), H.findprotohits(host(fdr), 10, snr=True))
