@@ -34,8 +34,15 @@ __device__ __forceinline__ void combine_even_odd(const cplx<T>* e, const cplx<T>
                                                  const double* cw, const double* sw) {
 #pragma unroll
   for (int k = 0; k < H; ++k) {
-    cplx<T> w = {static_cast<T>(cw[k]), static_cast<T>(-sw[k])};
-    cplx<T> ow = cmul(o[k], w);
+    cplx<T> ow;
+    if (k == 0) {
+      ow = o[0];
+    } else if (2 * k == H) {
+      ow = mul_neg_i(o[k]);
+    } else {
+      const cplx<T> w = {static_cast<T>(cw[k]), static_cast<T>(-sw[k])};
+      ow = cmul(o[k], w);
+    }
     out[k] = cadd(e[k], ow);
     out[k + H] = csub(e[k], ow);
   }
@@ -66,6 +73,21 @@ __device__ __forceinline__ void dft16(cplx<T>* v) {
   dft8(e);
   dft8(o);
   combine_even_odd<T, 8>(e, o, v, c16, s16);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft32(cplx<T>* v) {
+  constexpr double c32[16] = {1, 0.98078528040323043, 0.92387953251128674, 0.83146961230254524, 0.70710678118654757, 0.55557023301960229, 0.38268343236508984, 0.19509032201612833, 6.123233995736766e-17, -0.19509032201612819, -0.38268343236508973, -0.55557023301960196, -0.70710678118654746, -0.83146961230254535, -0.92387953251128674, -0.98078528040323043};
+  constexpr double s32[16] = {0, 0.19509032201612825, 0.38268343236508978, 0.55557023301960218, 0.70710678118654746, 0.83146961230254524, 0.92387953251128674, 0.98078528040323043, 1, 0.98078528040323043, 0.92387953251128674, 0.83146961230254546, 0.70710678118654757, 0.55557023301960218, 0.38268343236508989, 0.19509032201612861};
+  cplx<T> e[16], o[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    e[i] = v[2 * i];
+    o[i] = v[2 * i + 1];
+  }
+  dft16(e);
+  dft16(o);
+  combine_even_odd<T, 16>(e, o, v, c32, s32);
 }
 
 template <int R>
@@ -157,6 +179,8 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
     dft8(v);
   } else if constexpr (R == 16) {
     dft16(v);
+  } else if constexpr (R == 32) {
+    dft32(v);
   } else {
     dft_odd<T, R>(v);
   }

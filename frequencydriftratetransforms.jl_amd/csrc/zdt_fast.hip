@@ -28,6 +28,7 @@
 // one multiply by a per-rate table.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -160,45 +161,117 @@ __device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, con
   }
 }
 
+// ---------------------------------------------------------------- column FFT 512
+//
+// k1 and k3 transform 16 adjacent columns (n1 or k1 fixed, 128 contiguous
+// bytes per row) along a strided axis of length 512 = 16 x 32, register
+// resident: thread (c = tid&15, r = tid>>4) holds rows r + 16 m (m < 32) of
+// column c, does a DFT-32 over m, multiplies by W_512^{r m'}, and after one LDS
+// exchange the thread (c, g) does the DFT-16 over r for m' = g and g+16,
+// giving output rows m' + 32 r'.  Inverse transforms conjugate on entry and
+// exit.  Blocks are persistent and prefetch the next tile's 32 loads while
+// transforming the current one.
+
+constexpr int CF_PITCH = 16 * 16 + 8;        // complex per m' slab ([r][c] + pad)
+constexpr int CF_LDS = 32 * CF_PITCH;        // exchange buffer (complex)
+
+__device__ __forceinline__ void cf_stage1(c32 (&x)[32], int r, const c32* twl, c32* ex, int c, bool inv) {
+  if (inv) {
+#pragma unroll
+    for (int m = 0; m < 32; ++m) x[m].y = -x[m].y;
+  }
+  dft32(x);
+#pragma unroll
+  for (int m = 1; m < 32; ++m) x[m] = cmul(x[m], twl[r * m]);  // W_512^{r m'}
+#pragma unroll
+  for (int m = 0; m < 32; ++m) ex[m * CF_PITCH + r * 16 + c] = x[m];
+}
+
+// stage 2 for m' = g + 16 h: y[r'] is output row m' + 32 r'
+__device__ __forceinline__ void cf_stage2(c32 (&y)[16], const c32* ex, int c, int mp, bool inv) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) y[r] = ex[mp * CF_PITCH + r * 16 + c];
+  dft16(y);
+  if (inv) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) y[r].y = -y[r].y;
+  }
+}
+
+__device__ __forceinline__ void cf_load_twiddles(c32* twl, const c32* __restrict__ tw512) {
+  for (int i = threadIdx.x; i < 512; i += 256) twl[i] = tw512[i];
+}
+
 // ---------------------------------------------------------------- k1: input pass A
 
 struct K1Args {
   const float* spec;  // Nf x Nt, column stride ldspec floats
   int64_t ldspec;
   c32* G;             // [Nt][N2][N1]
-  const c32* tw2;     // W_N2^m
+  const c32* tw2;     // W_512^m
   int logN1;
   int64_t M;
+  int ntiles;         // Nt * N1/16
 };
 
-template <int N2>
-__global__ __launch_bounds__(256) void k1_input(K1Args a) {
-  constexpr int P = N2 + 1;
-  __shared__ c32 buf[FAST_COLS * P];
-  const int t = blockIdx.y;
+template <bool PREFETCH>
+__global__ __launch_bounds__(256, 2) void k1_input(K1Args a) {
+  __shared__ c32 ex[CF_LDS];
+  __shared__ c32 twl[512];
+  cf_load_twiddles(twl, a.tw2);
+  const int c = threadIdx.x & 15, r = threadIdx.x >> 4;
   const int N1 = 1 << a.logN1;
-  const int n10 = blockIdx.x * FAST_COLS;
-  const c32* z = reinterpret_cast<const c32*>(a.spec + static_cast<int64_t>(t) * a.ldspec);
-  {
-    // all loads in flight before the LDS stores
-    constexpr int PER = FAST_COLS * N2 / 256;
-    c32 v[PER];
-    const int c = threadIdx.x & (FAST_COLS - 1), n2b = threadIdx.x >> 4;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) v[i] = z[n10 + c + (static_cast<int64_t>(n2b + 16 * i) << a.logN1)];
-#pragma unroll
-    for (int i = 0; i < PER; ++i) buf[c * P + n2b + 16 * i] = v[i];
-  }
-  __syncthreads();
-  lds_fft<N2, 256, FAST_COLS, false>(buf, P, FAST_COLS, a.tw2);
+  const int nblk = N1 >> 4;
   const float invM = 1.0f / static_cast<float>(a.M);
-  c32* G = a.G + static_cast<int64_t>(t) * N2 * N1;
-  for (int e = threadIdx.x; e < FAST_COLS * N2; e += 256) {
-    const int c = e & (FAST_COLS - 1), k2 = e >> 4;
-    const int n1 = n10 + c;
-    const int64_t m = (static_cast<int64_t>(n1) * k2) & (a.M - 1);
-    const c32 w = cis_rev(-static_cast<float>(m) * invM);  // W_M^{n1 k2}
-    G[static_cast<int64_t>(k2) * N1 + n1] = cmul(buf[c * P + k2], w);
+  int tile = blockIdx.x;
+  if (tile >= a.ntiles) return;
+  auto src = [&](int tl) {
+    const int t = tl / nblk, nb = tl - t * nblk;
+    return reinterpret_cast<const c32*>(a.spec + static_cast<int64_t>(t) * a.ldspec) + (nb << 4) + c;
+  };
+  c32 x[32], xn[PREFETCH ? 32 : 1];
+  {
+    const c32* z = src(tile);
+#pragma unroll
+    for (int m = 0; m < 32; ++m) x[m] = z[static_cast<int64_t>(r + 16 * m) << a.logN1];
+  }
+  __syncthreads();  // twiddle table
+  for (;;) {
+    const int next = tile + gridDim.x;
+    if (PREFETCH && next < a.ntiles) {
+      const c32* z = src(next);
+#pragma unroll
+      for (int m = 0; m < 32; ++m) xn[m] = z[static_cast<int64_t>(r + 16 * m) << a.logN1];
+    }
+    cf_stage1(x, r, twl, ex, c, false);
+    __syncthreads();
+    const int t = tile / nblk, nb = tile - t * nblk;
+    const int n1 = (nb << 4) + c;
+    c32* G = a.G + static_cast<int64_t>(t) * FAST_N2 * N1 + n1;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int mp = r + 16 * h;  // thread (c, g=r) takes m' = g and g + 16
+      c32 y[16];
+      cf_stage2(y, ex, c, mp, false);
+#pragma unroll
+      for (int rp = 0; rp < 16; ++rp) {
+        const int k2 = mp + 32 * rp;
+        const int64_t m = (static_cast<int64_t>(n1) * k2) & (a.M - 1);
+        const c32 w = cis_rev(-static_cast<float>(m) * invM);  // W_M^{n1 k2}
+        G[static_cast<int64_t>(k2) * N1] = cmul(y[rp], w);
+      }
+    }
+    __syncthreads();
+    if (next >= a.ntiles) break;
+    if (PREFETCH) {
+#pragma unroll
+      for (int m = 0; m < 32; ++m) x[m] = xn[m];
+    } else {
+      const c32* z = src(next);
+#pragma unroll
+      for (int m = 0; m < 32; ++m) x[m] = z[static_cast<int64_t>(r + 16 * m) << a.logN1];
+    }
+    tile = next;
   }
 }
 
@@ -210,31 +283,64 @@ struct K3Args {
   int64_t ld;
   const c32* tw2;
   int logN1;
+  int Nr;
+  int ntiles;    // Nr * N1/16
 };
 
-template <int N2>
-__global__ __launch_bounds__(256) void k3_output(K3Args a) {
-  constexpr int P = N2 + 1;
-  __shared__ c32 buf[FAST_COLS * P];
-  const int j = blockIdx.y;
+template <bool PREFETCH>
+__global__ __launch_bounds__(256, 2) void k3_output(K3Args a) {
+  __shared__ c32 ex[CF_LDS];
+  __shared__ c32 twl[512];
+  cf_load_twiddles(twl, a.tw2);
+  const int c = threadIdx.x & 15, r = threadIdx.x >> 4;
   const int N1 = 1 << a.logN1;
-  const int n10 = blockIdx.x * FAST_COLS;
-  const c32* Bj = a.B + static_cast<int64_t>(j) * N2 * N1;
+  const int nblk = N1 >> 4;
+  int tile = blockIdx.x;
+  if (tile >= a.ntiles) return;
+  // rates last to first: the most recently written spectra are read first
+  auto jof = [&](int tl) { return a.Nr - 1 - tl / nblk; };
+  auto src = [&](int tl) {
+    const int jr = tl / nblk, nb = tl - jr * nblk;
+    return a.B + static_cast<int64_t>(a.Nr - 1 - jr) * FAST_N2 * N1 + (nb << 4) + c;
+  };
+  c32 x[32], xn[PREFETCH ? 32 : 1];
   {
-    constexpr int PER = FAST_COLS * N2 / 256;
-    c32 v[PER];
-    const int c = threadIdx.x & (FAST_COLS - 1), k2b = threadIdx.x >> 4;
+    const c32* b = src(tile);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) v[i] = Bj[static_cast<int64_t>(k2b + 16 * i) * N1 + n10 + c];
-#pragma unroll
-    for (int i = 0; i < PER; ++i) buf[c * P + k2b + 16 * i] = v[i];
+    for (int m = 0; m < 32; ++m) x[m] = b[static_cast<int64_t>(r + 16 * m) * N1];
   }
   __syncthreads();
-  lds_fft<N2, 256, FAST_COLS, true>(buf, P, FAST_COLS, a.tw2);
-  c32* out = reinterpret_cast<c32*>(a.dest + static_cast<int64_t>(j) * a.ld);
-  for (int e = threadIdx.x; e < FAST_COLS * N2; e += 256) {
-    const int c = e & (FAST_COLS - 1), n2 = e >> 4;
-    out[n10 + c + (static_cast<int64_t>(n2) << a.logN1)] = buf[c * P + n2];
+  for (;;) {
+    const int next = tile + gridDim.x;
+    if (PREFETCH && next < a.ntiles) {
+      const c32* b = src(next);
+#pragma unroll
+      for (int m = 0; m < 32; ++m) xn[m] = b[static_cast<int64_t>(r + 16 * m) * N1];
+    }
+    cf_stage1(x, r, twl, ex, c, true);
+    __syncthreads();
+    const int j = jof(tile);
+    const int nb = tile % nblk;
+    c32* out = reinterpret_cast<c32*>(a.dest + static_cast<int64_t>(j) * a.ld) + (nb << 4) + c;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int mp = r + 16 * h;
+      c32 y[16];
+      cf_stage2(y, ex, c, mp, true);
+#pragma unroll
+      for (int rp = 0; rp < 16; ++rp) out[static_cast<int64_t>(mp + 32 * rp) << a.logN1] = y[rp];
+    }
+    __syncthreads();
+    if (next >= a.ntiles) break;
+    if (PREFETCH) {
+#pragma unroll
+      for (int m = 0; m < 32; ++m) x[m] = xn[m];
+    } else {
+      const c32* b = src(next);
+#pragma unroll
+      for (int m = 0; m < 32; ++m) x[m] = b[static_cast<int64_t>(r + 16 * m) * N1];
+    }
+    tile = next;
   }
 }
 
@@ -531,6 +637,7 @@ struct FastPlan {
   c32* tw2 = nullptr;   // W_N2^m
   c32* tabq = nullptr;  // e^{i pi dr j}
   bool have_B = false;
+  int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
 };
 
 static int ilog2i(int64_t n) {
@@ -588,6 +695,11 @@ int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, FastPlan** out) {
   p->logN1 = ilog2i(p->N1);
   p->NT = Nt <= 2 ? 2 : Nt <= 4 ? 4 : Nt <= 8 ? 8 : 16;
   p->dr = dr;
+  {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    p->resident_blocks = 2 * static_cast<int64_t>(ncu);
+  }
   int rc = ZDT_OK;
   auto alloc = [&](c32** q, size_t n) {
     if (rc != ZDT_OK) return;
@@ -628,9 +740,10 @@ int fast_input(FastPlan* p, const float* spec, hipStream_t st) {
   a.tw2 = p->tw2;
   a.logN1 = p->logN1;
   a.M = p->M;
-  dim3 grid(p->N1 / FAST_COLS, static_cast<unsigned>(p->Nt));
+  a.ntiles = static_cast<int>(p->Nt * (p->N1 / 16));
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
   ProfScope ps(st, "k1_input", 4.0 * p->Nf * p->Nt + 8.0 * p->M * p->Nt);
-  k1_input<FAST_N2><<<grid, 256, 0, st>>>(a);
+  k1_input<false><<<grid, 256, 0, st>>>(a);
   ZDT_CHECK_LAUNCH("k1_input");
   return ZDT_OK;
 }
@@ -710,9 +823,11 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st) {
   a.ld = p->Nf;
   a.tw2 = p->tw2;
   a.logN1 = p->logN1;
-  dim3 grid(p->N1 / FAST_COLS, static_cast<unsigned>(p->Nr));
+  a.Nr = static_cast<int>(p->Nr);
+  a.ntiles = static_cast<int>(p->Nr * (p->N1 / 16));
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
   ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
-  k3_output<FAST_N2><<<grid, 256, 0, st>>>(a);
+  k3_output<false><<<grid, 256, 0, st>>>(a);
   ZDT_CHECK_LAUNCH("k3_output");
   return ZDT_OK;
 }
