@@ -109,6 +109,11 @@ __device__ __forceinline__ void lds_fft(c32* buf, int P, int nrows, const c32* _
   }
 }
 
+// LDS row swizzle: one pad entry after every 8, so the strided Stockham
+// writes of the early passes spread over the banks.  Rows of length N occupy
+// N + N/8 entries.
+__device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
+
 // Ping-pong variant: radix-R butterflies stream from buffer `in` to buffer
 // `out` (one butterfly live per thread), one barrier per pass.
 template <int N, int R, int T, bool INV>
@@ -128,7 +133,7 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
     c32 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      v[r] = x[j + r * NR];
+      v[r] = x[lsw(j + r * NR)];
       if (INV) v[r].y = -v[r].y;
     }
     if (Ns > 1) {
@@ -137,12 +142,13 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
       for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(r * k) << sh]);
     }
     dft<float, R>(v);
-    c32* y = out + f * P + ((j >> logNs) << (logNs + LOGR)) + k;
+    c32* y = out + f * P;
+    const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       c32 o = v[r];
       if (INV) o.y = -o.y;
-      y[r * Ns] = o;
+      y[lsw(d0 + r * Ns)] = o;
     }
   }
   __syncthreads();
@@ -398,11 +404,13 @@ template <int N1, int NT, int ROWS>
 __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
   constexpr int T = N1 < 512 ? N1 : 512;
   constexpr int PP = N1 / T;
-  constexpr int P = N1 + 1;
+  constexpr int P = N1 + N1 / 8;  // swizzled row pitch (see lsw)
   constexpr int N2 = FAST_N2;
   constexpr int HALF = N1 / 2;
   __shared__ c32 buf[ROWS * P];
   __shared__ c32 buf2[ROWS * P];
+  __shared__ c32 twl[N1];  // W_N1^m
+  for (int i = threadIdx.x; i < N1; i += T) twl[i] = a.tw1[i];
   __shared__ c32 ahalf[NT];  // a_t of the bin M/2 (block 0 only)
 
   const int tid = threadIdx.x;
@@ -478,12 +486,12 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
       for (int r = 0; r < RB; ++r) {
         if (r0 + r < 2 * nt) {
 #pragma unroll
-          for (int i = 0; i < PP; ++i) buf[(r0 + r) * P + tid + i * T] = v[r][i];
+          for (int i = 0; i < PP; ++i) buf[(r0 + r) * P + lsw(tid + i * T)] = v[r][i];
         }
       }
     }
     __syncthreads();
-    const c32* fz = lds_fft_pp<N1, T, false>(buf, buf2, P, 2 * nt, a.tw1);
+    const c32* fz = lds_fft_pp<N1, T, false>(buf, buf2, P, 2 * nt, twl);
 #pragma unroll
     for (int uu = 0; uu < TG; ++uu) {
       const int u = t0 + uu;
@@ -492,13 +500,13 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
         const c32 rq = cis_nh(ud, a.r0half);  // e^{i pi u r0}
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
-          const c32 Za = fz[(2 * uu + rowA[i]) * P + k1A[i]];
+          const c32 Za = fz[(2 * uu + rowA[i]) * P + lsw(k1A[i])];
           c32 XA, XB;
           if (nyq[i]) {
             XA = {Za.x + Za.y, 0.0f};
             XB = {Za.x - Za.y, 0.0f};
           } else {
-            const c32 Zb = fz[(2 * uu + rowB[i]) * P + k1B[i]];
+            const c32 Zb = fz[(2 * uu + rowB[i]) * P + lsw(k1B[i])];
             const c32 E = {0.5f * (Za.x + Zb.x), 0.5f * (Za.y - Zb.y)};
             const c32 O = {0.5f * (Za.y + Zb.y), -0.5f * (Za.x - Zb.x)};
             const c32 WO = cmul(WA[i], O);
@@ -512,7 +520,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
         }
         if (special && tid == 0) {
           // bin M/2: X = conj(Z[M/2]); rotation e^{2 pi i (M/2) u r0/Nf} = e^{i pi u r0/2}
-          const c32 Zh = fz[(2 * uu) * P + HALF];
+          const c32 Zh = fz[(2 * uu) * P + lsw(HALF)];
           ahalf[u] = cmul(cconj(Zh), cis_nh(ud, 0.5 * a.r0half));
         }
       }
@@ -562,8 +570,8 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
           const c32 E = {0.5f * (YA.x + YB.x), 0.5f * (YA.y - YB.y)};
           const c32 D = {0.5f * (YA.x - YB.x), 0.5f * (YA.y + YB.y)};
           const c32 O = cmulc(D, WA[i]);  // W^{-k} D
-          buf[(2 * jj + rowA[i]) * P + k1A[i]] = {E.x - O.y, E.y + O.x};
-          buf[(2 * jj + rowB[i]) * P + k1B[i]] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
+          buf[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
+          buf[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
         }
       }
     }
@@ -574,16 +582,16 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
       for (int u = 0; u < NT; ++u) ah[u] = ahalf[u];
       const c32 zh = cis_nh(static_cast<double>(j0 + hl), 0.25 * a.hd2 * static_cast<double>(a.Nf));
       const c32 Y = horner<NT>(ah, zh);
-      buf[(2 * hl) * P + HALF] = {Y.x * wH, -Y.y * wH};
+      buf[(2 * hl) * P + lsw(HALF)] = {Y.x * wH, -Y.y * wH};
     }
     __syncthreads();
-    const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, a.tw1);
+    const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, twl);
     for (int r = 0; r < 2 * nj; ++r) {
       const int col = (r & 1) ? colB : colA;
       c32* dst = a.B + (static_cast<int64_t>(j0 + (r >> 1)) * N2 + col) * N1;
 #pragma unroll
       for (int i = 0; i < PP; ++i)
-        dst[tid + i * T] = cmul(fo[r * P + tid + i * T], (r & 1) ? twS1[i] : twS0[i]);
+        dst[tid + i * T] = cmul(fo[r * P + lsw(tid + i * T)], (r & 1) ? twS1[i] : twS0[i]);
     }
     __syncthreads();
   }
