@@ -52,8 +52,10 @@ def synthetic_spectrogram(Nf: int, Nt: int, seed: int = 1234) -> np.ndarray:
 
 def traffic_from_profiles(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (profiles/traffic_<kernel>_cfg2.txt), or None."""
-    p = ROOT / "profiles" / f"traffic_{kernel}_cfg2.txt"
+    summary (profiles/traffic_<kernel>_2p20x16x129.txt), or None.  The summary
+    holds FETCH_SIZE and WRITE_SIZE in KiB per dispatch and the factors that
+    map each counter onto the bytes of a plain device copy of known size."""
+    p = ROOT / "profiles" / f"traffic_{kernel}_2p20x16x129.txt"
     if not p.exists():
         return None
     vals = {}
@@ -65,9 +67,24 @@ def traffic_from_profiles(kernel: str):
             except ValueError:
                 pass
     try:
-        return (vals["FETCH_KIB_PER_DISPATCH"] + vals["WRITE_KIB_PER_DISPATCH"]) * 1024.0 * vals.get("COPY_SCALE", 1.0)
+        return 1024.0 * (vals["FETCH_KIB_PER_DISPATCH"] * vals["FETCH_SCALE"]
+                         + vals["WRITE_KIB_PER_DISPATCH"] * vals["WRITE_SCALE"])
     except KeyError:
         return None
+
+
+def copy_calibration() -> int:
+    """A plain 1 GiB device-to-device copy (three launches): run under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to get the counters' scale."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": 4 * n}))
+    return 0
 
 
 def cpu_baseline(seed: int = 1234) -> dict:
@@ -93,10 +110,15 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--generic", action="store_true", help="force the unfused kernels")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (to scale rocprof byte counters)")
     a = ap.parse_args(argv)
+    if a.copy_calibration:
+        return copy_calibration()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every GPU box in this setup has one; "
-              "the path shards by spectrogram and drift-rate batch (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU (N=1 only); the path shards across GPUs by "
+              "spectrogram and by drift-rate batch, which one GPU's throughput already shows",
+              file=sys.stderr)
         return 2
 
     import torch
@@ -109,8 +131,7 @@ def main(argv=None) -> int:
     spec_rows = torch.from_numpy(synthetic_spectrogram(NF, NT)).to(dev)
     spec = spec_rows.t()  # (Nf, Nt) column-major view
     flags = _lib.ZDT_FLAG_PROFILE | (_lib.ZDT_FLAG_GENERIC if a.generic else 0)
-    ws = F.ZDTWorkspace(spec, R0, DR, NR, generic=a.generic)
-    # a second handle with the profiling hook on the same stream
+    # the workspace is created through the C-ABI so the profiling hook is on
     L = _lib.lib()
     h = C.c_void_p()
     arr, n = _lib.int_array((2, 3, 5))

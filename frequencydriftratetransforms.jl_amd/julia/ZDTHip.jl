@@ -1,0 +1,328 @@
+"""
+    ZDTHip
+
+Julia host module for libzdt_hip.so, the MI355X (gfx950) implementation of the
+Chirp-Z De-Doppler Transform path of FrequencyDriftRateTransforms.jl.  It keeps
+the package's call signatures (`ZDTWorkspace`, `input!`, `preprocess!`,
+`convolve!`, `postprocess!`, `output!`, `zdtfdr!`, `zdtfdr`, `findprotohits`,
+`fdrstats`, `fdrnormalize!`, `fdrdenormalize`, `fdrsynchronize`, `calcNl`,
+`growNr`, `estimate_memory`, `driftrates`, `batchrates`) and reaches the
+device through `ccall` only: device memory is a small `HipMatrix` handle
+(pointer + size) managed through the HIP runtime's C API, so no GPU package is
+needed.  Every library call returns a status; a non-zero status throws
+`ErrorException(zdt_last_error())`.
+
+Reference interfaces replaced: src/zdtfdr.jl:10-421, src/zdtutils.jl:7-64,
+src/batchrates.jl:22-60, src/fdrutils.jl:11-169,
+ext/CUDAFrequencyDriftRateTransformsExt.jl:26-103.
+"""
+module ZDTHip
+
+export HipMatrix, ZDTWorkspace, input!, output!, preprocess!, convolve!, postprocess!,
+       zdtfdr, zdtfdr!, calcNl, growNr, estimate_memory, driftrates, batchrates,
+       create_fdr, fdrstats, fdrsynchronize, fdrnormalize!, fdrnormalize, fdrdenormalize,
+       findprotohits
+
+const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "lib", "libzdt_hip.so"))
+const LIBHIP = "libamdhip64.so"
+
+const ZDT_OK = 0
+const ZDT_ETRUNC = 5
+const WINDOW_RECT = 0
+const WINDOW_HAMMING = 1
+const WINDOW_CUSTOM = 2
+const FLAG_UNNORMALIZED = 0x1
+
+zdt_error() = unsafe_string(ccall((:zdt_last_error, LIBZDT), Cstring, ()))
+check(rc::Integer) = rc == ZDT_OK ? nothing : error(zdt_error())
+
+# ----------------------------------------------------------------- device memory
+
+"""
+    HipMatrix{T}(undef, m, n) / HipMatrix(A::Matrix)
+
+Column-major device matrix: the stand-in for a `CuMatrix`.  `Array(H)` copies
+back to the host.  Memory is released by a finalizer (`hipFree`).
+"""
+mutable struct HipMatrix{T} <: AbstractMatrix{T}
+    ptr::Ptr{T}
+    dims::NTuple{2,Int}
+    function HipMatrix{T}(::UndefInitializer, m::Integer, n::Integer) where {T}
+        p = Ref{Ptr{Cvoid}}(C_NULL)
+        nbytes = max(m * n * sizeof(T), 1)
+        rc = ccall((:hipMalloc, LIBHIP), Cint, (Ref{Ptr{Cvoid}}, Csize_t), p, nbytes)
+        rc == 0 || error("hipMalloc($nbytes) failed ($rc)")
+        H = new{T}(Ptr{T}(p[]), (Int(m), Int(n)))
+        finalizer(H) do h
+            ccall((:hipFree, LIBHIP), Cint, (Ptr{Cvoid},), h.ptr)
+        end
+        return H
+    end
+end
+
+function HipMatrix(A::AbstractMatrix{T}) where {T}
+    H = HipMatrix{T}(undef, size(A)...)
+    B = Matrix{T}(A)
+    rc = ccall((:hipMemcpy, LIBHIP), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint),
+               H.ptr, B, sizeof(B), 1)  # hipMemcpyHostToDevice
+    rc == 0 || error("hipMemcpy H2D failed ($rc)")
+    return H
+end
+
+Base.size(H::HipMatrix) = H.dims
+Base.getindex(::HipMatrix, ::Int...) = error("HipMatrix elements live on the device; use Array(H)")
+Base.pointer(H::HipMatrix) = H.ptr
+
+function Base.Array(H::HipMatrix{T}) where {T}
+    A = Matrix{T}(undef, H.dims...)
+    rc = ccall((:hipMemcpy, LIBHIP), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint),
+               A, H.ptr, sizeof(A), 2)  # hipMemcpyDeviceToHost
+    rc == 0 || error("hipMemcpy D2H failed ($rc)")
+    return A
+end
+
+# ----------------------------------------------------------------- sizing (src/zdtutils.jl)
+
+_factors(f) = Cint[Int(x) for x in f]
+
+"calcNl(Nt, Nr, factors=(2,3,5)): nextprod(factors, Nt+Nr-1) -- src/zdtutils.jl:7-9"
+function calcNl(Nt, Nr, factors=(2, 3, 5))
+    f = _factors(factors)
+    v = ccall((:zdt_calc_nl, LIBZDT), Int64, (Int64, Int64, Ptr{Cint}, Cint), Nt, Nr, f, length(f))
+    v < 0 && error("calcNl: invalid arguments")
+    return Int(v)
+end
+
+"growNr(Nt, Nr, factors=(2,3,5)) -- src/zdtutils.jl:17-19"
+growNr(Nt, Nr, factors=(2, 3, 5)) = calcNl(Nt, Nr, factors) - Nt + 1
+
+"estimate_memory(Nf, Nt, Nr, Ni=1, No=1; factors) -- src/zdtutils.jl:35-53"
+function estimate_memory(Nf, Nt, Nr, Ni=1, No=1; factors=(2, 3, 5))
+    f = _factors(factors)
+    ccall((:zdt_estimate_memory, LIBZDT), Int64,
+          (Int64, Int64, Int64, Int64, Int64, Ptr{Cint}, Cint), Nf, Nt, Nr, Ni, No, f, length(f))
+end
+
+"batchrates(Nt, δhzps, r1, r2=-r1; Nrb=360) -- src/batchrates.jl:22-60"
+function batchrates(Nt::Integer, δhzps, r1, r2=-r1; Nrb=360)
+    if r2 < r1
+        r1, r2 = r2, r1
+    end
+    nc1 = round(Int, r1 / δhzps)
+    nc2 = round(Int, r2 / δhzps)
+    Nrspan = abs(nc2 - nc1) + 1
+    Nrb = growNr(Nt, min(Nrb, Nrspan))
+    Nr = cld(Nrspan, Nrb) * Nrb
+    Nrbonus = Nr - Nrspan
+    nc1 -= round(Int, sign(δhzps) * Nrbonus / 2, RoundFromZero)
+    nc2 += round(Int, sign(δhzps) * Nrbonus / 2, RoundToZero)
+    rs = range(nc1 // (Nt - 1), stop=nc2 // (Nt - 1), length=Nr)
+    @assert length(rs) % Nrb == 0
+    collect(Iterators.partition(rs, Nrb))
+end
+
+# ----------------------------------------------------------------- workspace
+
+"""
+    ZDTWorkspace(spectrogram::HipMatrix{Float32}, rates[, factors]; unnormalized=false)
+    ZDTWorkspace(spectrogram::HipMatrix{Float32}, r0, δr, Nr[, factors]; unnormalized=false)
+
+src/zdtfdr.jl:37-87.  Allocates the device buffers and runs `input!`.
+"""
+mutable struct ZDTWorkspace
+    handle::Ptr{Cvoid}
+    Nf::Int
+    Nt::Int
+    r0::Float32
+    δr::Float32
+    Nr::Int
+    Nl::Int
+    factors::Tuple
+    stream::Ptr{Cvoid}
+    window::Union{Nothing,HipMatrix{Float32}}
+end
+
+function ZDTWorkspace(spec::HipMatrix{Float32}, r0::Real, δr::Real, Nr::Integer,
+                      factors=(2, 3, 5); unnormalized::Bool=false, stream::Ptr{Cvoid}=C_NULL)
+    Nf, Nt = size(spec)
+    f = _factors(factors)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    flags = unnormalized ? FLAG_UNNORMALIZED : 0
+    check(ccall((:zdt_create, LIBZDT), Cint,
+                (Ptr{Float32}, Int64, Int64, Cfloat, Cfloat, Int64, Ptr{Cint}, Cint, Cint, Ptr{Cvoid},
+                 Ref{Ptr{Cvoid}}),
+                spec.ptr, Nf, Nt, Float32(r0), Float32(δr), Nr, f, length(f), flags, stream, h))
+    nl = Ref{Int64}(0)
+    check(ccall((:zdt_info, LIBZDT), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Cfloat}, Ptr{Cfloat}, Ptr{Int64}, Ref{Int64}),
+                h[], C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, nl))
+    ws = ZDTWorkspace(h[], Nf, Nt, Float32(r0), Float32(δr), Nr, Int(nl[]), Tuple(factors), stream, nothing)
+    finalizer(ws) do w
+        ccall((:zdt_destroy, LIBZDT), Cint, (Ptr{Cvoid},), w.handle)
+    end
+    return ws
+end
+
+function ZDTWorkspace(spec::HipMatrix{Float32}, rates::AbstractRange, factors=(2, 3, 5); kw...)
+    ZDTWorkspace(spec, first(rates), step(rates), length(rates), factors; kw...)
+end
+
+function Base.show(io::IO, ws::ZDTWorkspace)
+    print(io, "ZDTWorkspace(Nf=", ws.Nf, ",Nt=", ws.Nt, ",r0=", ws.r0, ",δr=", ws.δr,
+          ",Nr=", ws.Nr, ",Nl=", ws.Nl, ",", ws.factors, ")")
+end
+
+function Base.sizeof(ws::ZDTWorkspace)
+    b = Ref{Int64}(0)
+    check(ccall((:zdt_sizeof, LIBZDT), Cint, (Ptr{Cvoid}, Ref{Int64}), ws.handle, b))
+    Int(b[])
+end
+
+"driftrates(ws, r0=ws.r0) -- src/zdtutils.jl:62-64"
+driftrates(ws::ZDTWorkspace, r0=ws.r0) = range(r0, step=ws.δr, length=ws.Nr)
+
+"fdrsynchronize -- ext/CUDAFrequencyDriftRateTransformsExt.jl:26-28"
+fdrsynchronize(ws::ZDTWorkspace) = check(ccall((:zdt_sync, LIBZDT), Cint, (Ptr{Cvoid},), ws.handle))
+fdrsynchronize(::Type{<:HipMatrix}) = ccall((:hipDeviceSynchronize, LIBHIP), Cint, ())
+
+# window -> (kind, device pointer); src/zdtfdr.jl:297-321
+function _window(ws::ZDTWorkspace, w)
+    w isa Val && (w = typeof(w).parameters[1])
+    if w === :rect
+        return WINDOW_RECT, Ptr{Float32}(C_NULL)
+    elseif w === :hamming
+        return WINDOW_HAMMING, Ptr{Float32}(C_NULL)
+    elseif w isa Symbol
+        error("unsupported window type ($w)")
+    elseif w isa Function
+        v = Float32[w(n, ws.Nf) for n in 0:ws.Nf÷2]
+        ws.window = HipMatrix(reshape(v, :, 1))
+        return WINDOW_CUSTOM, ws.window.ptr
+    end
+    error("unsupported window argument $(typeof(w))")
+end
+
+"input!(workspace, spectrogram) -- src/zdtfdr.jl:209-212"
+function input!(ws::ZDTWorkspace, spec::HipMatrix{Float32})
+    size(spec) == (ws.Nf, ws.Nt) || error("input spectrogram has unexpected size")
+    check(ccall((:zdt_input, LIBZDT), Cint, (Ptr{Cvoid}, Ptr{Float32}), ws.handle, spec.ptr))
+end
+
+"preprocess!(workspace, r0=workspace.r0) -- src/zdtfdr.jl:221-239"
+preprocess!(ws::ZDTWorkspace, r0::Real=ws.r0) =
+    check(ccall((:zdt_preprocess, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat), ws.handle, Float32(r0)))
+
+"convolve!(workspace) -- src/zdtfdr.jl:249-253"
+convolve!(ws::ZDTWorkspace) = check(ccall((:zdt_convolve, LIBZDT), Cint, (Ptr{Cvoid},), ws.handle))
+
+"postprocess!([w,] workspace) -- src/zdtfdr.jl:297-321"
+function postprocess!(w, ws::ZDTWorkspace)
+    kind, ptr = _window(ws, w)
+    check(ccall((:zdt_postprocess, LIBZDT), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}), ws.handle, kind, ptr))
+end
+postprocess!(ws::ZDTWorkspace) = postprocess!(:rect, ws)
+
+"output!(dest, workspace) -- src/zdtfdr.jl:328-331"
+function output!(dest::HipMatrix{Float32}, ws::ZDTWorkspace)
+    size(dest) == (ws.Nf, ws.Nr) || error("dest has unexpected size")
+    check(ccall((:zdt_output, LIBZDT), Cint, (Ptr{Cvoid}, Ptr{Float32}), ws.handle, dest.ptr))
+    return dest
+end
+
+"zdtfdr!([w,] dests, workspace[, spectrogram]; r0) -- src/zdtfdr.jl:349-362"
+function zdtfdr!(w::Union{Function,Symbol,Val}, dests, ws::ZDTWorkspace, spectrogram=nothing; r0=ws.r0)
+    spectrogram === nothing || input!(ws, spectrogram)
+    kind, ptr = _window(ws, w)
+    for (dest, rate) in zip(dests, Iterators.cycle(r0))
+        size(dest) == (ws.Nf, ws.Nr) || error("dest has unexpected size")
+        check(ccall((:zdt_run, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat, Cint, Ptr{Float32}, Ptr{Float32}),
+                    ws.handle, Float32(rate), kind, ptr, dest.ptr))
+    end
+    return dests
+end
+zdtfdr!(dests, ws::ZDTWorkspace, spectrogram=nothing; r0=ws.r0) = zdtfdr!(:rect, dests, ws, spectrogram; r0)
+
+function zdtfdr!(w::Union{Function,Symbol,Val}, dest::HipMatrix{Float32}, ws::ZDTWorkspace, spectrogram=nothing;
+                 r0::Real=ws.r0)
+    zdtfdr!(w, (dest,), ws, spectrogram; r0)
+    return dest
+end
+zdtfdr!(dest::HipMatrix{Float32}, ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0) =
+    zdtfdr!(:rect, dest, ws, spectrogram; r0)
+
+# no dest: the result stays in the workspace for output! (src/zdtfdr.jl:382-396)
+function zdtfdr!(w::Union{Function,Symbol,Val}, ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0)
+    spectrogram === nothing || input!(ws, spectrogram)
+    kind, ptr = _window(ws, w)
+    check(ccall((:zdt_run, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat, Cint, Ptr{Float32}, Ptr{Float32}),
+                ws.handle, Float32(r0), kind, ptr, C_NULL))
+    return nothing
+end
+zdtfdr!(ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0) = zdtfdr!(:rect, ws, spectrogram; r0)
+
+"zdtfdr([w,] workspace[, spectrogram]; r0) -- src/zdtfdr.jl:412-421"
+function zdtfdr(w::Union{Function,Symbol,Val}, ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0)
+    dest = HipMatrix{Float32}(undef, ws.Nf, ws.Nr)
+    zdtfdr!(w, dest, ws, spectrogram; r0)
+end
+zdtfdr(ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0) = zdtfdr(:rect, ws, spectrogram; r0)
+
+# ----------------------------------------------------------------- statistics (src/fdrutils.jl)
+
+"create_fdr(spectrogram, Nr_or_rates) -- src/fdrutils.jl:11-26"
+create_fdr(spec::HipMatrix{Float32}, Nr::Integer) = HipMatrix{Float32}(undef, size(spec, 1), Nr)
+create_fdr(spec::HipMatrix{Float32}, rates) = create_fdr(spec, length(rates))
+
+_ptrs(fdrs) = Ptr{Float32}[f.ptr for f in fdrs]
+
+"fdrstats(fdr) / fdrstats(fdrs) -- src/fdrutils.jl:39-50"
+fdrstats(fdr::HipMatrix{Float32}) = fdrstats((fdr,))
+function fdrstats(fdrs)
+    Nf, Nr = size(first(fdrs))
+    m = Ref{Cdouble}(0)
+    s = Ref{Cdouble}(0)
+    check(ccall((:zdt_fdrstats, LIBZDT), Cint,
+                (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Ptr{Cvoid}, Ref{Cdouble}, Ref{Cdouble}),
+                _ptrs(fdrs), length(fdrs), Nf, Nr, Nf, C_NULL, m, s))
+    (m[], s[])
+end
+
+fdrnormalize(x::Number, m, s) = (x - m) / s
+
+"fdrnormalize!(fdr[s][, m, s]) -- src/fdrutils.jl:65-90"
+fdrnormalize!(fdr::HipMatrix{Float32}, m, s) = (fdrnormalize!((fdr,), m, s); fdr)
+fdrnormalize!(fdr::HipMatrix{Float32}) = fdrnormalize!(fdr, fdrstats(fdr)...)
+function fdrnormalize!(fdrs, m, s)
+    Nf, Nr = size(first(fdrs))
+    check(ccall((:zdt_fdrnormalize, LIBZDT), Cint,
+                (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cdouble, Cdouble, Ptr{Cvoid}),
+                _ptrs(fdrs), length(fdrs), Nf, Nr, Nf, m, s, C_NULL))
+    fdrs
+end
+fdrnormalize!(fdrs) = fdrnormalize!(fdrs, fdrstats(fdrs)...)
+
+"fdrdenormalize(snr, m, s) / fdrdenormalize(snr, fdr[s]) -- src/fdrutils.jl:104-119"
+fdrdenormalize(snr, m, s) = snr * s + m
+fdrdenormalize(snr, fdrs) = fdrdenormalize(snr, fdrstats(fdrs)...)
+
+"findprotohits(fdr[s], threshold; snr=false) -- src/fdrutils.jl:148-169"
+findprotohits(fdr::HipMatrix{Float32}, threshold; snr::Bool=false) = findprotohits((fdr,), threshold; snr)
+function findprotohits(fdrs, threshold; snr::Bool=false)
+    snr && (threshold = fdrdenormalize(threshold, fdrs))
+    Nf, Nr = size(first(fdrs))
+    n = Ref{Int64}(0)
+    ptrs = _ptrs(fdrs)
+    # count, then fetch the 0-based linear indices i + Nf*(b*Nr + j)
+    rc = ccall((:zdt_findprotohits, LIBZDT), Cint,
+               (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cdouble, Ptr{Int64}, Int64, Ref{Int64}, Ptr{Cvoid}),
+               ptrs, length(fdrs), Nf, Nr, Nf, threshold, C_NULL, 0, n, C_NULL)
+    rc in (ZDT_OK, ZDT_ETRUNC) || check(rc)
+    H = HipMatrix{Int64}(undef, max(n[], 1), 1)
+    check(ccall((:zdt_findprotohits, LIBZDT), Cint,
+                (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cdouble, Ptr{Int64}, Int64, Ref{Int64}, Ptr{Cvoid}),
+                ptrs, length(fdrs), Nf, Nr, Nf, threshold, H.ptr, n[], n, C_NULL))
+    lin = n[] == 0 ? Int64[] : vec(Array(H))[1:n[]]
+    [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
+end
+
+end # module ZDTHip
