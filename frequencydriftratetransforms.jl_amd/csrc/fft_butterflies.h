@@ -186,4 +186,84 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
   }
 }
 
+// ---------------------------------------------------------------- direction-templated
+// INV = false: W_R = exp(-2 pi i/R); INV = true: W_R = exp(+2 pi i/R) (unnormalized inverse)
+
+template <bool INV, typename T>
+__device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) {  // a * (-i) forward, a * (+i) inverse
+  return INV ? cplx<T>{-a.y, a.x} : cplx<T>{a.y, -a.x};
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void dft4d(cplx<T>* v) {
+  cplx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  cplx<T> t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+
+template <bool INV, typename T, int H>
+__device__ __forceinline__ void combine_d(const cplx<T>* e, const cplx<T>* o, cplx<T>* out, const double* cw,
+                                          const double* sw) {
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    cplx<T> ow;
+    if (k == 0) {
+      ow = o[0];
+    } else if (2 * k == H) {
+      ow = mul_mi<INV>(o[k]);
+    } else {
+      const cplx<T> w = {static_cast<T>(cw[k]), static_cast<T>(INV ? sw[k] : -sw[k])};
+      ow = cmul(o[k], w);
+    }
+    out[k] = cadd(e[k], ow);
+    out[k + H] = csub(e[k], ow);
+  }
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void dft8d(cplx<T>* v) {
+  constexpr double c8[4] = {1, 0.70710678118654757, 0, -0.70710678118654757};
+  constexpr double s8[4] = {0, 0.70710678118654757, 1, 0.70710678118654757};
+  cplx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft4d<INV>(e);
+  dft4d<INV>(o);
+  combine_d<INV, T, 4>(e, o, v, c8, s8);
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void dft16d(cplx<T>* v) {
+  constexpr double c16[8] = {1, 0.92387953251128674, 0.70710678118654757, 0.38268343236508978,
+                             0, -0.38268343236508978, -0.70710678118654757, -0.92387953251128674};
+  constexpr double s16[8] = {0, 0.38268343236508978, 0.70710678118654757, 0.92387953251128674,
+                             1, 0.92387953251128674, 0.70710678118654757, 0.38268343236508978};
+  cplx<T> e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    e[i] = v[2 * i];
+    o[i] = v[2 * i + 1];
+  }
+  dft8d<INV>(e);
+  dft8d<INV>(o);
+  combine_d<INV, T, 8>(e, o, v, c16, s16);
+}
+
+template <bool INV, typename T, int R>
+__device__ __forceinline__ void dftd(cplx<T>* v) {
+  if constexpr (R == 2) {
+    cplx<T> t = v[0];
+    v[0] = cadd(t, v[1]);
+    v[1] = csub(t, v[1]);
+  } else if constexpr (R == 4) {
+    dft4d<INV>(v);
+  } else if constexpr (R == 8) {
+    dft8d<INV>(v);
+  } else {
+    static_assert(R == 16, "radix");
+    dft16d<INV>(v);
+  }
+}
+
 }  // namespace zdt

@@ -115,7 +115,11 @@ __device__ __forceinline__ void lds_fft(c32* buf, int P, int nrows, const c32* _
 __device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
 
 // Ping-pong variant: radix-R butterflies stream from buffer `in` to buffer
-// `out` (one butterfly live per thread), one barrier per pass.
+// `out` (one butterfly live per thread), one barrier per pass.  Rows are
+// swizzled (lsw); since NR and Ns are multiples of 8 or the write base is,
+// every butterfly needs the swizzle only once for its reads and once for its
+// writes.  Twiddles come from the LDS table tw (W_N^m, m < N); INV selects the
+// inverse butterflies and conjugate twiddles.
 template <int N, int R, int T, bool INV>
 __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int nrows, int Ns,
                                             const c32* __restrict__ tw) {
@@ -123,43 +127,50 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
   constexpr int LOGNR = ilog2c(NR);
   constexpr int LOGN = ilog2c(N);
   constexpr int LOGR = ilog2c(R);
+  static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
   const int logNs = 31 - __builtin_clz(Ns);
   const int total = nrows * NR;
+  const int sh = LOGN - logNs - LOGR;
   for (int b = threadIdx.x; b < total; b += T) {
     const int f = b >> LOGNR;
     const int j = b & (NR - 1);
-    const c32* x = in + f * P;
     const int k = j & (Ns - 1);
+    const c32* x = in + f * P + lsw(j);
     c32 v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      v[r] = x[lsw(j + r * NR)];
-      if (INV) v[r].y = -v[r].y;
-    }
+    for (int r = 0; r < R; ++r) v[r] = x[r * (NR + NR / 8)];
     if (Ns > 1) {
-      const int sh = LOGN - logNs - LOGR;
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(r * k) << sh]);
+      for (int r = 1; r < R; ++r) {
+        const c32 w = tw[(r * k) << sh];
+        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+      }
     }
-    dft<float, R>(v);
-    c32* y = out + f * P;
+    dftd<INV, float, R>(v);
     const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
+    c32* y = out + f * P;
+    if (Ns == 1) {
+      // d0 = j*R is a multiple of 8 (R >= 8): lsw(d0 + r) = lsw(d0) + r + r/8
+      c32* yy = y + lsw(d0);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      c32 o = v[r];
-      if (INV) o.y = -o.y;
-      y[lsw(d0 + r * Ns)] = o;
+      for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
+    } else {
+      // Ns >= 8: the stride Ns keeps the swizzle offset linear
+      c32* yy = y + lsw(d0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
     }
   }
   __syncthreads();
 }
 
-// radix-8 schedule (remainder last); returns the buffer holding the result
+// radix schedule: 1024 = 16*8*8, 512 = 8*8*8, 256 = 16*16, 128 = 16*8,
+// 64 = 8*8; returns the buffer holding the result
 template <int N, int T, bool INV, int NS = 1>
 __device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, const c32* __restrict__ tw) {
   if constexpr (NS < N) {
     constexpr int REM = N / NS;
-    constexpr int R = REM >= 8 ? 8 : REM;
+    constexpr int R = (REM == 16 || (NS == 1 && ilog2c(N) % 3 != 0)) ? 16 : (REM >= 8 ? 8 : REM);
     lds_pass_pp<N, R, T, INV>(a, b, P, nrows, NS, tw);
     return lds_fft_pp<N, T, INV, NS * R>(b, a, P, nrows, tw);
   } else {
