@@ -3,14 +3,13 @@
 // The frequency-axis transforms of the reference (rfft along dim 1,
 // src/zdtfdr.jl:119,211, and irfft per rate, :120,330) are real transforms of
 // length Nf done as complex transforms of length M = Nf/2 on the packed pairs
-// z[n] = x[2n] + i x[2n+1], split four-step as M = N1*N2 (N2 = 512):
+// z[n] = x[2n] + i x[2n+1], split four-step as M = N1*N2 (N2 = 1024, or 512 for Nf = 2^16):
 //   k = k2 + N2*k1 (spectrum index),  n = n1 + N1*n2 (time/channel index).
 // Three kernels, each touching HBM once per array:
 //   k1_input   for each (t, n1): DFT_N2 over n2 of z[n1 + N1 n2] (strided
-//              column FFT in LDS, 16 columns per block), times W_M^{n1 k2}
-//              -> G[t][k2][n1]                                  (input!)
-//   k2_czt     one block per Hermitian column pair (k2, N2-k2): DFT_N1 over n1
-//              -> Z[k] -> r2c untangle -> X[k,t] (= F of the reference) ->
+//              column FFT, 16 columns per block) -> G[t][k2][n1]  (input!)
+//   k2_czt     one block per Hermitian column pair (k2, N2-k2): times
+//              W_M^{n1 k2}, DFT_N1 over n1 -> Z[k] -> r2c untangle -> X[k,t] (= F of the reference) ->
 //              chirp-Z over t for every rate (preprocess!/convolve!/
 //              postprocess!) -> c2r pre-pass -> inverse DFT_N1 over k1, times
 //              W_M^{-n1 k2} -> B[j][k2][n1]
@@ -38,9 +37,9 @@
 
 namespace zdt {
 
-constexpr int FAST_N2 = 512;   // strided FFT length of k1/k3
-constexpr int FAST_COLS = 16;  // columns per k1/k3 block (16 x 8 B = 128 B rows)
-constexpr int FAST_ROWS = 16;  // LDS rows per k2 group (2 columns x 8 steps)
+// strided FFT length N2 of k1/k3: 1024 (32 x 32) when that leaves N1 = M/N2 >= 64,
+// else 512 (32 x 16)
+constexpr int fast_n2(int64_t Nf) { return Nf >= (int64_t(1) << 17) ? 1024 : 512; }
 
 // ---------------------------------------------------------------- LDS FFT
 
@@ -134,7 +133,10 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
   for (int b = threadIdx.x; b < total; b += T) {
     const int f = b >> LOGNR;
     const int j = b & (NR - 1);
-    const int k = j & (Ns - 1);
+    int k = j & (Ns - 1);
+    // Opaque to the optimizer: keeps the R-1 twiddle addresses from being
+    // hoisted out of the caller's rate loop, where they would be spilled.
+    asm volatile("" : "+v"(k));
     const c32* x = in + f * P + lsw(j);
     c32 v[R];
 #pragma unroll
@@ -178,20 +180,26 @@ __device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, con
   }
 }
 
-// ---------------------------------------------------------------- column FFT 512
+// ---------------------------------------------------------------- column FFT (N2 = 32*R1)
 //
 // k1 and k3 transform 16 adjacent columns (n1 or k1 fixed, 128 contiguous
-// bytes per row) along a strided axis of length 512 = 16 x 32, register
-// resident: thread (c = tid&15, r = tid>>4) holds rows r + 16 m (m < 32) of
-// column c, does a DFT-32 over m, multiplies by W_512^{r m'}, and after one LDS
-// exchange the thread (c, g) does the DFT-16 over r for m' = g and g+16,
-// giving output rows m' + 32 r'.  Inverse transforms conjugate on entry and
-// exit.  Blocks are persistent and prefetch the next tile's 32 loads while
-// transforming the current one.
+// bytes per row) along a strided axis of length N2 = 32*R1 (R1 = 16 or 32),
+// register resident: thread (c = tid&15, r = tid>>4) holds rows r + R1*m
+// (m < 32) of column c, does a DFT-32 over m, multiplies by W_N2^{r m'}, and
+// after one LDS exchange the thread (c, g) does the DFT-R1 over r for
+// m' = g + R1*h, giving output rows m' + 32 r'.  Inverse transforms conjugate
+// on entry and exit.  Blocks are persistent over tiles.
 
-constexpr int CF_PITCH = 16 * 16 + 8;        // complex per m' slab ([r][c] + pad)
-constexpr int CF_LDS = 32 * CF_PITCH;        // exchange buffer (complex)
+template <int R1>
+struct CF {
+  static constexpr int N2 = 32 * R1;
+  static constexpr int THREADS = 16 * R1;
+  static constexpr int PITCH = R1 * 16 + 8;  // complex per m' slab ([r][c] + pad)
+  static constexpr int LDS = 32 * PITCH;
+  static constexpr int MPER = 32 / R1;       // m' values per thread in stage 2
+};
 
+template <int R1>
 __device__ __forceinline__ void cf_stage1(c32 (&x)[32], int r, const c32* twl, c32* ex, int c, bool inv) {
   if (inv) {
 #pragma unroll
@@ -199,24 +207,26 @@ __device__ __forceinline__ void cf_stage1(c32 (&x)[32], int r, const c32* twl, c
   }
   dft32(x);
 #pragma unroll
-  for (int m = 1; m < 32; ++m) x[m] = cmul(x[m], twl[r * m]);  // W_512^{r m'}
+  for (int m = 1; m < 32; ++m) x[m] = cmul(x[m], twl[r * m]);  // W_N2^{r m'}
 #pragma unroll
-  for (int m = 0; m < 32; ++m) ex[m * CF_PITCH + r * 16 + c] = x[m];
+  for (int m = 0; m < 32; ++m) ex[m * CF<R1>::PITCH + r * 16 + c] = x[m];
 }
 
-// stage 2 for m' = g + 16 h: y[r'] is output row m' + 32 r'
-__device__ __forceinline__ void cf_stage2(c32 (&y)[16], const c32* ex, int c, int mp, bool inv) {
+// stage 2 for one m': y[r'] is output row m' + 32 r'
+template <int R1>
+__device__ __forceinline__ void cf_stage2(c32 (&y)[R1], const c32* ex, int c, int mp, bool inv) {
 #pragma unroll
-  for (int r = 0; r < 16; ++r) y[r] = ex[mp * CF_PITCH + r * 16 + c];
-  dft16(y);
+  for (int r = 0; r < R1; ++r) y[r] = ex[mp * CF<R1>::PITCH + r * 16 + c];
+  dft<float, R1>(y);
   if (inv) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) y[r].y = -y[r].y;
+    for (int r = 0; r < R1; ++r) y[r].y = -y[r].y;
   }
 }
 
-__device__ __forceinline__ void cf_load_twiddles(c32* twl, const c32* __restrict__ tw512) {
-  for (int i = threadIdx.x; i < 512; i += 256) twl[i] = tw512[i];
+template <int N>
+__device__ __forceinline__ void cf_load_twiddles(c32* twl, const c32* __restrict__ tw, int nthreads) {
+  for (int i = threadIdx.x; i < N; i += nthreads) twl[i] = tw[i];
 }
 
 // ---------------------------------------------------------------- k1: input pass A
@@ -225,70 +235,48 @@ struct K1Args {
   const float* spec;  // Nf x Nt, column stride ldspec floats
   int64_t ldspec;
   c32* G;             // [Nt][N2][N1]
-  const c32* tw2;     // W_512^m
+  const c32* tw2;     // W_N2^m
   int logN1;
   int64_t M;
   int ntiles;         // Nt * N1/16
 };
 
-template <bool PREFETCH>
-__global__ __launch_bounds__(256, 2) void k1_input(K1Args a) {
-  __shared__ c32 ex[CF_LDS];
-  __shared__ c32 twl[512];
-  cf_load_twiddles(twl, a.tw2);
+template <int R1>
+__global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k1_input(K1Args a) {
+  using C = CF<R1>;
+  __shared__ c32 ex[C::LDS];
+  __shared__ c32 twl[C::N2];
+  cf_load_twiddles<C::N2>(twl, a.tw2, C::THREADS);
   const int c = threadIdx.x & 15, r = threadIdx.x >> 4;
   const int N1 = 1 << a.logN1;
   const int nblk = N1 >> 4;
-  const float invM = 1.0f / static_cast<float>(a.M);
   int tile = blockIdx.x;
   if (tile >= a.ntiles) return;
-  auto src = [&](int tl) {
-    const int t = tl / nblk, nb = tl - t * nblk;
-    return reinterpret_cast<const c32*>(a.spec + static_cast<int64_t>(t) * a.ldspec) + (nb << 4) + c;
-  };
-  c32 x[32], xn[PREFETCH ? 32 : 1];
-  {
-    const c32* z = src(tile);
-#pragma unroll
-    for (int m = 0; m < 32; ++m) x[m] = z[static_cast<int64_t>(r + 16 * m) << a.logN1];
-  }
   __syncthreads();  // twiddle table
   for (;;) {
-    const int next = tile + gridDim.x;
-    if (PREFETCH && next < a.ntiles) {
-      const c32* z = src(next);
-#pragma unroll
-      for (int m = 0; m < 32; ++m) xn[m] = z[static_cast<int64_t>(r + 16 * m) << a.logN1];
-    }
-    cf_stage1(x, r, twl, ex, c, false);
-    __syncthreads();
     const int t = tile / nblk, nb = tile - t * nblk;
-    const int n1 = (nb << 4) + c;
-    c32* G = a.G + static_cast<int64_t>(t) * FAST_N2 * N1 + n1;
+    c32 x[32];
+    {
+      const c32* z = reinterpret_cast<const c32*>(a.spec + static_cast<int64_t>(t) * a.ldspec);
+      const unsigned o = (nb << 4) + c;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int mp = r + 16 * h;  // thread (c, g=r) takes m' = g and g + 16
-      c32 y[16];
-      cf_stage2(y, ex, c, mp, false);
+      for (int m = 0; m < 32; ++m) x[m] = z[(static_cast<unsigned>(r + R1 * m) << a.logN1) + o];
+    }
+    cf_stage1<R1>(x, r, twl, ex, c, false);
+    __syncthreads();
+    const unsigned n1 = (nb << 4) + c;
+    c32* G = a.G + static_cast<int64_t>(t) * C::N2 * N1;  // uniform base, 32-bit offsets
+#pragma unroll 1
+    for (int h = 0; h < C::MPER; ++h) {
+      const unsigned mp = r + R1 * h;
+      c32 y[R1];
+      cf_stage2<R1>(y, ex, c, mp, false);
 #pragma unroll
-      for (int rp = 0; rp < 16; ++rp) {
-        const int k2 = mp + 32 * rp;
-        const int64_t m = (static_cast<int64_t>(n1) * k2) & (a.M - 1);
-        const c32 w = cis_rev(-static_cast<float>(m) * invM);  // W_M^{n1 k2}
-        G[static_cast<int64_t>(k2) * N1] = cmul(y[rp], w);
-      }
+      for (int rp = 0; rp < R1; ++rp) G[((mp + 32 * rp) << a.logN1) + n1] = y[rp];
     }
     __syncthreads();
-    if (next >= a.ntiles) break;
-    if (PREFETCH) {
-#pragma unroll
-      for (int m = 0; m < 32; ++m) x[m] = xn[m];
-    } else {
-      const c32* z = src(next);
-#pragma unroll
-      for (int m = 0; m < 32; ++m) x[m] = z[static_cast<int64_t>(r + 16 * m) << a.logN1];
-    }
-    tile = next;
+    tile += gridDim.x;
+    if (tile >= a.ntiles) break;
   }
 }
 
@@ -296,6 +284,7 @@ __global__ __launch_bounds__(256, 2) void k1_input(K1Args a) {
 
 struct K3Args {
   const c32* B;  // [Nr][N2][N1]
+  const c32* yhalf;  // [Nr] bin M/2 term: B[j][0][n1] += yhalf[j] (-1)^{n1}
   float* dest;   // Nf x Nr, column stride ld floats
   int64_t ld;
   const c32* tw2;
@@ -304,60 +293,48 @@ struct K3Args {
   int ntiles;    // Nr * N1/16
 };
 
-template <bool PREFETCH>
-__global__ __launch_bounds__(256, 2) void k3_output(K3Args a) {
-  __shared__ c32 ex[CF_LDS];
-  __shared__ c32 twl[512];
-  cf_load_twiddles(twl, a.tw2);
+template <int R1>
+__global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_output(K3Args a) {
+  using C = CF<R1>;
+  __shared__ c32 ex[C::LDS];
+  __shared__ c32 twl[C::N2];
+  cf_load_twiddles<C::N2>(twl, a.tw2, C::THREADS);
   const int c = threadIdx.x & 15, r = threadIdx.x >> 4;
   const int N1 = 1 << a.logN1;
   const int nblk = N1 >> 4;
   int tile = blockIdx.x;
   if (tile >= a.ntiles) return;
-  // rates last to first: the most recently written spectra are read first
-  auto jof = [&](int tl) { return a.Nr - 1 - tl / nblk; };
-  auto src = [&](int tl) {
-    const int jr = tl / nblk, nb = tl - jr * nblk;
-    return a.B + static_cast<int64_t>(a.Nr - 1 - jr) * FAST_N2 * N1 + (nb << 4) + c;
-  };
-  c32 x[32], xn[PREFETCH ? 32 : 1];
-  {
-    const c32* b = src(tile);
-#pragma unroll
-    for (int m = 0; m < 32; ++m) x[m] = b[static_cast<int64_t>(r + 16 * m) * N1];
-  }
   __syncthreads();
   for (;;) {
-    const int next = tile + gridDim.x;
-    if (PREFETCH && next < a.ntiles) {
-      const c32* b = src(next);
+    // rates last to first: the most recently written spectra are read first
+    const int jr = tile / nblk, nb = tile - jr * nblk;
+    const int j = a.Nr - 1 - jr;
+    c32 x[32];
+    {
+      const c32* b = a.B + static_cast<int64_t>(j) * C::N2 * N1;
+      const unsigned o = (nb << 4) + c;
 #pragma unroll
-      for (int m = 0; m < 32; ++m) xn[m] = b[static_cast<int64_t>(r + 16 * m) * N1];
+      for (int m = 0; m < 32; ++m) x[m] = b[(static_cast<unsigned>(r + R1 * m) << a.logN1) + o];
     }
-    cf_stage1(x, r, twl, ex, c, true);
+    if (r == 0) {
+      const c32 h = a.yhalf[j];
+      x[0] = (c & 1) ? csub(x[0], h) : cadd(x[0], h);
+    }
+    cf_stage1<R1>(x, r, twl, ex, c, true);
     __syncthreads();
-    const int j = jof(tile);
-    const int nb = tile % nblk;
-    c32* out = reinterpret_cast<c32*>(a.dest + static_cast<int64_t>(j) * a.ld) + (nb << 4) + c;
+    c32* out = reinterpret_cast<c32*>(a.dest + static_cast<int64_t>(j) * a.ld);
+    const unsigned o = (nb << 4) + c;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int mp = r + 16 * h;
-      c32 y[16];
-      cf_stage2(y, ex, c, mp, true);
+    for (int h = 0; h < C::MPER; ++h) {
+      const unsigned mp = r + R1 * h;
+      c32 y[R1];
+      cf_stage2<R1>(y, ex, c, mp, true);
 #pragma unroll
-      for (int rp = 0; rp < 16; ++rp) out[static_cast<int64_t>(mp + 32 * rp) << a.logN1] = y[rp];
+      for (int rp = 0; rp < R1; ++rp) out[((mp + 32 * rp) << a.logN1) + o] = y[rp];
     }
     __syncthreads();
-    if (next >= a.ntiles) break;
-    if (PREFETCH) {
-#pragma unroll
-      for (int m = 0; m < 32; ++m) x[m] = xn[m];
-    } else {
-      const c32* b = src(next);
-#pragma unroll
-      for (int m = 0; m < 32; ++m) x[m] = b[static_cast<int64_t>(r + 16 * m) * N1];
-    }
-    tile = next;
+    tile += gridDim.x;
+    if (tile >= a.ntiles) break;
   }
 }
 
@@ -368,6 +345,7 @@ struct K2Args {
   c32* B;            // [Nr][N2][N1]
   const c32* tw1;    // W_N1^m
   const c32* tabq;   // e^{i pi dr j}, j < Nr
+  c32* yhalf;        // [Nr] c2r pre-pass value of the bin M/2, added by k3
   int Nt, Nr;
   int logN1;
   int64_t M, Nf;
@@ -404,19 +382,26 @@ __device__ __forceinline__ c32 horner(const c32* a, c32 z) {
   return acc;
 }
 
+// k2 occupancy target: two 512-thread blocks per CU (4 waves per SIMD) for
+// N1 = 512, whose LDS (2 x 8 rows x 576 + 512 entries, 78 KB) allows it.
+constexpr int k2_waves_per_eu(int n1) { return n1 >= 512 ? 4 : 2; }
+constexpr int k2_rows(int n1) { return n1 >= 512 ? 8 : 16; }
+
 // Block of T = min(N1, 512) threads; each thread owns PP = N1/T Hermitian
 // pairs q = tid + i*T (i < PP).  Pair q of a regular block (columns c, N2-c)
 // is (k = c + N2 q, M - k = (N2-c) + N2 (N1-1-q)); the block for p = 0 holds the
 // self-paired columns 0 and N2/2: q < N1/2 pairs (N2 q, N2 (N1-q)) in column 0
 // (q = 0 pairs bin 0 with the Nyquist bin M), q >= N1/2 pairs within column
-// N2/2.  The one bin left over, M/2 (column 0, k1 = N1/2), is evaluated by the
-// lanes of the block's last wave, one rate each.
-template <int N1, int NT, int ROWS>
-__global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
+// N2/2.  The one bin left over, M/2 (column 0, k1 = N1/2), enters the c2r
+// pre-pass at row 0, k1 = N1/2, hence B[j][0][n1] as C_j (-1)^{n1}: block 0
+// evaluates C_j for every rate up front into yhalf[j] and k3 adds it, which
+// keeps its registers out of the rate loop.  The window weight and output
+// scale are folded into the Vandermonde coefficients a_t.
+template <int N1, int N2, int NT, int ROWS>
+__global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_czt(K2Args a) {
   constexpr int T = N1 < 512 ? N1 : 512;
   constexpr int PP = N1 / T;
   constexpr int P = N1 + N1 / 8;  // swizzled row pitch (see lsw)
-  constexpr int N2 = FAST_N2;
   constexpr int HALF = N1 / 2;
   __shared__ c32 buf[ROWS * P];
   __shared__ c32 buf2[ROWS * P];
@@ -461,11 +446,28 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
     kA[i] = cA + static_cast<int64_t>(N2) * k1A[i];
     WA[i] = cis_rev(-static_cast<float>(kA[i]) * invNf);  // W_Nf^{kA}
   }
-  const bool half_lane = special && tid >= T - 64;
-  const int hl = tid - (T - 64);
   if (special && tid < NT) ahalf[tid] = {0.0f, 0.0f};  // rows u >= Nt stay zero
 
-  // ---- input: DFT_N1 over n1, r2c untangle, pre-rotation -> aA, aB
+  // window weight and c2r scale, folded into the coefficients
+  float wA[PP], wB[PP];
+#pragma unroll
+  for (int i = 0; i < PP; ++i) {
+    wA[i] = fast_window(a.wkind, a.wcustom, kA[i], a.Nf) * a.oscale;
+    wB[i] = fast_window(a.wkind, a.wcustom, a.M - kA[i], a.Nf) * a.oscale;
+  }
+
+  // four-step twiddles W_M^{-n1 k2} of the two columns (n1 = tid + i*T):
+  // conjugated on input, as they are on output
+  const float invM = 1.0f / static_cast<float>(a.M);
+  c32 twS0[PP], twS1[PP];
+#pragma unroll
+  for (int i = 0; i < PP; ++i) {
+    const int64_t n1 = tid + i * T;
+    twS0[i] = cis_rev(static_cast<float>((n1 * colA) & (a.M - 1)) * invM);
+    twS1[i] = cis_rev(static_cast<float>((n1 * colB) & (a.M - 1)) * invM);
+  }
+
+  // ---- input: W_M^{n1 k2}, DFT_N1 over n1, r2c untangle, pre-rotation -> aA, aB
   c32 aA[PP][NT], aB[PP][NT];
 #pragma unroll
   for (int i = 0; i < PP; ++i)
@@ -497,7 +499,8 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
       for (int r = 0; r < RB; ++r) {
         if (r0 + r < 2 * nt) {
 #pragma unroll
-          for (int i = 0; i < PP; ++i) buf[(r0 + r) * P + lsw(tid + i * T)] = v[r][i];
+          for (int i = 0; i < PP; ++i)
+            buf[(r0 + r) * P + lsw(tid + i * T)] = cmulc(v[r][i], ((r0 + r) & 1) ? twS1[i] : twS0[i]);
         }
       }
     }
@@ -526,39 +529,40 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
           }
           // pre-rotation e^{2 pi i k u r0/Nf}; partner: e^{i pi u r0} conj(.)
           const c32 rot = cis_nh(static_cast<double>(kA[i]) * ud, a.h0);
-          aA[i][u] = cmul(XA, rot);
-          aB[i][u] = cmul(XB, cmulc(rq, rot));
+          const c32 ra = cmul(XA, rot), rb = cmul(XB, cmulc(rq, rot));
+          aA[i][u] = {ra.x * wA[i], ra.y * wA[i]};
+          aB[i][u] = {rb.x * wB[i], rb.y * wB[i]};
         }
         if (special && tid == 0) {
           // bin M/2: X = conj(Z[M/2]); rotation e^{2 pi i (M/2) u r0/Nf} = e^{i pi u r0/2}
+          const float wH = fast_window(a.wkind, a.wcustom, a.M / 2, a.Nf) * a.oscale;
           const c32 Zh = fz[(2 * uu) * P + lsw(HALF)];
-          ahalf[u] = cmul(cconj(Zh), cis_nh(ud, 0.5 * a.r0half));
+          const c32 h = cmul(cconj(Zh), cis_nh(ud, 0.5 * a.r0half));
+          ahalf[u] = {h.x * wH, h.y * wH};
         }
       }
     }
     __syncthreads();
   }
 
+  if (special) {
+    // bin M/2 (self-paired): gamma = e^{i pi dr/2}; its c2r pre-pass value is
+    // conj(Ys) for every rate
+    for (int j = tid; j < a.Nr; j += T) {
+      const c32 zh = cis_nh(static_cast<double>(j), 0.25 * a.hd2 * static_cast<double>(a.Nf));
+      c32 acc = ahalf[NT - 1];
+      for (int u = NT - 2; u >= 0; --u) acc = cfma(ahalf[u], acc, zh);
+      a.yhalf[j] = cconj(acc);
+    }
+  }
+
   // ---- chirp-Z for every rate, c2r pre-pass, inverse DFT_N1, store
-  float wA[PP], wB[PP];
   c64 gam[PP], z[PP];
 #pragma unroll
   for (int i = 0; i < PP; ++i) {
-    wA[i] = fast_window(a.wkind, a.wcustom, kA[i], a.Nf) * a.oscale;
-    wB[i] = fast_window(a.wkind, a.wcustom, a.M - kA[i], a.Nf) * a.oscale;
     gam[i] = cis64f(static_cast<double>(kA[i]) * a.hd2);
     z[i] = {1.0, 0.0};
   }
-  // final twiddles W_M^{-n1 k2} for the store role (n1 = tid + i*T)
-  const float invM = 1.0f / static_cast<float>(a.M);
-  c32 twS0[PP], twS1[PP];
-#pragma unroll
-  for (int i = 0; i < PP; ++i) {
-    const int64_t n1 = tid + i * T;
-    twS0[i] = cis_rev(static_cast<float>((n1 * colA) & (a.M - 1)) * invM);
-    twS1[i] = cis_rev(static_cast<float>((n1 * colB) & (a.M - 1)) * invM);
-  }
-  const float wH = special ? fast_window(a.wkind, a.wcustom, a.M / 2, a.Nf) * a.oscale : 0.0f;
 
   constexpr int JG = ROWS / 2;
   for (int j0 = 0; j0 < a.Nr; j0 += JG) {
@@ -570,13 +574,13 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
         const c32 zA = {static_cast<float>(z[i].x), static_cast<float>(z[i].y)};
         z[i] = cmul(z[i], gam[i]);
         const c32 zB = cmulc(qj, zA);  // e^{i pi dr j} conj(zA)
-        c32 YA = horner<NT>(aA[i], zA);
-        c32 YB = horner<NT>(aB[i], zB);
-        YA = {YA.x * wA[i], YA.y * wA[i]};
-        YB = {YB.x * wB[i], YB.y * wB[i]};
+        const c32 YA = horner<NT>(aA[i], zA);
+        const c32 YB = horner<NT>(aB[i], zB);
         if (nyq[i]) {
-          // pair (0, M): only the real parts enter the c2r
+          // pair (0, M): only the real parts enter the c2r; the bin M/2 slot
+          // of the row is zero here (k3 adds it)
           buf[(2 * jj + rowA[i]) * P] = {0.5f * (YA.x + YB.x), 0.5f * (YA.x - YB.x)};
+          buf[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
         } else {
           const c32 E = {0.5f * (YA.x + YB.x), 0.5f * (YA.y - YB.y)};
           const c32 D = {0.5f * (YA.x - YB.x), 0.5f * (YA.y + YB.y)};
@@ -585,15 +589,6 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
           buf[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
         }
       }
-    }
-    if (half_lane && hl < nj) {
-      // bin M/2 (self-paired): gamma = e^{i pi dr/2}, c2r pre gives conj(Ys)
-      c32 ah[NT];
-#pragma unroll
-      for (int u = 0; u < NT; ++u) ah[u] = ahalf[u];
-      const c32 zh = cis_nh(static_cast<double>(j0 + hl), 0.25 * a.hd2 * static_cast<double>(a.Nf));
-      const c32 Y = horner<NT>(ah, zh);
-      buf[(2 * hl) * P + lsw(HALF)] = {Y.x * wH, -Y.y * wH};
     }
     __syncthreads();
     const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, twl);
@@ -610,9 +605,23 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512) void k2_czt(K2Args a) {
 
 // ---------------------------------------------------------------- F unpack
 
-// natural F[k, t] (k <= M) from Z[t][k2][k1] (the DFT_N1 of G over n1)
+// Gt[t][k2][n1] = G[t][k2][n1] W_M^{n1 k2} (the four-step twiddle k2 applies
+// on load), for the generic DFT_N1 that rebuilds F
+__global__ __launch_bounds__(256) void twiddle_G(const c32* __restrict__ G, c32* __restrict__ Gt, int64_t total,
+                                                 int logN1, int logN2) {
+  const int64_t M = int64_t(1) << (logN1 + logN2);
+  const float invM = 1.0f / static_cast<float>(M);
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t n1 = g & ((int64_t(1) << logN1) - 1);
+    const int64_t k2 = (g >> logN1) & ((int64_t(1) << logN2) - 1);
+    Gt[g] = cmul(G[g], cis_rev(-static_cast<float>((n1 * k2) & (M - 1)) * invM));
+  }
+}
+
+// natural F[k, t] (k <= M) from Z[t][k2][k1] (the DFT_N1 of Gt over n1)
 __global__ __launch_bounds__(256) void unpack_F(const c32* __restrict__ Z, c32* __restrict__ F, int Nt, int logN1,
-                                                int64_t M, int64_t Nf) {
+                                                int logN2, int64_t M, int64_t Nf) {
   const int64_t Nk = M + 1;
   const int64_t total = Nk * Nt;
   for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
@@ -621,7 +630,7 @@ __global__ __launch_bounds__(256) void unpack_F(const c32* __restrict__ Z, c32* 
     const int t = static_cast<int>(g / Nk);
     const c32* Zt = Z + static_cast<int64_t>(t) * M;
     auto at = [&](int64_t kk) {  // Z[kk] for kk in [0, M)
-      const int64_t k2 = kk & (FAST_N2 - 1), k1 = kk >> 9;
+      const int64_t k2 = kk & ((int64_t(1) << logN2) - 1), k1 = kk >> logN2;
       return Zt[(k2 << logN1) + k1];
     };
     c32 X;
@@ -648,13 +657,14 @@ namespace zdt {
 
 struct FastPlan {
   int64_t Nf = 0, Nt = 0, Nr = 0, M = 0;
-  int N1 = 0, logN1 = 0, NT = 0;
+  int N1 = 0, logN1 = 0, N2 = 0, logN2 = 0, NT = 0;
   float dr = 0;
   c32* G = nullptr;     // [Nt][N2][N1]
   c32* B = nullptr;     // [Nr][N2][N1]
   c32* tw1 = nullptr;   // W_N1^m
   c32* tw2 = nullptr;   // W_N2^m
   c32* tabq = nullptr;  // e^{i pi dr j}
+  c32* yhalf = nullptr;  // [Nr] bin M/2 term of the c2r (k2 -> k3)
   bool have_B = false;
   int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
 };
@@ -674,8 +684,9 @@ bool fast_supported(int64_t Nf, int64_t Nt, int64_t Nr) {
 
 int64_t fast_bytes(int64_t Nf, int64_t Nt, int64_t Nr) {
   const int64_t M = Nf / 2;
-  const int64_t N1 = M / FAST_N2;
-  return 8 * M * (Nt + Nr) + 8 * (N1 + FAST_N2 + Nr);
+  const int64_t N2 = fast_n2(Nf);
+  const int64_t N1 = M / N2;
+  return 8 * M * (Nt + Nr) + 8 * (N1 + N2 + 2 * Nr);
 }
 
 static int upload_table(c32** dst, const std::vector<c32>& h) {
@@ -697,7 +708,7 @@ static std::vector<c32> twiddles(int N) {
 void fast_destroy(FastPlan* p) {
   if (p == nullptr) return;
   for (void* q : {static_cast<void*>(p->G), static_cast<void*>(p->B), static_cast<void*>(p->tw1),
-                  static_cast<void*>(p->tw2), static_cast<void*>(p->tabq)})
+                  static_cast<void*>(p->tw2), static_cast<void*>(p->tabq), static_cast<void*>(p->yhalf)})
     if (q) (void)hipFree(q);
   delete p;
 }
@@ -710,7 +721,9 @@ int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, FastPlan** out) {
   p->Nt = Nt;
   p->Nr = Nr;
   p->M = Nf / 2;
-  p->N1 = static_cast<int>(p->M / FAST_N2);
+  p->N2 = fast_n2(Nf);
+  p->logN2 = ilog2i(p->N2);
+  p->N1 = static_cast<int>(p->M / p->N2);
   p->logN1 = ilog2i(p->N1);
   p->NT = Nt <= 2 ? 2 : Nt <= 4 ? 4 : Nt <= 8 ? 8 : 16;
   p->dr = dr;
@@ -730,8 +743,9 @@ int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, FastPlan** out) {
   };
   alloc(&p->G, static_cast<size_t>(Nt * p->M));
   alloc(&p->B, static_cast<size_t>(Nr * p->M));
+  alloc(&p->yhalf, static_cast<size_t>(Nr));
   if (rc == ZDT_OK) rc = upload_table(&p->tw1, twiddles(p->N1));
-  if (rc == ZDT_OK) rc = upload_table(&p->tw2, twiddles(FAST_N2));
+  if (rc == ZDT_OK) rc = upload_table(&p->tw2, twiddles(p->N2));
   if (rc == ZDT_OK) {
     std::vector<c32> q(Nr);
     const long double pi = 3.14159265358979323846264338327950288L;
@@ -760,29 +774,35 @@ int fast_input(FastPlan* p, const float* spec, hipStream_t st) {
   a.logN1 = p->logN1;
   a.M = p->M;
   a.ntiles = static_cast<int>(p->Nt * (p->N1 / 16));
-  const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
   ProfScope ps(st, "k1_input", 4.0 * p->Nf * p->Nt + 8.0 * p->M * p->Nt);
-  k1_input<false><<<grid, 256, 0, st>>>(a);
+  if (p->N2 == 1024) {
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
+    k1_input<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
+  } else {
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
+    k1_input<16><<<grid, CF<16>::THREADS, 0, st>>>(a);
+  }
   ZDT_CHECK_LAUNCH("k1_input");
   return ZDT_OK;
 }
 
-template <int N1>
+template <int N1, int N2>
 static int launch_k2(const K2Args& a, int NT, hipStream_t st) {
-  const unsigned grid = FAST_N2 / 2;
+  constexpr unsigned grid = N2 / 2;
   constexpr unsigned TB = N1 < 512 ? N1 : 512;
+  constexpr int RW = k2_rows(N1);
   switch (NT) {
     case 2:
-      k2_czt<N1, 2, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      k2_czt<N1, N2, 2, RW><<<grid, TB, 0, st>>>(a);
       break;
     case 4:
-      k2_czt<N1, 4, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      k2_czt<N1, N2, 4, RW><<<grid, TB, 0, st>>>(a);
       break;
     case 8:
-      k2_czt<N1, 8, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      k2_czt<N1, N2, 8, RW><<<grid, TB, 0, st>>>(a);
       break;
     default:
-      k2_czt<N1, 16, (N1 >= 1024) ? 8 : 16><<<grid, TB, 0, st>>>(a);
+      k2_czt<N1, N2, 16, RW><<<grid, TB, 0, st>>>(a);
       break;
   }
   ZDT_CHECK_LAUNCH("k2_czt");
@@ -795,6 +815,7 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
   a.B = p->B;
   a.tw1 = p->tw1;
   a.tabq = p->tabq;
+  a.yhalf = p->yhalf;
   a.Nt = static_cast<int>(p->Nt);
   a.Nr = static_cast<int>(p->Nr);
   a.logN1 = p->logN1;
@@ -808,24 +829,25 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
   a.oscale = static_cast<float>(scale / static_cast<double>(p->M));
   {
     ProfScope ps(st, "k2_czt", 8.0 * p->M * (p->Nt + p->Nr));
-    switch (p->N1) {
-      case 64:
-        ZDT_TRY(launch_k2<64>(a, p->NT, st));
+    const int key = p->N1 * 2048 + p->N2;
+    switch (key) {
+      case 64 * 2048 + 512:
+        ZDT_TRY((launch_k2<64, 512>(a, p->NT, st)));
         break;
-      case 128:
-        ZDT_TRY(launch_k2<128>(a, p->NT, st));
+      case 64 * 2048 + 1024:
+        ZDT_TRY((launch_k2<64, 1024>(a, p->NT, st)));
         break;
-      case 256:
-        ZDT_TRY(launch_k2<256>(a, p->NT, st));
+      case 128 * 2048 + 1024:
+        ZDT_TRY((launch_k2<128, 1024>(a, p->NT, st)));
         break;
-      case 512:
-        ZDT_TRY(launch_k2<512>(a, p->NT, st));
+      case 256 * 2048 + 1024:
+        ZDT_TRY((launch_k2<256, 1024>(a, p->NT, st)));
         break;
-      case 1024:
-        ZDT_TRY(launch_k2<1024>(a, p->NT, st));
+      case 512 * 2048 + 1024:
+        ZDT_TRY((launch_k2<512, 1024>(a, p->NT, st)));
         break;
       default:
-        return fail(ZDT_ENOTSUP, "fused path: N1=%d", p->N1);
+        return fail(ZDT_ENOTSUP, "fused path: N1=%d N2=%d", p->N1, p->N2);
     }
   }
   p->have_B = true;
@@ -838,15 +860,21 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st) {
   if (!p->have_B) return fail(ZDT_ESTATE, "output!: no post-processed spectra");
   K3Args a;
   a.B = p->B;
+  a.yhalf = p->yhalf;
   a.dest = dest;
   a.ld = p->Nf;
   a.tw2 = p->tw2;
   a.logN1 = p->logN1;
   a.Nr = static_cast<int>(p->Nr);
   a.ntiles = static_cast<int>(p->Nr * (p->N1 / 16));
-  const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
   ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
-  k3_output<false><<<grid, 256, 0, st>>>(a);
+  if (p->N2 == 1024) {
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
+    k3_output<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
+  } else {
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
+    k3_output<16><<<grid, CF<16>::THREADS, 0, st>>>(a);
+  }
   ZDT_CHECK_LAUNCH("k3_output");
   return ZDT_OK;
 }
@@ -855,27 +883,36 @@ int fast_unpack_F(FastPlan* p, c32* F, hipStream_t st) {
   const FftPlan* fp = nullptr;
   ZDT_TRY(fft_plan_get(p->N1, false, &fp));
   const size_t n = static_cast<size_t>(p->Nt * p->M);
-  void *Z = nullptr, *s0 = nullptr, *s1 = nullptr;
+  void *Z = nullptr, *s0 = nullptr, *s1 = nullptr, *Gt = nullptr;
   int rc = ZDT_OK;
   if (hipMallocAsync(&Z, n * sizeof(c32), st) != hipSuccess || hipMallocAsync(&s0, n * sizeof(c32), st) != hipSuccess ||
-      hipMallocAsync(&s1, n * sizeof(c32), st) != hipSuccess) {
+      hipMallocAsync(&s1, n * sizeof(c32), st) != hipSuccess || hipMallocAsync(&Gt, n * sizeof(c32), st) != hipSuccess) {
     (void)hipGetLastError();
     rc = fail(ZDT_ENOMEM, "F unpack: allocation failed");
   }
-  // DFT_N1 over n1 (contiguous rows of G): Z[t][k2][k1]
+  if (rc == ZDT_OK) {
+    int64_t g = (static_cast<int64_t>(n) + 255) / 256;
+    if (g > (1 << 20)) g = 1 << 20;
+    twiddle_G<<<static_cast<unsigned>(g), 256, 0, st>>>(p->G, static_cast<c32*>(Gt), static_cast<int64_t>(n), p->logN1,
+                                                         p->logN2);
+    if (hipGetLastError() != hipSuccess) rc = fail(ZDT_EHIP, "twiddle_G launch failed");
+  }
+  // DFT_N1 over n1 (contiguous rows of Gt): Z[t][k2][k1]
   if (rc == ZDT_OK)
-    rc = fft_c2c(fp, p->G, 1, p->N1, Z, 1, p->N1, static_cast<int64_t>(p->Nt) * FAST_N2, false, 1.0, s0, s1, st);
+    rc = fft_c2c(fp, static_cast<c32*>(Gt), 1, p->N1, Z, 1, p->N1, static_cast<int64_t>(p->Nt) * p->N2, false, 1.0, s0,
+                 s1, st);
   if (rc == ZDT_OK) {
     const int64_t total = (p->M + 1) * p->Nt;
     int64_t g = (total + 255) / 256;
     if (g > (1 << 20)) g = 1 << 20;
     unpack_F<<<static_cast<unsigned>(g), 256, 0, st>>>(static_cast<const c32*>(Z), F, static_cast<int>(p->Nt),
-                                                        p->logN1, p->M, p->Nf);
+                                                        p->logN1, p->logN2, p->M, p->Nf);
     if (hipGetLastError() != hipSuccess) rc = fail(ZDT_EHIP, "unpack_F launch failed");
   }
   if (Z) (void)hipFreeAsync(Z, st);
   if (s0) (void)hipFreeAsync(s0, st);
   if (s1) (void)hipFreeAsync(s1, st);
+  if (Gt) (void)hipFreeAsync(Gt, st);
   return rc;
 }
 
