@@ -11,9 +11,9 @@
 //   k2_czt     one block per Hermitian column pair (k2, N2-k2): times
 //              W_M^{n1 k2}, DFT_N1 over n1 -> Z[k] -> r2c untangle -> X[k,t] (= F of the reference) ->
 //              chirp-Z over t for every rate (preprocess!/convolve!/
-//              postprocess!) -> c2r pre-pass -> inverse DFT_N1 over k1, times
-//              W_M^{-n1 k2} -> B[j][k2][n1]
-//   k3_output  for each (j, n1): inverse DFT_N2 over k2 (column FFT in LDS)
+//              postprocess!) -> c2r pre-pass -> inverse DFT_N1 over k1
+//              -> B[j][k2][n1]
+//   k3_output  for each (j, n1): times W_M^{-n1 k2}, inverse DFT_N2 over k2
 //              -> z'[n1 + N1 n2] = FDR[2n, j] + i FDR[2n+1, j]  (output!)
 //
 // The chirp-Z transform per bin k is evaluated in its Vandermonde form
@@ -44,6 +44,13 @@ constexpr int fast_n2(int64_t Nf) { return Nf >= (int64_t(1) << 17) ? 1024 : 512
 // ---------------------------------------------------------------- LDS FFT
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// radix of the pass at stride NS in the LDS FFTs: 1024 = 16*8*8,
+// 512 = 8*8*8, 256 = 16*16, 128 = 16*8, 64 = 8*8
+template <int N, int NS>
+constexpr int fft_radix() {
+  return (N / NS == 16 || (NS == 1 && ilog2c(N) % 3 != 0)) ? 16 : (N / NS >= 8 ? 8 : N / NS);
+}
 
 // One Stockham radix-R pass over nrows length-N FFTs held in LDS (row f at
 // buf + f*P).  All threads read their butterflies' inputs, a barrier
@@ -171,12 +178,94 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
 template <int N, int T, bool INV, int NS = 1>
 __device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, const c32* __restrict__ tw) {
   if constexpr (NS < N) {
-    constexpr int REM = N / NS;
-    constexpr int R = (REM == 16 || (NS == 1 && ilog2c(N) % 3 != 0)) ? 16 : (REM >= 8 ? 8 : REM);
+    constexpr int R = fft_radix<N, NS>();
     lds_pass_pp<N, R, T, INV>(a, b, P, nrows, NS, tw);
     return lds_fft_pp<N, T, INV, NS * R>(b, a, P, nrows, tw);
   } else {
     return a;
+  }
+}
+
+// Wavefront-level passes.  The rows [0, nrw) at `rows` (pitch P, swizzled)
+// belong to one wavefront, and lane b < nrw*N/R owns butterfly b of each
+// pass.  A wavefront issues all its reads of a pass before any of its
+// writes and its LDS accesses complete in order, so the passes run in place
+// with no workgroup barrier; wave_sync only keeps the compiler from moving
+// LDS accesses across pass boundaries.  Requires nrw*N/R <= 64 for every
+// radix R of the schedule.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int N, int R, bool INV>
+__device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, const c32* __restrict__ tw, int lane,
+                                          c32 (&v)[R], bool last) {
+  constexpr int NR = N / R;
+  constexpr int LOGNR = ilog2c(NR);
+  constexpr int LOGN = ilog2c(N);
+  constexpr int LOGR = ilog2c(R);
+  static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
+  if (lane < nrw * NR) {
+    const int logNs = 31 - __builtin_clz(Ns);
+    const int f = lane >> LOGNR;
+    const int j = lane & (NR - 1);
+    int k = j & (Ns - 1);
+    asm volatile("" : "+v"(k));  // keeps twiddle addresses out of the caller's loop
+    c32* x = rows + f * P;
+    const c32* xr = x + lsw(j);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = xr[r * (NR + NR / 8)];
+    if (Ns > 1) {
+      const int sh = LOGN - logNs - LOGR;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const c32 w = tw[(r * k) << sh];
+        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+      }
+    }
+    dftd<INV, float, R>(v);
+    if (!last) {
+      const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
+      c32* yy = x + lsw(d0);
+      if (Ns == 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
+      }
+    }
+  }
+  wave_sync();
+}
+
+template <int N, int NS = 1>
+constexpr int fft_max_nr() {
+  if constexpr (NS >= N) {
+    return 0;
+  } else {
+    constexpr int R = fft_radix<N, NS>();
+    constexpr int a = N / R, b = fft_max_nr<N, NS * R>();
+    return a > b ? a : b;
+  }
+}
+
+// In-place inverse/forward FFT of the wavefront's rows; the last pass leaves
+// its butterfly outputs in registers (lane (f, j): row f, elements
+// j + r*N/R) and calls emit(f, j, v) instead of writing them back.
+template <int N, bool INV, int NS = 1, typename Emit>
+__device__ __forceinline__ void wave_fft(c32* rows, int P, int nrw, const c32* __restrict__ tw, int lane, Emit&& emit) {
+  constexpr int R = fft_radix<N, NS>();
+  c32 v[R];
+  if constexpr (NS * R < N) {
+    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, false);
+    wave_fft<N, INV, NS * R>(rows, P, nrw, tw, lane, emit);
+  } else {
+    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, true);
+    constexpr int NR = N / R;
+    if (lane < nrw * NR) emit(lane / NR, lane % NR, v);
   }
 }
 
@@ -252,18 +341,23 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k1_inp
   const int nblk = N1 >> 4;
   int tile = blockIdx.x;
   if (tile >= a.ntiles) return;
+  auto load = [&](int tl, c32(&x)[32]) {
+    const int t = tl / nblk, nb = tl - t * nblk;
+    const c32* z = reinterpret_cast<const c32*>(a.spec + static_cast<int64_t>(t) * a.ldspec);
+    const unsigned o = (nb << 4) + c;
+#pragma unroll
+    for (int m = 0; m < 32; ++m) x[m] = z[(static_cast<unsigned>(r + R1 * m) << a.logN1) + o];
+  };
+  c32 x[32];
+  load(tile, x);
   __syncthreads();  // twiddle table
   for (;;) {
     const int t = tile / nblk, nb = tile - t * nblk;
-    c32 x[32];
-    {
-      const c32* z = reinterpret_cast<const c32*>(a.spec + static_cast<int64_t>(t) * a.ldspec);
-      const unsigned o = (nb << 4) + c;
-#pragma unroll
-      for (int m = 0; m < 32; ++m) x[m] = z[(static_cast<unsigned>(r + R1 * m) << a.logN1) + o];
-    }
     cf_stage1<R1>(x, r, twl, ex, c, false);
     __syncthreads();
+    // the next tile's loads fly while this one finishes
+    const int next = tile + gridDim.x;
+    if (next < a.ntiles) load(next, x);
     const unsigned n1 = (nb << 4) + c;
     c32* G = a.G + static_cast<int64_t>(t) * C::N2 * N1;  // uniform base, 32-bit offsets
 #pragma unroll 1
@@ -275,7 +369,7 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k1_inp
       for (int rp = 0; rp < R1; ++rp) G[((mp + 32 * rp) << a.logN1) + n1] = y[rp];
     }
     __syncthreads();
-    tile += gridDim.x;
+    tile = next;
     if (tile >= a.ntiles) break;
   }
 }
@@ -304,36 +398,53 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
   const int nblk = N1 >> 4;
   int tile = blockIdx.x;
   if (tile >= a.ntiles) return;
+  // rates last to first: the most recently written spectra are read first
+  auto load = [&](int tl, c32(&x)[32], c32& h) {
+    const int jr = tl / nblk, nb = tl - jr * nblk;
+    const c32* b = a.B + static_cast<int64_t>(a.Nr - 1 - jr) * C::N2 * N1;
+    const unsigned o = (nb << 4) + c;
+#pragma unroll
+    for (int m = 0; m < 32; ++m) x[m] = b[(static_cast<unsigned>(r + R1 * m) << a.logN1) + o];
+    h = a.yhalf[a.Nr - 1 - jr];
+  };
+  c32 x[32], h;
+  load(tile, x, h);
   __syncthreads();
   for (;;) {
-    // rates last to first: the most recently written spectra are read first
     const int jr = tile / nblk, nb = tile - jr * nblk;
     const int j = a.Nr - 1 - jr;
-    c32 x[32];
+    if (r == 0) x[0] = (c & 1) ? csub(x[0], h) : cadd(x[0], h);
     {
-      const c32* b = a.B + static_cast<int64_t>(j) * C::N2 * N1;
-      const unsigned o = (nb << 4) + c;
+      // four-step twiddle W_M^{-n1 k2}, k2 = r + R1 m, as A[m mod 8] B[m / 8]
+      // (n1 k2 < M: every phase is an exact float)
+      const unsigned n1 = (nb << 4) + c;
+      const float invM = 1.0f / static_cast<float>(N1 * C::N2);
+      c32 A[8], Bq[4];
 #pragma unroll
-      for (int m = 0; m < 32; ++m) x[m] = b[(static_cast<unsigned>(r + R1 * m) << a.logN1) + o];
-    }
-    if (r == 0) {
-      const c32 h = a.yhalf[j];
-      x[0] = (c & 1) ? csub(x[0], h) : cadd(x[0], h);
+      for (int i = 0; i < 8; ++i) A[i] = cis_rev(static_cast<float>(n1 * (r + R1 * i)) * invM);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Bq[q] = cis_rev(static_cast<float>(n1 * (8 * R1 * q)) * invM);
+#pragma unroll
+      for (int m = 1; m < 32; ++m) x[m] = cmul(x[m], (m < 8) ? A[m] : cmul(A[m & 7], Bq[m >> 3]));
+      x[0] = cmul(x[0], A[0]);
     }
     cf_stage1<R1>(x, r, twl, ex, c, true);
     __syncthreads();
+    // the next tile's loads fly while this one finishes
+    const int next = tile + gridDim.x;
+    if (next < a.ntiles) load(next, x, h);
     c32* out = reinterpret_cast<c32*>(a.dest + static_cast<int64_t>(j) * a.ld);
     const unsigned o = (nb << 4) + c;
 #pragma unroll
-    for (int h = 0; h < C::MPER; ++h) {
-      const unsigned mp = r + R1 * h;
+    for (int hh = 0; hh < C::MPER; ++hh) {
+      const unsigned mp = r + R1 * hh;
       c32 y[R1];
       cf_stage2<R1>(y, ex, c, mp, true);
 #pragma unroll
       for (int rp = 0; rp < R1; ++rp) out[((mp + 32 * rp) << a.logN1) + o] = y[rp];
     }
     __syncthreads();
-    tile += gridDim.x;
+    tile = next;
     if (tile >= a.ntiles) break;
   }
 }
@@ -385,7 +496,9 @@ __device__ __forceinline__ c32 horner(const c32* a, c32 z) {
 // k2 occupancy target: two 512-thread blocks per CU (4 waves per SIMD) for
 // N1 = 512, whose LDS (2 x 8 rows x 576 + 512 entries, 78 KB) allows it.
 constexpr int k2_waves_per_eu(int n1) { return n1 >= 512 ? 4 : 2; }
-constexpr int k2_rows(int n1) { return n1 >= 512 ? 8 : 16; }
+// rows per group: as many as keep one butterfly per lane in the
+// wavefront-level passes (8 rows for 512 = 8*8*8 over 8 wavefronts, ...)
+constexpr int k2_rows(int n1) { return n1 == 256 ? 16 : 8; }
 
 // Block of T = min(N1, 512) threads; each thread owns PP = N1/T Hermitian
 // pairs q = tid + i*T (i < PP).  Pair q of a regular block (columns c, N2-c)
@@ -456,8 +569,8 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     wB[i] = fast_window(a.wkind, a.wcustom, a.M - kA[i], a.Nf) * a.oscale;
   }
 
-  // four-step twiddles W_M^{-n1 k2} of the two columns (n1 = tid + i*T):
-  // conjugated on input, as they are on output
+  // four-step twiddles W_M^{-n1 k2} of the two columns (n1 = tid + i*T),
+  // applied conjugated on input (k3 applies them on output)
   const float invM = 1.0f / static_cast<float>(a.M);
   c32 twS0[PP], twS1[PP];
 #pragma unroll
@@ -564,7 +677,18 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     z[i] = {1.0, 0.0};
   }
 
+  // Rate groups of JG rates (2 JG rows: columns A and B of each rate).  The
+  // rows alternate between buf and buf2 from group to group, so the one
+  // barrier per group (after the c2r pre-pass writes) also orders a group's
+  // writes after the previous-but-one group's FFT reads.  Each wavefront
+  // then runs the inverse DFT_N1 of its own rows in place and stores them.
   constexpr int JG = ROWS / 2;
+  constexpr int NW = T / 64;      // wavefronts
+  constexpr int RPW = ROWS / NW;  // rows per wavefront
+  static_assert(RPW * NW == ROWS && RPW * fft_max_nr<N1>() <= 64, "one butterfly per lane and pass");
+  const int wv = tid >> 6, lane = tid & 63;
+  c32* cur = buf;
+  c32* nxt = buf2;
   for (int j0 = 0; j0 < a.Nr; j0 += JG) {
     const int nj = min(JG, a.Nr - j0);
     for (int jj = 0; jj < nj; ++jj) {
@@ -579,27 +703,29 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
         if (nyq[i]) {
           // pair (0, M): only the real parts enter the c2r; the bin M/2 slot
           // of the row is zero here (k3 adds it)
-          buf[(2 * jj + rowA[i]) * P] = {0.5f * (YA.x + YB.x), 0.5f * (YA.x - YB.x)};
-          buf[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
+          cur[(2 * jj + rowA[i]) * P] = {0.5f * (YA.x + YB.x), 0.5f * (YA.x - YB.x)};
+          cur[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
         } else {
           const c32 E = {0.5f * (YA.x + YB.x), 0.5f * (YA.y - YB.y)};
           const c32 D = {0.5f * (YA.x - YB.x), 0.5f * (YA.y + YB.y)};
           const c32 O = cmulc(D, WA[i]);  // W^{-k} D
-          buf[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
-          buf[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
+          cur[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
+          cur[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
         }
       }
     }
     __syncthreads();
-    const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, twl);
-    for (int r = 0; r < 2 * nj; ++r) {
-      const int col = (r & 1) ? colB : colA;
-      c32* dst = a.B + (static_cast<int64_t>(j0 + (r >> 1)) * N2 + col) * N1;
+    const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));  // uniform per wavefront
+    wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+      constexpr int RL = sizeof(v) / sizeof(c32);
+      const int row = wv * RPW + f;
+      c32* dst = a.B + (static_cast<int64_t>(j0 + (row >> 1)) * N2 + ((row & 1) ? colB : colA)) * N1 + jx;
 #pragma unroll
-      for (int i = 0; i < PP; ++i)
-        dst[tid + i * T] = cmul(fo[r * P + lsw(tid + i * T)], (r & 1) ? twS1[i] : twS0[i]);
-    }
-    __syncthreads();
+      for (int r = 0; r < RL; ++r) dst[r * (N1 / RL)] = v[r];
+    });
+    c32* t = cur;
+    cur = nxt;
+    nxt = t;
   }
 }
 
