@@ -561,12 +561,13 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   }
   if (special && tid < NT) ahalf[tid] = {0.0f, 0.0f};  // rows u >= Nt stay zero
 
-  // window weight and c2r scale, folded into the coefficients
+  // window weight, c2r scale and the 1/2 of the c2r pre-pass, folded into
+  // the coefficients
   float wA[PP], wB[PP];
 #pragma unroll
   for (int i = 0; i < PP; ++i) {
-    wA[i] = fast_window(a.wkind, a.wcustom, kA[i], a.Nf) * a.oscale;
-    wB[i] = fast_window(a.wkind, a.wcustom, a.M - kA[i], a.Nf) * a.oscale;
+    wA[i] = fast_window(a.wkind, a.wcustom, kA[i], a.Nf) * (0.5f * a.oscale);
+    wB[i] = fast_window(a.wkind, a.wcustom, a.M - kA[i], a.Nf) * (0.5f * a.oscale);
   }
 
   // four-step twiddles W_M^{-n1 k2} of the two columns (n1 = tid + i*T),
@@ -698,16 +699,17 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
         const c32 zA = {static_cast<float>(z[i].x), static_cast<float>(z[i].y)};
         z[i] = cmul(z[i], gam[i]);
         const c32 zB = cmulc(qj, zA);  // e^{i pi dr j} conj(zA)
+        // YA, YB: Ys of bins kA and M - kA, halved
         const c32 YA = horner<NT>(aA[i], zA);
         const c32 YB = horner<NT>(aB[i], zB);
         if (nyq[i]) {
           // pair (0, M): only the real parts enter the c2r; the bin M/2 slot
           // of the row is zero here (k3 adds it)
-          cur[(2 * jj + rowA[i]) * P] = {0.5f * (YA.x + YB.x), 0.5f * (YA.x - YB.x)};
+          cur[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
           cur[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
         } else {
-          const c32 E = {0.5f * (YA.x + YB.x), 0.5f * (YA.y - YB.y)};
-          const c32 D = {0.5f * (YA.x - YB.x), 0.5f * (YA.y + YB.y)};
+          const c32 E = {YA.x + YB.x, YA.y - YB.y};  // (YA + conj YB)/2
+          const c32 D = {YA.x - YB.x, YA.y + YB.y};  // (YA - conj YB)/2
           const c32 O = cmulc(D, WA[i]);  // W^{-k} D
           cur[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
           cur[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
