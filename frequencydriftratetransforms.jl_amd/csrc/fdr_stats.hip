@@ -122,7 +122,9 @@ __device__ __forceinline__ float load_lin(const float* fdr, int64_t Nf, int64_t 
 }
 
 __global__ __launch_bounds__(HIT_THREADS) void hits_count(const float* __restrict__ fdr, int64_t Nf, int64_t n,
-                                                          int64_t ld, double thr, int64_t* __restrict__ counts) {
+                                                          int64_t ld, double thr_h, const double* thr_d,
+                                                          int64_t* __restrict__ counts) {
+  const double thr = thr_d ? *thr_d : thr_h;
   const int64_t tile = blockIdx.x;
   const int64_t base = tile * HIT_TILE;
   int c = 0;
@@ -167,13 +169,14 @@ __global__ __launch_bounds__(1024) void hits_scan(int64_t* __restrict__ counts, 
 }
 
 __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restrict__ fdr, int64_t Nf, int64_t n,
-                                                          int64_t ld, double thr,
+                                                          int64_t ld, double thr_h, const double* thr_d,
                                                           const int64_t* __restrict__ offsets, int64_t ntiles,
-                                                          int64_t total, int64_t base_index,
+                                                          const int64_t* __restrict__ total, int64_t base_index,
                                                           int64_t* __restrict__ hits, int64_t cap) {
+  const double thr = thr_d ? *thr_d : thr_h;
   const int64_t tile = blockIdx.x;
   const int64_t start = offsets[tile];
-  const int64_t end = (tile + 1 < ntiles) ? offsets[tile + 1] : total;
+  const int64_t end = (tile + 1 < ntiles) ? offsets[tile + 1] : *total;
   if (end == start || start >= cap) return;  // uniform: no hits here (or out of room)
   // thread handles HIT_PER_THREAD consecutive elements, so a block scan of
   // the per-thread counts gives each hit its ordered slot
@@ -203,11 +206,11 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restric
   }
 }
 
-int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, int64_t* counts,
-                      hipStream_t st) {
+int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, const double* thr_dev,
+                      int64_t* counts, hipStream_t st) {
   const int64_t nt = hits_tiles(Nf, Nr);
   if (nt == 0) return ZDT_OK;
-  hits_count<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, counts);
+  hits_count<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev, counts);
   ZDT_CHECK_LAUNCH("hits_count");
   return ZDT_OK;
 }
@@ -219,17 +222,51 @@ int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, hipStream_t st)
 }
 
 int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr,
-                           const int64_t* offsets, int64_t ntiles_all, int64_t tile0, const int64_t* total_dev,
-                           int64_t total, int64_t base_index, int64_t* hits, int64_t cap, hipStream_t st) {
-  (void)total_dev;
+                           const double* thr_dev, const int64_t* offsets, int64_t ntiles_all, int64_t tile0,
+                           const int64_t* total_dev, int64_t base_index, int64_t* hits, int64_t cap,
+                           hipStream_t st) {
   const int64_t nt = hits_tiles(Nf, Nr);
   if (nt == 0) return ZDT_OK;
   // offsets of this matrix's tiles start at tile0 of the global scan; the end
   // of its last tile is the next matrix's first offset (or the total)
-  hits_write<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, offsets + tile0,
-                                                                ntiles_all - tile0, total, base_index, hits,
-                                                                cap);
+  hits_write<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev,
+                                                                offsets + tile0, ntiles_all - tile0, total_dev,
+                                                                base_index, hits, cap);
   ZDT_CHECK_LAUNCH("hits_write");
+  return ZDT_OK;
+}
+
+// fdrdenormalize on the device (src/fdrutils.jl:104-119 with :39-50):
+// thr = snr * s + m, s = sqrt(min_j colvar[j]) (NaN if any is NaN),
+// m = colmean[0]; the minimum is taken in index order
+__global__ void hits_threshold(const double* __restrict__ colmean, const double* __restrict__ colvar, int64_t Nr,
+                               double snr, double* __restrict__ thr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double vmin = INFINITY;
+  bool nan = false;
+  for (int64_t j = 0; j < Nr; ++j) {
+    const double v = colvar[j];
+    if (v != v) nan = true;
+    vmin = fmin(vmin, v);
+  }
+  const double sd = nan ? NAN : sqrt(fmax(vmin, 0.0));
+  *thr = snr * sd + colmean[0];
+}
+
+__global__ void set_double(double* p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+int launch_set_double(double* p, double v, hipStream_t st) {
+  set_double<<<1, 64, 0, st>>>(p, v);
+  ZDT_CHECK_LAUNCH("set_double");
+  return ZDT_OK;
+}
+
+int launch_hits_threshold(const double* colmean, const double* colvar, int64_t Nr, double snr, double* thr,
+                          hipStream_t st) {
+  hits_threshold<<<1, 64, 0, st>>>(colmean, colvar, Nr, snr, thr);
+  ZDT_CHECK_LAUNCH("hits_threshold");
   return ZDT_OK;
 }
 

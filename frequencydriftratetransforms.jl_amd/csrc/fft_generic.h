@@ -14,11 +14,13 @@ struct FftPlan {
   void* tw = nullptr;        // device table W_N^m, m < N (c32 or c64)
 };
 
-// Factor N into the supported radices; false if a prime factor is unsupported.
-bool fft_factorize(int64_t N, std::vector<int>* radices);
+// Factor N into the supported radices, largest power-of-two radix first and
+// none above maxr (16, 8 or 4); false if a prime factor is unsupported.
+bool fft_factorize(int64_t N, std::vector<int>* radices, int maxr = 16);
 
-// Cached plan (per device, length and precision).  Plans live until exit.
-int fft_plan_get(int64_t N, bool dbl, const FftPlan** out);
+// Cached plan (per device, length, precision and radix cap).  Plans live
+// until exit.
+int fft_plan_get(int64_t N, bool dbl, const FftPlan** out, int maxr = 16);
 
 // Batched complex transform.  Element n of batch b of `in` is at
 // in[b*ibs + n*ies] (complex elements); same for out.  s0/s1 are scratch
@@ -35,11 +37,11 @@ size_t fft_real_scratch_elems(int64_t N, int64_t batch);
 // Real-to-complex along the contiguous axis: in is N x batch (column stride
 // ldi floats), out is (N/2+1) x batch (column stride ldo complex).
 int fft_r2c(int64_t N, const float* in, int64_t ldi, c32* out, int64_t ldo, int64_t batch, void* s0,
-            void* s1, void* s2, hipStream_t st);
+            void* s1, void* s2, hipStream_t st, int maxr = 16);
 
 // Complex-to-real inverse: the normalized inverse (1/N) times `scale`; the imaginary
 // parts of the DC and (even N) Nyquist bins are ignored.
 int fft_c2r(int64_t N, const c32* in, int64_t ldi, float* out, int64_t ldo, int64_t batch, double scale,
-            void* s0, void* s1, void* s2, hipStream_t st);
+            void* s0, void* s1, void* s2, hipStream_t st, int maxr = 16);
 
 }  // namespace zdt

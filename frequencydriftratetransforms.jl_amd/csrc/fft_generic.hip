@@ -130,15 +130,12 @@ __global__ __launch_bounds__(256) void scale_copy(const cplx<T>* __restrict__ in
 
 // ---------------------------------------------------------------- plans
 
-bool fft_factorize(int64_t N, std::vector<int>* radices) {
+bool fft_factorize(int64_t N, std::vector<int>* radices, int maxr) {
   radices->clear();
   if (N < 1) return false;
   int64_t n = N;
-  while (n % 16 == 0) {
-    radices->push_back(16);
-    n /= 16;
-  }
-  for (int r : {8, 4, 2, 3, 5, 7, 11, 13}) {
+  for (int r : {16, 8, 4, 2, 3, 5, 7, 11, 13}) {
+    if ((r & (r - 1)) == 0 && r > maxr) continue;  // power-of-two radices above the cap
     while (n % r == 0) {
       radices->push_back(r);
       n /= r;
@@ -152,23 +149,25 @@ struct PlanKey {
   int device;
   int64_t N;
   bool dbl;
+  int maxr;
   bool operator<(const PlanKey& o) const {
-    return std::tie(device, N, dbl) < std::tie(o.device, o.N, o.dbl);
+    return std::tie(device, N, dbl, maxr) < std::tie(o.device, o.N, o.dbl, o.maxr);
   }
 };
 std::mutex g_plan_mu;
 std::map<PlanKey, FftPlan*> g_plans;
 }  // namespace
 
-int fft_plan_get(int64_t N, bool dbl, const FftPlan** out) {
+int fft_plan_get(int64_t N, bool dbl, const FftPlan** out, int maxr) {
   std::vector<int> radices;
-  if (!fft_factorize(N, &radices))
+  if (maxr != 16 && maxr != 8 && maxr != 4) return fail(ZDT_EINVAL, "FFT radix cap must be 4, 8 or 16");
+  if (!fft_factorize(N, &radices, maxr))
     return fail(ZDT_ENOTSUP, "FFT length %lld has a prime factor outside {2,3,5,7,11,13}",
                 static_cast<long long>(N));
   int dev = 0;
   ZDT_TRY_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_plan_mu);
-  PlanKey key{dev, N, dbl};
+  PlanKey key{dev, N, dbl, maxr};
   auto it = g_plans.find(key);
   if (it != g_plans.end()) {
     *out = it->second;
@@ -396,19 +395,19 @@ size_t fft_real_scratch_elems(int64_t N, int64_t batch) {
 }
 
 int fft_r2c(int64_t N, const float* in, int64_t ldi, c32* out, int64_t ldo, int64_t batch, void* s0,
-            void* s1, void* s2, hipStream_t st) {
+            void* s1, void* s2, hipStream_t st, int maxr) {
   if (N % 2 == 0) {
     const int64_t M = N / 2;
     if (ldi % 2 != 0) return fail(ZDT_EINVAL, "r2c: column stride must be even for even N");
     const FftPlan *pM = nullptr, *pN = nullptr;
-    ZDT_TRY(fft_plan_get(N, false, &pN));
+    ZDT_TRY(fft_plan_get(N, false, &pN, maxr));
     c32* Z = static_cast<c32*>(s2);
     if (M == 1) {
       // length-1 complex FFT of the packed pair is the pair itself
       ZDT_TRY_HIP(hipMemcpy2DAsync(Z, sizeof(c32), in, ldi * sizeof(float), sizeof(c32), batch,
                                    hipMemcpyDeviceToDevice, st));
     } else {
-      ZDT_TRY(fft_plan_get(M, false, &pM));
+      ZDT_TRY(fft_plan_get(M, false, &pM, maxr));
       ZDT_TRY(fft_c2c(pM, in, 1, ldi / 2, Z, 1, M, batch, false, 1.0, s0, s1, st));
     }
     ProfScope ps(st, "r2c_post", 16.0 * static_cast<double>(M * batch));
@@ -418,7 +417,7 @@ int fft_r2c(int64_t N, const float* in, int64_t ldi, c32* out, int64_t ldo, int6
     return ZDT_OK;
   }
   const FftPlan* p = nullptr;
-  ZDT_TRY(fft_plan_get(N, false, &p));
+  ZDT_TRY(fft_plan_get(N, false, &p, maxr));
   c32* full = static_cast<c32*>(s2);
   real_to_complex<<<grid_for(N * batch, 256), 256, 0, st>>>(in, ldi, full, N, batch);
   ZDT_CHECK_LAUNCH("real_to_complex");
@@ -431,12 +430,12 @@ int fft_r2c(int64_t N, const float* in, int64_t ldi, c32* out, int64_t ldo, int6
 }
 
 int fft_c2r(int64_t N, const c32* in, int64_t ldi, float* out, int64_t ldo, int64_t batch, double scale,
-            void* s0, void* s1, void* s2, hipStream_t st) {
+            void* s0, void* s1, void* s2, hipStream_t st, int maxr) {
   if (N % 2 == 0) {
     const int64_t M = N / 2;
     if (ldo % 2 != 0) return fail(ZDT_EINVAL, "c2r: column stride must be even for even N");
     const FftPlan *pM = nullptr, *pN = nullptr;
-    ZDT_TRY(fft_plan_get(N, false, &pN));
+    ZDT_TRY(fft_plan_get(N, false, &pN, maxr));
     c32* Z = static_cast<c32*>(s2);
     {
       ProfScope ps(st, "c2r_pre", 16.0 * static_cast<double>(M * batch));
@@ -448,13 +447,13 @@ int fft_c2r(int64_t N, const c32* in, int64_t ldi, float* out, int64_t ldo, int6
       ZDT_TRY_HIP(hipMemcpy2DAsync(out, ldo * sizeof(float), Z, sizeof(c32), sizeof(c32), batch,
                                    hipMemcpyDeviceToDevice, st));
     } else {
-      ZDT_TRY(fft_plan_get(M, false, &pM));
+      ZDT_TRY(fft_plan_get(M, false, &pM, maxr));
       ZDT_TRY(fft_c2c(pM, Z, 1, M, out, 1, ldo / 2, batch, true, 1.0, s0, s1, st));
     }
     return ZDT_OK;
   }
   const FftPlan* p = nullptr;
-  ZDT_TRY(fft_plan_get(N, false, &p));
+  ZDT_TRY(fft_plan_get(N, false, &p, maxr));
   c32* full = static_cast<c32*>(s2);
   hermitian_extend<<<grid_for(N * batch, 256), 256, 0, st>>>(in, ldi, full, N, batch);
   ZDT_CHECK_LAUNCH("hermitian_extend");

@@ -12,6 +12,7 @@
 
 #include "fft_generic.h"
 #include "zdt_fast.h"
+#include "zdt_ws.h"
 #include "zdt_internal.h"
 #include "zdt_kernels.h"
 
@@ -160,6 +161,7 @@ struct zdt_ws {
 
   int64_t bytes = 0;
   Profiler* prof = nullptr;
+  int maxr = 16;  // radix cap of the generic FFTs (autotuned by plans)
 };
 
 namespace {
@@ -271,7 +273,7 @@ int output_impl(zdt_ws* ws, float* dest) {
   if (ws->ys_src == nullptr) return fail(ZDT_ESTATE, "output!: no post-processed spectra (run zdtfdr! or the steps first)");
   ZDT_TRY(ensure_scratch(ws, real_scratch_bytes(ws)));
   return fft_c2r(ws->Nf, ws->ys_src, ws->ys_ld, dest, ws->Nf, ws->Nr, out_scale(ws), ws->s[0], ws->s[1], ws->s[2],
-                 ws->stream);
+                 ws->stream, ws->maxr);
 }
 
 int run_impl(zdt_ws* ws, float r0, int wkind, const float* window, float* dest) {
@@ -343,8 +345,14 @@ int64_t zdt_workspace_bytes(int64_t Nf, int64_t Nt, int64_t Nr, const int* facto
   return Nk * Nt * 8 + Nk * Nr * 8 + 3 * sc;
 }
 
-int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, const int* factors,
-               int nfactors, int flags, void* stream, zdt_ws** out) {
+}  // extern "C"
+
+namespace zdt {
+
+// zdt_create with the tunables the plan autotuner chooses: n2 (the fused
+// path's strided FFT length, 0 = default) and maxr (generic FFT radix cap).
+int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, const int* factors,
+                    int nfactors, int flags, void* stream, int n2, int maxr, zdt_ws** out) {
   if (out == nullptr) return fail(ZDT_EINVAL, "out is NULL");
   *out = nullptr;
   if (Nf < 1 || Nt < 1 || Nr < 1)
@@ -371,8 +379,9 @@ int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, in
   int rc = ZDT_OK;
   if (hipGetDevice(&ws->device) != hipSuccess) rc = fail(ZDT_EHIP, "hipGetDevice failed");
   if (flags & ZDT_FLAG_PROFILE) ws->prof = new Profiler();
+  ws->maxr = maxr;
   if (rc == ZDT_OK && !(flags & ZDT_FLAG_GENERIC) && fast_supported(Nf, Nt, Nr)) {
-    rc = fast_create(Nf, Nt, Nr, dr, &ws->fast);
+    rc = fast_create(Nf, Nt, Nr, dr, n2, &ws->fast);
     if (rc == ZDT_OK) ws->bytes += fast_bytes(Nf, Nt, Nr);
   } else if (rc == ZDT_OK) {
     rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->F), static_cast<size_t>(ws->Nk * Nt) * sizeof(c32));
@@ -380,8 +389,8 @@ int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, in
     if (rc == ZDT_OK) rc = ensure_scratch(ws, real_scratch_bytes(ws));
     if (rc == ZDT_OK) {
       const FftPlan* p = nullptr;
-      rc = fft_plan_get(Nf, false, &p);
-      if (rc == ZDT_OK && Nf % 2 == 0 && Nf > 2) rc = fft_plan_get(Nf / 2, false, &p);
+      rc = fft_plan_get(Nf, false, &p, maxr);
+      if (rc == ZDT_OK && Nf % 2 == 0 && Nf > 2) rc = fft_plan_get(Nf / 2, false, &p, maxr);
     }
   }
   if (rc == ZDT_OK && spec != nullptr) rc = zdt_input(ws, spec);
@@ -393,6 +402,24 @@ int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, in
   }
   *out = ws;
   return ZDT_OK;
+}
+
+bool ws_is_fused(const zdt_ws* ws) { return ws != nullptr && ws->fast != nullptr; }
+
+}  // namespace zdt
+
+extern "C" {
+
+int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, const int* factors,
+               int nfactors, int flags, void* stream, zdt_ws** out) {
+  int n2 = 0;
+  if (flags & ZDT_FLAG_N2_512) {
+    if (!zdt::fast_n2_ok(Nf, 512) || (Nf & (Nf - 1)) != 0)
+      return zdt::fail(ZDT_ENOTSUP, "ZDT_FLAG_N2_512: no fused N2 = 512 variant for Nf=%lld", (long long)Nf);
+    n2 = 512;
+  }
+  return zdt::create_ws_tuned(spec, Nf, Nt, r0, dr, Nr, factors, nfactors, flags & ~ZDT_FLAG_N2_512, stream, n2, 16,
+                              out);
 }
 
 int zdt_destroy(zdt_ws* ws) {
@@ -450,7 +477,7 @@ int zdt_input(zdt_ws* ws, const float* spec) {
     ZDT_TRY(fast_input(ws->fast, spec, ws->stream));
     if (ws->F != nullptr) ZDT_TRY(fast_unpack_F(ws->fast, ws->F, ws->stream));
   } else {
-    ZDT_TRY(fft_r2c(ws->Nf, spec, ws->Nf, ws->F, ws->Nk, ws->Nt, ws->s[0], ws->s[1], ws->s[2], ws->stream));
+    ZDT_TRY(fft_r2c(ws->Nf, spec, ws->Nf, ws->F, ws->Nk, ws->Nt, ws->s[0], ws->s[1], ws->s[2], ws->stream, ws->maxr));
   }
   ws->have_input = true;
   return ZDT_OK;
@@ -684,7 +711,7 @@ int zdt_findprotohits(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr, 
   int64_t* total_dev = buf + ntall;
   int rc = ZDT_OK;
   for (int b = 0; b < nb && rc == ZDT_OK; ++b)
-    rc = launch_hits_count(fdrs[b], Nf, Nr, ld, threshold, counts + b * nt, st);
+    rc = launch_hits_count(fdrs[b], Nf, Nr, ld, threshold, nullptr, counts + b * nt, st);
   if (rc == ZDT_OK) rc = launch_hits_scan(counts, ntall, total_dev, st);
   int64_t total = 0;
   if (rc == ZDT_OK) {
@@ -694,7 +721,7 @@ int zdt_findprotohits(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr, 
   }
   if (rc == ZDT_OK && cap > 0 && total > 0) {
     for (int b = 0; b < nb && rc == ZDT_OK; ++b)
-      rc = launch_hits_write_impl(fdrs[b], Nf, Nr, ld, threshold, counts, ntall, b * nt, total_dev, total,
+      rc = launch_hits_write_impl(fdrs[b], Nf, Nr, ld, threshold, nullptr, counts, ntall, b * nt, total_dev,
                                   static_cast<int64_t>(b) * Nf * Nr, hits, cap, st);
   }
   (void)hipFreeAsync(buf, st);

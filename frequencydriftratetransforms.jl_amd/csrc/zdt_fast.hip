@@ -841,15 +841,24 @@ void fast_destroy(FastPlan* p) {
   delete p;
 }
 
-int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, FastPlan** out) {
+bool fast_n2_ok(int64_t Nf, int n2) {
+  if (n2 == 0) return true;
+  if (n2 != 512 && n2 != 1024) return false;
+  const int64_t N1 = Nf / 2 / n2;
+  // compiled (N1, N2) pairs of k2_czt
+  return (n2 == 1024 && N1 >= 64 && N1 <= 512) || (n2 == 512 && N1 >= 64 && N1 <= 512);
+}
+
+int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, FastPlan** out) {
   *out = nullptr;
   if (!fast_supported(Nf, Nt, Nr)) return fail(ZDT_ENOTSUP, "fused path: unsupported shape");
+  if (!fast_n2_ok(Nf, n2)) return fail(ZDT_ENOTSUP, "fused path: N2=%d not available for Nf=%lld", n2, (long long)Nf);
   FastPlan* p = new FastPlan();
   p->Nf = Nf;
   p->Nt = Nt;
   p->Nr = Nr;
   p->M = Nf / 2;
-  p->N2 = fast_n2(Nf);
+  p->N2 = n2 ? n2 : fast_n2(Nf);
   p->logN2 = ilog2i(p->N2);
   p->N1 = static_cast<int>(p->M / p->N2);
   p->logN1 = ilog2i(p->N1);
@@ -964,6 +973,15 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
         break;
       case 64 * 2048 + 1024:
         ZDT_TRY((launch_k2<64, 1024>(a, p->NT, st)));
+        break;
+      case 128 * 2048 + 512:
+        ZDT_TRY((launch_k2<128, 512>(a, p->NT, st)));
+        break;
+      case 256 * 2048 + 512:
+        ZDT_TRY((launch_k2<256, 512>(a, p->NT, st)));
+        break;
+      case 512 * 2048 + 512:
+        ZDT_TRY((launch_k2<512, 512>(a, p->NT, st)));
         break;
       case 128 * 2048 + 1024:
         ZDT_TRY((launch_k2<128, 1024>(a, p->NT, st)));

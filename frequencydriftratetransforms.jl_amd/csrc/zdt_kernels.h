@@ -40,11 +40,18 @@ int launch_colstats2(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, doubl
 int launch_normalize(float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double m, double s, hipStream_t st);
 // proto-hits: count per tile, exclusive scan, ordered write (see fdr_stats.hip)
 int64_t hits_tiles(int64_t Nf, int64_t Nr);
-int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, int64_t* counts,
-                      hipStream_t st);
+// thr_dev, when not NULL, holds the threshold on the device (thr is ignored)
+int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, const double* thr_dev,
+                      int64_t* counts, hipStream_t st);
 int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, hipStream_t st);
 int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr,
-                           const int64_t* offsets, int64_t ntiles_all, int64_t tile0, const int64_t* total_dev,
-                           int64_t total, int64_t base_index, int64_t* hits, int64_t cap, hipStream_t st);
+                           const double* thr_dev, const int64_t* offsets, int64_t ntiles_all, int64_t tile0,
+                           const int64_t* total_dev, int64_t base_index, int64_t* hits, int64_t cap,
+                           hipStream_t st);
+// *p = v by a kernel (graph-capturable, unlike a host-to-device copy)
+int launch_set_double(double* p, double v, hipStream_t st);
+// thr = snr * sqrt(min colvar) + colmean[0], on the device
+int launch_hits_threshold(const double* colmean, const double* colvar, int64_t Nr, double snr, double* thr,
+                          hipStream_t st);
 
 }  // namespace zdt

@@ -10,6 +10,7 @@ import torch  # noqa: F401  (loads the HIP runtime libzdt_hip.so binds to)
 from ._lib import ZDTError, lib
 from .fdrutils import (create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_, fdrstats,
                        findprotohits)
+from .plan import RaggedPlan
 from .zdt import (ZDTWorkspace, convolve_, fdrsynchronize, input_, output_, postprocess_,
                   preprocess_, zdtfdr, zdtfdr_)
 from .zdtutils import (RateRange, batchrates, calcNl, driftrates, estimate_memory, growNr,
@@ -20,4 +21,5 @@ __all__ = [
     "zdtfdr", "zdtfdr_", "calcNl", "growNr", "estimate_memory", "driftrates", "batchrates",
     "RateRange", "workspace_bytes", "create_fdr", "fdrstats", "fdrsynchronize",
     "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "ZDTError", "lib",
+    "RaggedPlan",
 ]

@@ -28,6 +28,9 @@ ZDT_WINDOW_CUSTOM = 2
 ZDT_FLAG_UNNORMALIZED = 0x1
 ZDT_FLAG_GENERIC = 0x2
 ZDT_FLAG_PROFILE = 0x4
+ZDT_FLAG_N2_512 = 0x8
+
+ZDT_PLAN_AUTOTUNE = 0x100
 
 ZDT_BUF_F = 0
 ZDT_BUF_Y = 1
@@ -69,6 +72,14 @@ SIGNATURES = {
     "zdt_fdrnormalize": (C.c_int, [C.POINTER(_vp), C.c_int, _i64, _i64, _i64, C.c_double, C.c_double, _vp]),
     "zdt_findprotohits": (C.c_int, [C.POINTER(_vp), C.c_int, _i64, _i64, _i64, C.c_double, _vp, _i64, _i64p,
                                     _vp]),
+    "zdt_plan_ragged": (C.c_int, [C.c_int, _i64p, _i64p, C.POINTER(C.c_float), C.POINTER(C.c_float), _i64p,
+                                  C.c_int, _vp, C.POINTER(_vp)]),
+    "zdt_plan_destroy": (C.c_int, [_vp]),
+    "zdt_plan_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)]),
+    "zdt_plan_bind": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_double), C.c_int, _vp, _i64,
+                                _vp, _vp]),
+    "zdt_plan_launch": (C.c_int, [_vp]),
 }
 
 

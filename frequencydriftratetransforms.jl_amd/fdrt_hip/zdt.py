@@ -59,11 +59,12 @@ class ZDTWorkspace:
 
     Keyword-only extras: ``unnormalized=True`` scales the output by Nf*Nl
     (test/runtests.jl:31 convention); ``generic=True`` forces the unfused
-    kernels.
+    kernels; ``n2=512`` selects the fused path's 512-point strided FFTs
+    where 1024 is the default.
     """
 
     def __init__(self, spectrogram, *args, factors=None, output_aligned: bool = False,
-                 unnormalized: bool = False, generic: bool = False):
+                 unnormalized: bool = False, generic: bool = False, n2: int = 0):
         if len(args) in (1, 2) and not isinstance(args[0], (int, float, np.floating, np.integer)):
             r0, dr, Nr = rates_params(args[0])
             if len(args) == 2:
@@ -84,7 +85,8 @@ class ZDTWorkspace:
         self.Nr = int(Nr)
         self.device = spec.device
         self._stream = D.stream_ptr()
-        flags = (_lib.ZDT_FLAG_UNNORMALIZED if unnormalized else 0) | (_lib.ZDT_FLAG_GENERIC if generic else 0)
+        flags = ((_lib.ZDT_FLAG_UNNORMALIZED if unnormalized else 0) | (_lib.ZDT_FLAG_GENERIC if generic else 0)
+                 | (_lib.ZDT_FLAG_N2_512 if n2 == 512 else 0))
         arr, n = _lib.int_array(self.factors)
         h = C.c_void_p()
         self._h = None

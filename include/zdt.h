@@ -51,6 +51,8 @@ extern "C" {
 #define ZDT_FLAG_UNNORMALIZED 0x1 /* scale the FDR by Nf*Nl                  */
 #define ZDT_FLAG_GENERIC 0x2      /* force the generic (unfused) kernels     */
 #define ZDT_FLAG_PROFILE 0x4      /* time every kernel launch (HIP events)    */
+#define ZDT_FLAG_N2_512 0x8       /* fused path: strided FFT length 512 where  *
+                                   * 1024 is the default (2^17 <= Nf <= 2^19)  */
 
 /* buffers exposed by zdt_buffer */
 #define ZDT_BUF_F 0  /* Nk x Nt ComplexF32: input FFT (natural layout)        */
@@ -183,6 +185,53 @@ int zdt_fdrnormalize(float* const* fdrs, int nb, int64_t Nf, int64_t Nr,
 int zdt_findprotohits(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr,
                       int64_t ld, double threshold, int64_t* hits, int64_t cap,
                       int64_t* nhits, void* stream);
+
+/* --------------------------------------------------------- ragged plans
+ * Many coarse channels of uneven Nf and Nt, every one through input!, one
+ * zdtfdr! batch, fdrdenormalize and findprotohits, replayed as ONE hipGraph
+ * launch.  No single counterpart in the reference: it replaces the host loop
+ * a reference user writes over channels around zdtfdr! (src/zdtfdr.jl:349-421),
+ * fdrstats/fdrdenormalize (src/fdrutils.jl:39-50,104-119) and findprotohits
+ * (src/fdrutils.jl:148-169).
+ *
+ * Channels with equal (Nf, Nt, r0, dr, Nr) share one workspace and run one
+ * after another; distinct shapes run on parallel branches of the graph.
+ * With ZDT_PLAN_AUTOTUNE, plan creation times the candidate kernel shapes of
+ * each workspace (the fused path's strided FFT length N2 in {512, 1024}, or
+ * the generic FFTs' radix cap in {16, 8}) and keeps the fastest. */
+typedef struct zdt_plan zdt_plan;
+
+#define ZDT_PLAN_AUTOTUNE 0x100
+
+/* Channel c has an Nf[c] x Nt[c] spectrogram and rates r0[c] + j*dr[c],
+ * j < Nr[c].  flags: ZDT_FLAG_UNNORMALIZED, ZDT_FLAG_GENERIC, ZDT_PLAN_AUTOTUNE.
+ * stream: the caller's stream that zdt_plan_launch orders against. */
+int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* r0,
+                    const float* dr, const int64_t* Nr, int flags, void* stream,
+                    zdt_plan** out);
+int zdt_plan_destroy(zdt_plan* p);
+
+/* The choices made for channel c: shape index (channels with one shape share
+ * a workspace), path (1 = fused, 0 = generic), n2 (fused) and maxr (generic). */
+int zdt_plan_info(const zdt_plan* p, int ch, int* shape, int* path, int* n2, int* maxr);
+
+/* Binds device buffers and captures the graph (no work is done yet).
+ * specs[c]: channel c's spectrogram.  dests[c]: its Nf x Nr FDR, or NULL (or
+ * dests NULL) to use a buffer owned by the plan and shared by the channels of
+ * one shape.  thr[c] (host): the threshold, or the SNR when snr != 0
+ * (threshold = snr*s + m from the channel's own fdrstats).  hits (device):
+ * nch*cap int64, channel c's 0-based column-major linear indices at
+ * hits[c*cap ...]; nhits (device, nch int64): channel c's hit count, which
+ * may exceed cap (then only the first cap are written).  thr_out (device,
+ * nch doubles, may be NULL) receives the thresholds used.  Host arrays are
+ * read during the call only.  Binding again replaces the graph. */
+int zdt_plan_bind(zdt_plan* p, const float* const* specs, float* const* dests,
+                  const double* thr, int snr, int64_t* hits, int64_t cap,
+                  int64_t* nhits, double* thr_out);
+
+/* One hipGraphLaunch of the bound graph, ordered after the work already on
+ * the plan's stream and before the work queued on it afterwards. */
+int zdt_plan_launch(zdt_plan* p);
 
 #ifdef __cplusplus
 }

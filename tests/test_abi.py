@@ -96,6 +96,21 @@ def test_null_workspace_errors(L):
     assert L.zdt_fdrstats(None, 0, 10, 10, 10, None, None, None) == _lib.ZDT_EINVAL
 
 
+def test_plan_validation_errors_before_device(L):
+    from fdrt_hip import _lib
+    h = C.c_void_p()
+    one = (C.c_int64 * 1)(1 << 16)
+    f = (C.c_float * 1)(0.0)
+    assert L.zdt_plan_ragged(0, one, one, f, f, one, 0, None, C.byref(h)) == _lib.ZDT_EINVAL
+    assert L.zdt_plan_ragged(1, one, one, f, f, one, 0x4, None, C.byref(h)) == _lib.ZDT_EINVAL
+    assert "flags" in L.zdt_last_error().decode()
+    assert L.zdt_plan_ragged(1, None, one, f, f, one, 0, None, C.byref(h)) == _lib.ZDT_EINVAL
+    assert L.zdt_plan_launch(None) == _lib.ZDT_EINVAL
+    assert L.zdt_plan_info(None, 0, None, None, None, None) == _lib.ZDT_EINVAL
+    assert L.zdt_plan_bind(None, None, None, None, 0, None, 0, None, None) == _lib.ZDT_EINVAL
+    assert L.zdt_plan_destroy(None) == _lib.ZDT_OK
+
+
 def test_batchrates_mirror_matches_oracle():
     import fdrt_hip as F
     for args, kw in [((16, 0.1, 4.0), {"Nrb": 100}), ((16, 0.1, 8.0), {"Nrb": 50}),

@@ -51,6 +51,21 @@ def test_fast_sizes(F, Nf, Nt):
     assert normwise(out, ref) <= 2e-6
 
 
+@pytest.mark.parametrize("Nf", [1 << 17, 1 << 18, 1 << 19])
+def test_fast_n2_512(F, Nf):
+    Nr = 33
+    s = synth.spectrogram(Nf, 16, [(Nf * 0.43, 2.5, 25.0), (Nf * 0.12, -0.75, 35.0)], seed=Nf % 997)
+    say(f"fast N2=512 Nf={Nf}")
+    out = host(F.zdtfdr(F.ZDTWorkspace(s, -3.0, 0.1875, Nr, n2=512)))
+    assert normwise(out, Z.fdr64(s, -3.0, 0.1875, Nr)) <= 2e-6
+
+
+def test_fast_n2_512_unavailable(F):
+    s = synth.spectrogram(1 << 20, 4, [(1000.0, 1.0, 20.0)], seed=3)
+    with pytest.raises(F.ZDTError):
+        F.ZDTWorkspace(s, -1.0, 0.5, 5, n2=512)
+
+
 @pytest.mark.parametrize("Nt", [2, 4, 8, 12, 16])
 def test_fast_vs_generic(F, Nt):
     Nf = 1 << 17
