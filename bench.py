@@ -7,6 +7,10 @@ One step = zdtfdr!(dest, ws, spectrogram) = input! + one ZDT batch writing the
 2^20 x 129 Float32 FDR, with the spectrogram already resident in HBM.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+    python bench.py --ragged [--channels 64] [--steps 3] [--warmup 1]
+
+The second form times the ragged multi-channel plan (one hipGraph launch
+over 64 channels of uneven Nf and Nt) and prints its own JSON line.
 
 Prints ONE JSON line.  value = FDR cells/s = Nf*Nr / (seconds per step).
 The roofline object covers the kernel with the largest share of the step
@@ -103,6 +107,65 @@ def cpu_baseline(seed: int = 1234) -> dict:
                       f"2^20x16x129 by the numpy/scipy FP32 restatement, {dt:.2f} s"}
 
 
+def ragged(a) -> int:
+    """Many coarse channels in one plan launch: 64 channels, Nf drawn from
+    {2^18, 2^19, 2^20}, Nt from {16, 32, 60}, Exponential(1) noise + 8 tones
+    each, rates -4:1/Nt:4, proto-hits at SNR 10 compacted on the device.
+    Prints one JSON line (aggregate FDR cells/s over all channels)."""
+    import torch
+
+    import fdrt_hip as F
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(4321)
+    nch = a.channels
+    shapes = [(int(rng.choice([1 << 18, 1 << 19, 1 << 20])), int(rng.choice([16, 32, 60]))) for _ in range(nch)]
+    gen = torch.Generator(device=dev).manual_seed(4321)
+    specs, rates = [], []
+    for Nf, Nt in shapes:
+        sp = torch.empty((Nt, Nf), device=dev).exponential_(1.0, generator=gen)  # row t = time step
+        c0 = rng.uniform(0, Nf, 8)
+        rate = rng.uniform(-4, 4, 8)
+        amp = rng.uniform(10, 50, 8)
+        t = np.arange(Nt)
+        ch = (np.round(c0[:, None] + rate[:, None] * t[None, :]).astype(np.int64) % Nf).ravel()
+        tt = np.broadcast_to(t, (8, Nt)).ravel()
+        sp[torch.from_numpy(tt).to(dev), torch.from_numpy(ch).to(dev)] += torch.from_numpy(
+            np.repeat(amp, Nt).astype(np.float32)).to(dev)
+        specs.append(sp.t())
+        rates.append((-4.0, 1.0 / Nt, 8 * Nt + 1))
+    t0 = time.perf_counter()
+    plan = F.RaggedPlan(specs, rates, autotune=not a.no_autotune)
+    plan.bind(10.0, snr=True, cap=1 << 16, keep_fdrs=False)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    for _ in range(a.warmup):
+        plan.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        plan.launch()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    cells = sum(Nf * (8 * Nt + 1) for Nf, Nt in shapes)
+    nh = plan.nhits()
+    choice = {}
+    for c, (Nf, Nt) in enumerate(shapes):
+        i = plan.info(c)
+        choice[f"{Nf}x{Nt}"] = i["path"] + (f" n2={i['n2']}" if i["path"] == "fused" else f" maxr={i['maxr']}")
+    print(json.dumps({
+        "metric": "FDR cells/s, ragged channels in one hipGraph launch",
+        "value": cells / dt, "unit": "FDR cells/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic (seed 4321)",
+        "config": {"workload": f"{nch} channels, Nf in {{2^18,2^19,2^20}}, Nt in {{16,32,60}}, rates -4:1/Nt:4, "
+                               "SNR-10 proto-hits on device", "channels": nch, "cells": cells,
+                   "shapes": choice, "plan_seconds": t_plan},
+        "hits": {"total": int(nh.sum()), "max_per_channel": int(nh.max())},
+    }))
+    plan.close()
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,9 +175,17 @@ def main(argv=None) -> int:
     ap.add_argument("--generic", action="store_true", help="force the unfused kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (to scale rocprof byte counters)")
+    ap.add_argument("--ragged", action="store_true", help="the ragged multi-channel plan workload instead")
+    ap.add_argument("--channels", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
     a = ap.parse_args(argv)
     if a.copy_calibration:
         return copy_calibration()
+    if a.ragged:
+        if a.gpus != 1:
+            print("bench.py measures one GPU (N=1 only)", file=sys.stderr)
+            return 2
+        return ragged(a)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU (N=1 only); the path shards across GPUs by "
               "spectrogram and by drift-rate batch, which one GPU's throughput already shows",

@@ -299,7 +299,7 @@ int run_impl(zdt_ws* ws, float r0, int wkind, const float* window, float* dest) 
   a.wkind = wkind;
   a.wcustom = window;
   a.scale = 1.0f;
-  ZDT_TRY(launch_czt_direct(a, ws->stream));
+  ZDT_TRY(launch_czt(a, ws->stream));
   ws->ys_src = ws->Ys;
   ws->ys_ld = ws->Nk;
   ws->ys_fast = false;

@@ -20,6 +20,8 @@ struct CztArgs {
   float scale;
 };
 
+// F -> Ys by czt_horner (Nt <= 64) or czt_direct (any Nt)
+int launch_czt(const CztArgs& a, hipStream_t st);
 int launch_czt_direct(const CztArgs& a, hipStream_t st);
 
 int launch_step_prephase(const c32* F, int64_t ldF, c32* Y, int64_t ldY, int64_t Nk, int Nt, int Nl,

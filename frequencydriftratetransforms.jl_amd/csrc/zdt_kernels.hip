@@ -8,8 +8,8 @@
 // The reference evaluates the convolution over t as a length-Nl circular
 // convolution through FFTs (:249-253); the identity chirp(k,l) = chirp(k,-l)
 // makes the high lags of V (:182-184) the negative lags, so the result equals
-// the linear convolution above for any Nl >= Nt+Nr-1.  czt_direct evaluates
-// that convolution directly (Nt complex MACs per output) which for Nt <= 64
+// the linear convolution above for any Nl >= Nt+Nr-1.  czt_horner (Nt <= 64)
+// and czt_direct evaluate it directly (Nt complex MACs per output), which
 // costs less than the two length-Nl FFTs, needs no V, and is exact up to
 // float rounding of the products.  The step functions (preprocess!,
 // convolve!, postprocess!) keep the reference's FFT data flow on the
@@ -121,6 +121,77 @@ __global__ __launch_bounds__(256) void czt_direct(CztArgs a) {
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Vandermonde form for Nt <= 64: by Bluestein's identity the sum above is
+//   Ys[k,j] = w(k) sum_t a_t z_j^t,  a_t = F[k,t] e^{2 pi i k t r0/Nf},
+//   z_j = gamma^j, gamma = e^{2 pi i k dr/Nf},
+// a polynomial in z_j.  A thread keeps its bin's a_t in registers and writes
+// every Ys[k,j] once (czt_direct re-reads and re-writes Ys once per chunk of
+// 16 time steps).  z_j and z_j^16 come from FP64 recurrences and are rounded
+// to float once per rate; the polynomial is evaluated as 16-term Horner
+// blocks combined by Horner in z^16, so no power of a rounded z above the
+// 15th enters the sum.
+template <int NT>
+__global__ __launch_bounds__(256) void czt_horner(CztArgs a) {
+  constexpr int B = NT < 16 ? NT : 16;  // block length
+  constexpr int NB = NT / B;
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= a.Nk) return;
+  const double kd = static_cast<double>(k);
+  const float w = window_value(a.wkind, a.wcustom, k, a.Nf) * a.scale;
+  c32 c[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (t < a.Nt) {
+      const c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
+      const c32 r = cmul(f, cis_nh(kd * static_cast<double>(t), a.h0));
+      c[t] = {r.x * w, r.y * w};
+    } else {
+      c[t] = {0.0f, 0.0f};
+    }
+  }
+  const c64 g = cis64(2.0 * kd * a.hd);           // gamma = e^{2 pi i k dr/Nf}
+  const c64 gB = cis64(2.0 * B * kd * a.hd);      // gamma^B
+  c64 z = {1.0, 0.0}, zB = {1.0, 0.0};
+  c32* dst = a.Ys + k;
+  for (int j = 0; j < a.Nr; ++j) {
+    const c32 zf = {static_cast<float>(z.x), static_cast<float>(z.y)};
+    const c32 zBf = {static_cast<float>(zB.x), static_cast<float>(zB.y)};
+    c32 acc = {0.0f, 0.0f};
+#pragma unroll
+    for (int b = NB - 1; b >= 0; --b) {
+      c32 p = c[b * B + B - 1];
+#pragma unroll
+      for (int u = B - 2; u >= 0; --u) p = cfma(c[b * B + u], p, zf);
+      acc = (b == NB - 1) ? p : cfma(p, acc, zBf);
+    }
+    dst[static_cast<int64_t>(j) * a.ldY] = acc;
+    z = cmul(z, g);
+    zB = cmul(zB, gB);
+  }
+}
+
+int launch_czt(const CztArgs& a, hipStream_t st) {
+  if (a.Nt > 64) return launch_czt_direct(a, st);
+  const unsigned grid = static_cast<unsigned>((a.Nk + 255) / 256);
+  ProfScope ps(st, "czt_horner", 8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr));
+  if (a.Nt <= 2) {
+    czt_horner<2><<<grid, 256, 0, st>>>(a);
+  } else if (a.Nt <= 4) {
+    czt_horner<4><<<grid, 256, 0, st>>>(a);
+  } else if (a.Nt <= 8) {
+    czt_horner<8><<<grid, 256, 0, st>>>(a);
+  } else if (a.Nt <= 16) {
+    czt_horner<16><<<grid, 256, 0, st>>>(a);
+  } else if (a.Nt <= 32) {
+    czt_horner<32><<<grid, 256, 0, st>>>(a);
+  } else {
+    czt_horner<64><<<grid, 256, 0, st>>>(a);
+  }
+  ZDT_CHECK_LAUNCH("czt_horner");
+  return ZDT_OK;
 }
 
 int launch_czt_direct(const CztArgs& a, hipStream_t st) {

@@ -21,7 +21,7 @@ module ZDTHip
 export HipMatrix, ZDTWorkspace, input!, output!, preprocess!, convolve!, postprocess!,
        zdtfdr, zdtfdr!, calcNl, growNr, estimate_memory, driftrates, batchrates,
        create_fdr, fdrstats, fdrsynchronize, fdrnormalize!, fdrnormalize, fdrdenormalize,
-       findprotohits
+       findprotohits, RaggedPlan, bind!, launch!, planhits
 
 const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "lib", "libzdt_hip.so"))
 const LIBHIP = "libamdhip64.so"
@@ -322,6 +322,88 @@ function findprotohits(fdrs, threshold; snr::Bool=false)
                 (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cdouble, Ptr{Int64}, Int64, Ref{Int64}, Ptr{Cvoid}),
                 ptrs, length(fdrs), Nf, Nr, Nf, threshold, H.ptr, n[], n, C_NULL))
     lin = n[] == 0 ? Int64[] : vec(Array(H))[1:n[]]
+    [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
+end
+
+# ----------------------------------------------------------------- ragged plans
+
+const PLAN_AUTOTUNE = 0x100
+
+"""
+    RaggedPlan(specs::Vector{HipMatrix{Float32}}, rates::Vector; autotune=true)
+
+Many coarse channels of uneven Nf and Nt through `input!`, one `zdtfdr!`
+batch, `fdrdenormalize` and `findprotohits`, replayed as one hipGraph launch
+(include/zdt.h, zdt_plan_*).  `rates[c]` is a range (`r0:δr:r1`) or an
+`(r0, δr, Nr)` tuple.  No counterpart in the reference: it replaces the
+user's loop over channels around src/zdtfdr.jl:349-421 and
+src/fdrutils.jl:148-169.
+"""
+mutable struct RaggedPlan
+    handle::Ptr{Cvoid}
+    specs::Vector{HipMatrix{Float32}}
+    Nf::Vector{Int64}
+    Nr::Vector{Int64}
+    cap::Int
+    hits::Union{Nothing,HipMatrix{Int64}}
+    nhits::Union{Nothing,HipMatrix{Int64}}
+    dests::Vector{HipMatrix{Float32}}
+end
+
+_rate_params(r::Tuple) = (Float32(r[1]), Float32(r[2]), Int64(r[3]))
+_rate_params(r::AbstractRange) = (Float32(first(r)), Float32(step(r)), Int64(length(r)))
+
+function RaggedPlan(specs::Vector{HipMatrix{Float32}}, rates::Vector; autotune::Bool=true,
+                    unnormalized::Bool=false, stream::Ptr{Cvoid}=C_NULL)
+    n = length(specs)
+    n == length(rates) || error("need one rate range per spectrogram")
+    p = map(_rate_params, rates)
+    Nf = Int64[size(s, 1) for s in specs]
+    Nt = Int64[size(s, 2) for s in specs]
+    flags = (autotune ? PLAN_AUTOTUNE : 0) | (unnormalized ? FLAG_UNNORMALIZED : 0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:zdt_plan_ragged, LIBZDT), Cint,
+                (Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cfloat}, Ptr{Cfloat}, Ptr{Int64}, Cint, Ptr{Cvoid},
+                 Ref{Ptr{Cvoid}}),
+                n, Nf, Nt, Float32[x[1] for x in p], Float32[x[2] for x in p], Int64[x[3] for x in p],
+                flags, stream, h))
+    P = RaggedPlan(h[], specs, Nf, Int64[x[3] for x in p], 0, nothing, nothing, HipMatrix{Float32}[])
+    finalizer(P) do q
+        ccall((:zdt_plan_destroy, LIBZDT), Cint, (Ptr{Cvoid},), q.handle)
+    end
+    return P
+end
+
+"""
+    bind!(plan, thresholds; snr=true, cap=4096)
+
+Allocates one FDR per channel and the hit buffers, and captures the graph.
+`thresholds` is one value or one per channel (SNRs when `snr`).
+"""
+function bind!(P::RaggedPlan, thresholds; snr::Bool=true, cap::Integer=4096)
+    n = length(P.specs)
+    thr = Float64.(thresholds isa Number ? fill(thresholds, n) : thresholds)
+    P.dests = [HipMatrix{Float32}(undef, P.Nf[c], P.Nr[c]) for c in 1:n]
+    P.cap = cap
+    P.hits = HipMatrix{Int64}(undef, max(n * cap, 1), 1)
+    P.nhits = HipMatrix{Int64}(undef, n, 1)
+    check(ccall((:zdt_plan_bind, LIBZDT), Cint,
+                (Ptr{Cvoid}, Ptr{Ptr{Float32}}, Ptr{Ptr{Float32}}, Ptr{Cdouble}, Cint, Ptr{Int64}, Int64,
+                 Ptr{Int64}, Ptr{Cdouble}),
+                P.handle, [s.ptr for s in P.specs], [d.ptr for d in P.dests], thr, snr, P.hits.ptr, cap,
+                P.nhits.ptr, C_NULL))
+    P
+end
+
+"launch!(plan): one hipGraphLaunch of the bound plan."
+launch!(P::RaggedPlan) = (check(ccall((:zdt_plan_launch, LIBZDT), Cint, (Ptr{Cvoid},), P.handle)); P)
+
+"planhits(plan, c): channel c's proto-hits as `CartesianIndex`es (like `findprotohits`)."
+function planhits(P::RaggedPlan, c::Integer)
+    n = vec(Array(P.nhits))[c]
+    n <= P.cap || error("channel $c: $n hits exceed capacity $(P.cap)")
+    lin = vec(Array(P.hits))[(c-1)*P.cap+1:(c-1)*P.cap+n]
+    Nf = P.Nf[c]
     [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
 end
 

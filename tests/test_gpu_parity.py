@@ -88,6 +88,8 @@ CASES = [
     (12, 3, -1.0, 0.5, 5),
     (1 << 14, 32, -4.0, 1.0 / 32, 257),
     (6, 2, 0.0, 1.0, 1),
+    (8192, 64, -2.0, 1.0 / 16, 65),
+    (4096, 70, -1.0, 0.125, 17),
 ]
 
 
