@@ -55,8 +55,10 @@ struct Profiler {
       pool.pop_back();
       return e;
     }
+    // timing-only events: no system-scope cache writeback/invalidate at the
+    // record, which would otherwise stall the queue between the kernels
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
   }
   void reset() {
