@@ -563,6 +563,80 @@ int zdt_postprocess(zdt_ws* ws, int window_kind, const float* window) {
   return ZDT_OK;
 }
 
+// ---------------------------------------------------------------- fftfdr / fdshift
+
+namespace {
+struct AsyncBufs {
+  hipStream_t st;
+  std::vector<void*> p;
+  explicit AsyncBufs(hipStream_t s) : st(s) {}
+  int get(void** out, size_t bytes) {
+    *out = nullptr;
+    if (hipMallocAsync(out, bytes, st) != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      return fail(ZDT_ENOMEM, "temporary allocation of %zu bytes failed", bytes);
+    }
+    p.push_back(*out);
+    return ZDT_OK;
+  }
+  ~AsyncBufs() {
+    for (void* q : p) (void)hipFreeAsync(q, st);
+  }
+};
+}  // namespace
+
+int zdt_fftfdr(zdt_ws* ws, const float* rates, int64_t nrates, float* dest) {
+  ZDT_TRY(check_ws(ws));
+  ProfInstall prof_(ws);
+  if (rates == nullptr || dest == nullptr || nrates < 1 || nrates > (1 << 24))
+    return fail(ZDT_EINVAL, "fftfdr!: need a host rate list (1..2^24 rates) and a dest");
+  ZDT_TRY(set_device(ws));
+  if (!ws->have_input) return fail(ZDT_ESTATE, "fftfdr!: no input");
+  ZDT_TRY(ensure_F_natural(ws));
+  // phasor(k, t, Float32(rate), Nf) = cispi(2 k t r/Nf) (src/fftfdr.jl:10-12)
+  std::vector<double> rh(static_cast<size_t>(nrates));
+  for (int64_t j = 0; j < nrates; ++j) {
+    if (!std::isfinite(rates[j])) return fail(ZDT_EINVAL, "fftfdr!: rate %lld is not finite", (long long)j);
+    rh[static_cast<size_t>(j)] = static_cast<double>(rates[j]) / static_cast<double>(ws->Nf);
+  }
+  AsyncBufs tmp(ws->stream);
+  void *rd = nullptr, *Y = nullptr, *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
+  const size_t sc = fft_real_scratch_elems(ws->Nf, nrates) * sizeof(c32);
+  ZDT_TRY(tmp.get(&rd, rh.size() * sizeof(double)));
+  ZDT_TRY(tmp.get(&Y, static_cast<size_t>(ws->Nk * nrates) * sizeof(c32)));
+  ZDT_TRY(tmp.get(&s0, sc));
+  ZDT_TRY(tmp.get(&s1, sc));
+  ZDT_TRY(tmp.get(&s2, sc));
+  ZDT_TRY_HIP(hipMemcpyAsync(rd, rh.data(), rh.size() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+  ZDT_TRY(launch_czt_rates(ws->F, ws->Nk, static_cast<c32*>(Y), ws->Nk, ws->Nk, ws->Nf, static_cast<int>(ws->Nt),
+                           static_cast<const double*>(rd), static_cast<int>(nrates), ws->stream));
+  ZDT_TRY(fft_c2r(ws->Nf, static_cast<const c32*>(Y), ws->Nk, dest, ws->Nf, nrates, 1.0, s0, s1, s2, ws->stream,
+                  ws->maxr));
+  return ZDT_OK;
+}
+
+int zdt_fdshift(zdt_ws* ws, float rate, float* dest) {
+  ZDT_TRY(check_ws(ws));
+  ProfInstall prof_(ws);
+  if (dest == nullptr || !std::isfinite(rate)) return fail(ZDT_EINVAL, "fdshift!: need a finite rate and a dest");
+  ZDT_TRY(set_device(ws));
+  if (!ws->have_input) return fail(ZDT_ESTATE, "fdshift!: no input");
+  ZDT_TRY(ensure_F_natural(ws));
+  AsyncBufs tmp(ws->stream);
+  void *Y = nullptr, *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
+  const size_t sc = fft_real_scratch_elems(ws->Nf, ws->Nt) * sizeof(c32);
+  ZDT_TRY(tmp.get(&Y, static_cast<size_t>(ws->Nk * ws->Nt) * sizeof(c32)));
+  ZDT_TRY(tmp.get(&s0, sc));
+  ZDT_TRY(tmp.get(&s1, sc));
+  ZDT_TRY(tmp.get(&s2, sc));
+  ZDT_TRY(launch_shift_phase(ws->F, ws->Nk, static_cast<c32*>(Y), ws->Nk, ws->Nk, static_cast<int>(ws->Nt),
+                             static_cast<double>(rate) / static_cast<double>(ws->Nf), ws->stream));
+  ZDT_TRY(fft_c2r(ws->Nf, static_cast<const c32*>(Y), ws->Nk, dest, ws->Nf, ws->Nt, 1.0, s0, s1, s2, ws->stream,
+                  ws->maxr));
+  return ZDT_OK;
+}
+
 int zdt_buffer(zdt_ws* ws, int which, void** ptr, int64_t* rows, int64_t* cols) {
   ZDT_TRY(check_ws(ws));
   ZDT_TRY(set_device(ws));

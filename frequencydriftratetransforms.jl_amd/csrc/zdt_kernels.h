@@ -23,6 +23,12 @@ struct CztArgs {
 // F -> Ys by czt_horner (Nt <= 64) or czt_direct (any Nt)
 int launch_czt(const CztArgs& a, hipStream_t st);
 int launch_czt_direct(const CztArgs& a, hipStream_t st);
+// fftfdr: Ys[k,j] = sum_t F[k,t] e^{2 pi i k t rates[j]/Nf} (rates_over_nf: device, Nr doubles)
+int launch_czt_rates(const c32* F, int64_t ldF, c32* Ys, int64_t ldY, int64_t Nk, int64_t Nf, int Nt,
+                     const double* rates_over_nf, int Nr, hipStream_t st);
+// fdshift: Y[k,t] = F[k,t] e^{2 pi i k t rate/Nf}
+int launch_shift_phase(const c32* F, int64_t ldF, c32* Y, int64_t ldY, int64_t Nk, int Nt, double rate_over_nf,
+                       hipStream_t st);
 
 int launch_step_prephase(const c32* F, int64_t ldF, c32* Y, int64_t ldY, int64_t Nk, int Nt, int Nl,
                          double hd, double h0, hipStream_t st);

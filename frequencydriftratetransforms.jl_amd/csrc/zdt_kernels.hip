@@ -194,6 +194,93 @@ int launch_czt(const CztArgs& a, hipStream_t st) {
   return ZDT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// fftfdr (src/fftfdr.jl:86-96,126-148) for an arbitrary list of rates:
+//   Ys[k,j] = sum_t F[k,t] e^{2 pi i k t rates[j]/Nf}.
+// Per bin and rate z = e^{2 pi i k rates[j]/Nf} and z^16 are evaluated
+// directly (k*rate/Nf reduced in FP64), then the same blocked Horner as
+// czt_horner.  For Nt > 64 every term's phase is evaluated directly.
+template <int NT>
+__global__ __launch_bounds__(256) void czt_rates(const c32* __restrict__ F, int64_t ldF, c32* __restrict__ Ys,
+                                                 int64_t ldY, int64_t Nk, int64_t Nf, int Nt,
+                                                 const double* __restrict__ rh, int Nr) {
+  constexpr int B = NT < 16 ? NT : 16;
+  constexpr int NB = NT / B;
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= Nk) return;
+  const double kd = static_cast<double>(k);
+  c32 c[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) c[t] = (t < Nt) ? F[k + static_cast<int64_t>(t) * ldF] : c32{0.0f, 0.0f};
+  for (int j = 0; j < Nr; ++j) {
+    const c32 zf = cis_nh(kd, rh[j]);                          // rh[j] = rates[j]/Nf
+    const c32 zBf = cis_nh(kd * static_cast<double>(B), rh[j]);
+    c32 acc = {0.0f, 0.0f};
+#pragma unroll
+    for (int b = NB - 1; b >= 0; --b) {
+      c32 p = c[b * B + B - 1];
+#pragma unroll
+      for (int u = B - 2; u >= 0; --u) p = cfma(c[b * B + u], p, zf);
+      acc = (b == NB - 1) ? p : cfma(p, acc, zBf);
+    }
+    Ys[k + static_cast<int64_t>(j) * ldY] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void czt_rates_any(const c32* __restrict__ F, int64_t ldF, c32* __restrict__ Ys,
+                                                     int64_t ldY, int64_t Nk, int Nt, const double* __restrict__ rh,
+                                                     int Nr) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= Nk) return;
+  const double kd = static_cast<double>(k);
+  for (int j = 0; j < Nr; ++j) {
+    c32 acc = {0.0f, 0.0f};
+    for (int t = 0; t < Nt; ++t)
+      acc = cfma(acc, F[k + static_cast<int64_t>(t) * ldF], cis_nh(kd * static_cast<double>(t), rh[j]));
+    Ys[k + static_cast<int64_t>(j) * ldY] = acc;
+  }
+}
+
+int launch_czt_rates(const c32* F, int64_t ldF, c32* Ys, int64_t ldY, int64_t Nk, int64_t Nf, int Nt,
+                     const double* rates_over_nf, int Nr, hipStream_t st) {
+  const unsigned grid = static_cast<unsigned>((Nk + 255) / 256);
+  ProfScope ps(st, "czt_rates", 8.0 * static_cast<double>(Nk) * (Nt + Nr));
+  const double* rh = rates_over_nf;
+  if (Nt <= 4) {
+    czt_rates<4><<<grid, 256, 0, st>>>(F, ldF, Ys, ldY, Nk, Nf, Nt, rh, Nr);
+  } else if (Nt <= 16) {
+    czt_rates<16><<<grid, 256, 0, st>>>(F, ldF, Ys, ldY, Nk, Nf, Nt, rh, Nr);
+  } else if (Nt <= 32) {
+    czt_rates<32><<<grid, 256, 0, st>>>(F, ldF, Ys, ldY, Nk, Nf, Nt, rh, Nr);
+  } else if (Nt <= 64) {
+    czt_rates<64><<<grid, 256, 0, st>>>(F, ldF, Ys, ldY, Nk, Nf, Nt, rh, Nr);
+  } else {
+    czt_rates_any<<<grid, 256, 0, st>>>(F, ldF, Ys, ldY, Nk, Nt, rh, Nr);
+  }
+  ZDT_CHECK_LAUNCH("czt_rates");
+  return ZDT_OK;
+}
+
+// fdshift (src/fftfdr.jl:104-111): Y[k,t] = F[k,t] e^{2 pi i k t rate/Nf}
+__global__ __launch_bounds__(256) void shift_phase(const c32* __restrict__ F, int64_t ldF, c32* __restrict__ Y,
+                                                   int64_t ldY, int64_t Nk, int Nt, double rh) {
+  const int64_t total = Nk * Nt;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t k = g % Nk, t = g / Nk;
+    Y[k + t * ldY] = cmul(F[k + t * ldF], cis_nh(static_cast<double>(k) * static_cast<double>(t), rh));
+  }
+}
+
+int launch_shift_phase(const c32* F, int64_t ldF, c32* Y, int64_t ldY, int64_t Nk, int Nt, double rate_over_nf,
+                       hipStream_t st) {
+  int64_t g = (Nk * Nt + 255) / 256;
+  if (g > (1 << 20)) g = 1 << 20;
+  shift_phase<<<static_cast<unsigned>(g), 256, 0, st>>>(F, ldF, Y, ldY, Nk, Nt, rate_over_nf);
+  ZDT_CHECK_LAUNCH("shift_phase");
+  return ZDT_OK;
+}
+
 int launch_czt_direct(const CztArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>((a.Nk + 255) / 256);
   const int nchunks = (a.Nt + 15) / 16;

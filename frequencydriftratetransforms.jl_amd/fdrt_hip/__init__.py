@@ -10,6 +10,8 @@ import torch  # noqa: F401  (loads the HIP runtime libzdt_hip.so binds to)
 from ._lib import ZDTError, lib
 from .fdrutils import (create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_, fdrstats,
                        findprotohits)
+from .fftfdr import (FftFdrWorkspace, fdshift, fdshift_, fdshiftsum, fdshiftsum_, fftfdr, fftfdr_,
+                     fftfdr_workspace, fftfdr_workspace_)
 from .plan import RaggedPlan
 from .zdt import (ZDTWorkspace, convolve_, fdrsynchronize, input_, output_, postprocess_,
                   preprocess_, zdtfdr, zdtfdr_)
@@ -21,5 +23,6 @@ __all__ = [
     "zdtfdr", "zdtfdr_", "calcNl", "growNr", "estimate_memory", "driftrates", "batchrates",
     "RateRange", "workspace_bytes", "create_fdr", "fdrstats", "fdrsynchronize",
     "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "ZDTError", "lib",
-    "RaggedPlan",
+    "RaggedPlan", "FftFdrWorkspace", "fftfdr_workspace", "fftfdr_workspace_", "fftfdr", "fftfdr_",
+    "fdshift", "fdshift_", "fdshiftsum", "fdshiftsum_",
 ]

@@ -64,6 +64,8 @@ SIGNATURES = {
     "zdt_convolve": (C.c_int, [_vp]),
     "zdt_postprocess": (C.c_int, [_vp, C.c_int, _fp]),
     "zdt_buffer": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), _i64p, _i64p]),
+    "zdt_fftfdr": (C.c_int, [_vp, C.POINTER(C.c_float), _i64, _fp]),
+    "zdt_fdshift": (C.c_int, [_vp, C.c_float, _fp]),
     "zdt_profile_reset": (C.c_int, [_vp]),
     "zdt_profile_read": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_int, C.POINTER(C.c_double),
                                    _i64p, C.POINTER(C.c_double)]),

@@ -21,7 +21,8 @@ module ZDTHip
 export HipMatrix, ZDTWorkspace, input!, output!, preprocess!, convolve!, postprocess!,
        zdtfdr, zdtfdr!, calcNl, growNr, estimate_memory, driftrates, batchrates,
        create_fdr, fdrstats, fdrsynchronize, fdrnormalize!, fdrnormalize, fdrdenormalize,
-       findprotohits, RaggedPlan, bind!, launch!, planhits
+       findprotohits, RaggedPlan, bind!, launch!, planhits,
+       fftfdr_workspace, fftfdr_workspace!, fftfdr, fftfdr!, fdshift, fdshift!, fdshiftsum, fdshiftsum!
 
 const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "lib", "libzdt_hip.so"))
 const LIBHIP = "libamdhip64.so"
@@ -324,6 +325,45 @@ function findprotohits(fdrs, threshold; snr::Bool=false)
     lin = n[] == 0 ? Int64[] : vec(Array(H))[1:n[]]
     [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
 end
+
+# ----------------------------------------------------------------- Fourier-shift de-doppler
+
+"fftfdr_workspace(spectrogram) -- src/fftfdr.jl:39-59 (any ZDT workspace holds the input spectrum)"
+fftfdr_workspace(spec::HipMatrix{Float32}; bunaligned=true) = ZDTWorkspace(spec, 0f0, 1f0, 1)
+
+"fftfdr_workspace!(workspace, spectrogram) -- src/fftfdr.jl:69-84"
+function fftfdr_workspace!(ws::ZDTWorkspace, spec::HipMatrix{Float32}; bunaligned=true)
+    size(spec) == (ws.Nf, ws.Nt) || error("input spectrogram has unexpected size")
+    input!(ws, spec)
+    ws
+end
+fftfdr_workspace!(::Nothing, spec::HipMatrix{Float32}; bunaligned=true) = fftfdr_workspace(spec)
+
+"fftfdr!(fdr, workspace, rates) -- src/fftfdr.jl:126-134"
+function fftfdr!(fdr::HipMatrix{Float32}, ws::ZDTWorkspace, rates)
+    r = Float32.(collect(rates))
+    size(fdr) == (ws.Nf, length(r)) || error("fdr must be $(ws.Nf) x $(length(r))")
+    check(ccall((:zdt_fftfdr, LIBZDT), Cint, (Ptr{Cvoid}, Ptr{Cfloat}, Int64, Ptr{Float32}),
+                ws.handle, r, length(r), fdr.ptr))
+    fdr
+end
+
+"fftfdr(workspace, rates) -- src/fftfdr.jl:143-148"
+fftfdr(ws::ZDTWorkspace, rates) = fftfdr!(HipMatrix{Float32}(undef, ws.Nf, length(rates)), ws, rates)
+
+"fdshiftsum!(dest, workspace, rate) / fdshiftsum(workspace, rate) -- src/fftfdr.jl:86-102"
+fdshiftsum!(dest::HipMatrix{Float32}, ws::ZDTWorkspace, rate) = fftfdr!(dest, ws, (rate,))
+fdshiftsum(ws::ZDTWorkspace, rate) = fftfdr(ws, (rate,))
+
+"fdshift!(dest, workspace, rate) -- src/fftfdr.jl:104-111"
+function fdshift!(dest::HipMatrix{Float32}, ws::ZDTWorkspace, rate)
+    size(dest) == (ws.Nf, ws.Nt) || error("dest must be $(ws.Nf) x $(ws.Nt)")
+    check(ccall((:zdt_fdshift, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat, Ptr{Float32}), ws.handle, Float32(rate), dest.ptr))
+    dest
+end
+
+"fdshift(workspace, rate) -- src/fftfdr.jl:113-118"
+fdshift(ws::ZDTWorkspace, rate) = fdshift!(HipMatrix{Float32}(undef, ws.Nf, ws.Nt), ws, rate)
 
 # ----------------------------------------------------------------- ragged plans
 

@@ -147,6 +147,19 @@ int zdt_preprocess(zdt_ws* ws, float r0);
 int zdt_convolve(zdt_ws* ws);
 int zdt_postprocess(zdt_ws* ws, int window_kind, const float* window);
 
+/* fftfdr!(fdr, workspace, rates)  -- src/fftfdr.jl:126-148 (fdshiftsum!,
+ * :86-96, column by column): fdr[:, j] = irfft(sum_t F[:,t] cispi(2 k t
+ * rates[j]/Nf)) for an arbitrary HOST list of nrates Float32 rates, from the
+ * workspace's input spectrum F; dest is device Nf x nrates.  Always
+ * normalized (the reference's scaled irfft plan).  The reference keeps F in
+ * its own fftfdr_workspace (:39-59); here any ZDT workspace serves. */
+int zdt_fftfdr(zdt_ws* ws, const float* rates, int64_t nrates, float* dest);
+
+/* fdshift!(dest, workspace, rate)  -- src/fftfdr.jl:104-111: the spectrogram
+ * de-dopplered at one rate, dest[:, t] = irfft(F[:,t] cispi(2 k t rate/Nf));
+ * dest is device Nf x Nt. */
+int zdt_fdshift(zdt_ws* ws, float rate, float* dest);
+
 /* Device view of a workspace buffer (ZDT_BUF_*): pointer, rows, columns.
  * Y and V exist once a step function ran; F exists after zdt_input. */
 int zdt_buffer(zdt_ws* ws, int which, void** ptr, int64_t* rows, int64_t* cols);

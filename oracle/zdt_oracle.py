@@ -170,14 +170,14 @@ def fdr64(spec: np.ndarray, r0, dr=None, Nr: Optional[int] = None,
 
 
 def fftfdr64(spec: np.ndarray, rates: Iterable) -> np.ndarray:
-    """FP64 ``fftfdr`` (src/fftfdr.jl:187-209): each rate rounded to Float32
-    first, as ``phasor`` receives ``Float32(rate)`` (src/fftfdr.jl:150)."""
+    """FP64 ``fftfdr`` (src/fftfdr.jl:126-148): each rate rounded to Float32
+    first, as ``phasor`` receives ``Float32(rate)`` (src/fftfdr.jl:10-12,88-89)."""
     return fdr64(spec, None, rates=[exact(f32(r)) for r in rates])
 
 
 def fdshift64(spec: np.ndarray, rate) -> np.ndarray:
     """FP64 de-doppler of every time step by ``rate*t`` channels
-    (src/fftfdr.jl:165-179), exact phases."""
+    (``fdshift``, src/fftfdr.jl:104-118), exact phases."""
     spec = np.asarray(spec)
     Nf, Nt = spec.shape
     Nk = Nf // 2 + 1
