@@ -13,6 +13,7 @@ from .fdrutils import (create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_, 
 from .fftfdr import (FftFdrWorkspace, fdshift, fdshift_, fdshiftsum, fdshiftsum_, fftfdr, fftfdr_,
                      fftfdr_workspace, fftfdr_workspace_)
 from .plan import RaggedPlan
+from .stream import StreamSearch, pinned_spectrogram
 from .zdt import (ZDTWorkspace, convolve_, fdrsynchronize, input_, output_, postprocess_,
                   preprocess_, zdtfdr, zdtfdr_)
 from .zdtutils import (RateRange, batchrates, calcNl, driftrates, estimate_memory, growNr,
@@ -24,5 +25,5 @@ __all__ = [
     "RateRange", "workspace_bytes", "create_fdr", "fdrstats", "fdrsynchronize",
     "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "ZDTError", "lib",
     "RaggedPlan", "FftFdrWorkspace", "fftfdr_workspace", "fftfdr_workspace_", "fftfdr", "fftfdr_",
-    "fdshift", "fdshift_", "fdshiftsum", "fdshiftsum_",
+    "fdshift", "fdshift_", "fdshiftsum", "fdshiftsum_", "StreamSearch", "pinned_spectrogram",
 ]

@@ -1,0 +1,98 @@
+"""Long-integration search (include/zdt.h, zdt_stream_*): spectrograms of one
+shape streamed from host memory, rates cut into equal HBM-sized batches, and
+proto-hits at an SNR over all batches of each spectrogram.
+
+It replaces the loop a reference user writes around ``batchrates``
+(src/batchrates.jl:22-60), ``zdtfdr!(dests, ws; r0=...)``
+(src/zdtfdr.jl:349-362) and ``findprotohits(fdrs, snr; snr=true)``
+(src/fdrutils.jl:148-169).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .zdtutils import float32_of, rates_params
+
+
+def pinned_spectrogram(Nf: int, Nt: int) -> torch.Tensor:
+    """An (Nf, Nt) column-major float32 host tensor in pinned memory, whose
+    copies to the device overlap the search."""
+    return torch.empty((Nt, Nf), dtype=torch.float32, pin_memory=True).t()
+
+
+class StreamSearch:
+    """StreamSearch(Nf, Nt, rates; Nrb=0, hbm_bytes=0, generic=False, unnormalized=False)
+
+    ``rates``: a range ``ZDTWorkspace`` accepts, or an (r0, dr, Nr) tuple.
+    ``Nrb=0`` and ``hbm_bytes=0`` let the library size the batches to 90% of
+    the free device memory.  ``search(specs, snr)`` returns one (H, 2) array
+    of 1-based (channel, rate-column) indices per spectrogram (columns of the
+    hcat'd batches, the last batch padded past Nr like ``batchrates``) and the
+    thresholds used.
+    """
+
+    def __init__(self, Nf: int, Nt: int, rates, *, Nrb: int = 0, hbm_bytes: int = 0, generic: bool = False,
+                 unnormalized: bool = False):
+        if isinstance(rates, tuple) and len(rates) == 3:
+            r0, dr, Nr = float32_of(rates[0]), float32_of(rates[1]), int(rates[2])
+        else:
+            r0, dr, Nr = rates_params(rates)
+        self.Nf, self.Nt, self.Nr = int(Nf), int(Nt), int(Nr)
+        self.r0, self.dr = np.float32(r0), np.float32(dr)
+        flags = (_lib.ZDT_FLAG_GENERIC if generic else 0) | (_lib.ZDT_FLAG_UNNORMALIZED if unnormalized else 0)
+        h = C.c_void_p()
+        self._h = None
+        _lib.check(_lib.lib().zdt_stream_create(self.Nf, self.Nt, float(self.r0), float(self.dr), self.Nr, int(Nrb),
+                                                int(hbm_bytes), flags, C.byref(h)))
+        self._h = h
+        nrb, nb, res, by = C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()
+        _lib.check(_lib.lib().zdt_stream_info(self._h, C.byref(nrb), C.byref(nb), C.byref(res), C.byref(by)))
+        self.Nrb, self.nbatch, self.resident, self.nbytes = nrb.value, nb.value, bool(res.value), by.value
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.lib().zdt_stream_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def search(self, specs: Sequence, snr: float = 10.0, cap: int = 1 << 20) -> Tuple[List[np.ndarray], np.ndarray]:
+        keep = []
+        ptrs = []
+        for s in specs:
+            if isinstance(s, torch.Tensor):
+                if s.is_cuda or s.dtype != torch.float32 or tuple(s.shape) != (self.Nf, self.Nt):
+                    raise ValueError(f"expected host float32 ({self.Nf}, {self.Nt}) spectrograms")
+                if not (s.stride(0) == 1 and s.stride(1) == self.Nf):
+                    s = s.t().contiguous().t()
+                keep.append(s)
+                ptrs.append(s.data_ptr())
+            else:
+                a = np.asfortranarray(np.asarray(s, dtype=np.float32))
+                if a.shape != (self.Nf, self.Nt):
+                    raise ValueError(f"expected ({self.Nf}, {self.Nt}) spectrograms")
+                keep.append(a)
+                ptrs.append(a.ctypes.data)
+        n = len(ptrs)
+        hits = np.zeros(max(1, n * cap), dtype=np.int64)
+        nh = np.zeros(n, dtype=np.int64)
+        thr = np.zeros(n, dtype=np.float64)
+        rc = _lib.lib().zdt_stream_search(self._h, (C.c_void_p * n)(*ptrs), n, float(snr),
+                                          hits.ctypes.data_as(C.c_void_p), int(cap), nh.ctypes.data_as(C.c_void_p),
+                                          thr.ctypes.data_as(C.c_void_p))
+        _lib.check(rc)
+        out = []
+        for i in range(n):
+            lin = hits[i * cap: i * cap + nh[i]]
+            out.append(np.stack([lin % self.Nf + 1, lin // self.Nf + 1], axis=1).astype(np.int64)
+                       if nh[i] else np.zeros((0, 2), np.int64))
+        return out, thr

@@ -22,7 +22,8 @@ export HipMatrix, ZDTWorkspace, input!, output!, preprocess!, convolve!, postpro
        zdtfdr, zdtfdr!, calcNl, growNr, estimate_memory, driftrates, batchrates,
        create_fdr, fdrstats, fdrsynchronize, fdrnormalize!, fdrnormalize, fdrdenormalize,
        findprotohits, RaggedPlan, bind!, launch!, planhits,
-       fftfdr_workspace, fftfdr_workspace!, fftfdr, fftfdr!, fdshift, fdshift!, fdshiftsum, fdshiftsum!
+       fftfdr_workspace, fftfdr_workspace!, fftfdr, fftfdr!, fdshift, fdshift!, fdshiftsum, fdshiftsum!,
+       StreamSearch, search
 
 const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "lib", "libzdt_hip.so"))
 const LIBHIP = "libamdhip64.so"
@@ -445,6 +446,64 @@ function planhits(P::RaggedPlan, c::Integer)
     lin = vec(Array(P.hits))[(c-1)*P.cap+1:(c-1)*P.cap+n]
     Nf = P.Nf[c]
     [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
+end
+
+# ----------------------------------------------------------------- long-integration search
+
+"""
+    StreamSearch(Nf, Nt, rates; Nrb=0, hbm_bytes=0)
+
+Spectrograms of one shape streamed from host memory, rates cut into equal
+HBM-sized batches (include/zdt.h, zdt_stream_*).  Replaces the loop around
+`batchrates` (src/batchrates.jl:22-60), `zdtfdr!(dests, ws; r0)`
+(src/zdtfdr.jl:349-362) and `findprotohits(fdrs, snr; snr=true)`
+(src/fdrutils.jl:148-169).
+"""
+mutable struct StreamSearch
+    handle::Ptr{Cvoid}
+    Nf::Int
+    Nt::Int
+    Nrb::Int
+    nbatch::Int
+end
+
+function StreamSearch(Nf::Integer, Nt::Integer, rates; Nrb::Integer=0, hbm_bytes::Integer=0)
+    r0, δr, Nr = _rate_params(rates)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:zdt_stream_create, LIBZDT), Cint,
+                (Int64, Int64, Cfloat, Cfloat, Int64, Int64, Int64, Cint, Ref{Ptr{Cvoid}}),
+                Nf, Nt, r0, δr, Nr, Nrb, hbm_bytes, 0, h))
+    nrb, nb = Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:zdt_stream_info, LIBZDT), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ptr{Cint}, Ptr{Int64}),
+                h[], nrb, nb, C_NULL, C_NULL))
+    S = StreamSearch(h[], Nf, Nt, nrb[], nb[])
+    finalizer(S) do q
+        ccall((:zdt_stream_destroy, LIBZDT), Cint, (Ptr{Cvoid},), q.handle)
+    end
+    S
+end
+
+"""
+    search(S, specs::Vector{Matrix{Float32}}, snr; cap=2^20) -> (hits, thresholds)
+
+`hits[i]` lists spectrogram i's proto-hits as `CartesianIndex`es over the
+hcat'd batches, like `findprotohits(fdrs, snr; snr=true)`.
+"""
+function search(S::StreamSearch, specs::Vector{Matrix{Float32}}, snr; cap::Integer=2^20)
+    n = length(specs)
+    all(size(s) == (S.Nf, S.Nt) for s in specs) || error("spectrograms must be $(S.Nf) x $(S.Nt)")
+    hits = Vector{Int64}(undef, max(n * cap, 1))
+    nh = zeros(Int64, n)
+    thr = zeros(Float64, n)
+    GC.@preserve specs begin
+        rc = ccall((:zdt_stream_search, LIBZDT), Cint,
+                   (Ptr{Cvoid}, Ptr{Ptr{Float32}}, Cint, Cdouble, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Cdouble}),
+                   S.handle, [pointer(s) for s in specs], n, snr, hits, cap, nh, thr)
+    end
+    check(rc)
+    out = [[CartesianIndex(Int(l % S.Nf) + 1, Int(l ÷ S.Nf) + 1) for l in hits[(i-1)*cap+1:(i-1)*cap+nh[i]]]
+           for i in 1:n]
+    out, thr
 end
 
 end # module ZDTHip

@@ -246,6 +246,42 @@ int zdt_plan_bind(zdt_plan* p, const float* const* specs, float* const* dests,
  * the plan's stream and before the work queued on it afterwards. */
 int zdt_plan_launch(zdt_plan* p);
 
+/* -------------------------------------------------- long-integration search
+ * Spectrograms of one shape streamed from host memory (the copy of the next
+ * one overlaps the search of the current one), the rates r0 + j*dr cut into
+ * equal batches of Nrb rates sized to an HBM budget, and proto-hits at an SNR
+ * threshold over all batches of each spectrogram.  No single counterpart in
+ * the reference: it replaces the user's loop around batchrates
+ * (src/batchrates.jl:22-60), estimate_memory (src/zdtutils.jl:35-53),
+ * zdtfdr!(dests, ws; r0=batch starts) (src/zdtfdr.jl:349-362) and
+ * findprotohits(fdrs, snr; snr=true) (src/fdrutils.jl:148-169).  Like
+ * batchrates, the last batch is padded: nbatch*Nrb >= Nr rates are searched. */
+typedef struct zdt_stream zdt_stream;
+
+/* Pinned host memory (hipHostMalloc) for spectrograms that should stream
+ * asynchronously; pageable memory works too but its copies do not overlap. */
+int zdt_host_alloc(size_t bytes, void** p);
+int zdt_host_free(void* p);
+
+/* Nrb = 0 chooses the batch length: the largest that keeps every batch FDR
+ * resident within hbm_bytes, else the largest that fits with one FDR buffer
+ * (each batch is then computed twice: once for the statistics, once for the
+ * hits).  hbm_bytes = 0 means 90% of the free device memory.
+ * flags: ZDT_FLAG_UNNORMALIZED, ZDT_FLAG_GENERIC. */
+int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr,
+                      int64_t Nrb, int64_t hbm_bytes, int flags, zdt_stream** out);
+int zdt_stream_destroy(zdt_stream* s);
+int zdt_stream_info(const zdt_stream* s, int64_t* Nrb, int64_t* nbatch, int* resident,
+                    int64_t* bytes);
+
+/* Searches nspec HOST spectrograms (Nf x Nt Float32, column-major).  For
+ * spectrogram i: thr[i] = snr*s + m over all its batches (may be NULL),
+ * nhits[i] = its hit count, and hits[i*cap ...] its first min(nhits[i], cap)
+ * 0-based linear indices i + Nf*(b*Nrb + j) (HOST arrays).  Returns
+ * ZDT_ETRUNC when some spectrogram has more than cap hits. */
+int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, double snr,
+                      int64_t* hits, int64_t cap, int64_t* nhits, double* thr);
+
 #ifdef __cplusplus
 }
 #endif
