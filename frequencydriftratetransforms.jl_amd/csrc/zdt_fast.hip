@@ -12,7 +12,7 @@
 //              W_M^{n1 k2}, DFT_N1 over n1 -> Z[k] -> r2c untangle -> X[k,t] (= F of the reference) ->
 //              chirp-Z over t for every rate (preprocess!/convolve!/
 //              postprocess!) -> c2r pre-pass -> inverse DFT_N1 over k1
-//              -> B[j][k2][n1]
+//              -> B[j][n1/16][k2][n1%16]
 //   k3_output  for each (j, n1): times W_M^{-n1 k2}, inverse DFT_N2 over k2
 //              -> z'[n1 + N1 n2] = FDR[2n, j] + i FDR[2n+1, j]  (output!)
 //
@@ -377,8 +377,8 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k1_inp
 // ---------------------------------------------------------------- k3: output pass 2
 
 struct K3Args {
-  const c32* B;  // [Nr][N2][N1]
-  const c32* yhalf;  // [Nr] bin M/2 term: B[j][0][n1] += yhalf[j] (-1)^{n1}
+  const c32* B;  // [Nr][N1/16][N2][16]
+  const c32* yhalf;  // [Nr] bin M/2 term: B(j, k2 = 0, n1) += yhalf[j] (-1)^{n1}
   float* dest;   // Nf x Nr, column stride ld floats
   int64_t ld;
   const c32* tw2;
@@ -401,10 +401,10 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
   // rates last to first: the most recently written spectra are read first
   auto load = [&](int tl, c32(&x)[32], c32& h) {
     const int jr = tl / nblk, nb = tl - jr * nblk;
-    const c32* b = a.B + static_cast<int64_t>(a.Nr - 1 - jr) * C::N2 * N1;
-    const unsigned o = (nb << 4) + c;
+    // the tile's 16 columns are one contiguous [k2][16] block of B
+    const c32* b = a.B + static_cast<int64_t>(a.Nr - 1 - jr) * C::N2 * N1 + static_cast<int64_t>(nb) * C::N2 * 16;
 #pragma unroll
-    for (int m = 0; m < 32; ++m) x[m] = b[(static_cast<unsigned>(r + R1 * m) << a.logN1) + o];
+    for (int m = 0; m < 32; ++m) x[m] = b[static_cast<unsigned>(r + R1 * m) * 16 + c];
     h = a.yhalf[a.Nr - 1 - jr];
   };
   c32 x[32], h;
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
 
 struct K2Args {
   const c32* G;      // [Nt][N2][N1]
-  c32* B;            // [Nr][N2][N1]
+  c32* B;            // [Nr][N1/16][N2][16]
   const c32* tw1;    // W_N1^m
   const c32* tabq;   // e^{i pi dr j}, j < Nr
   c32* yhalf;        // [Nr] c2r pre-pass value of the bin M/2, added by k3
@@ -506,7 +506,7 @@ constexpr int k2_rows(int n1) { return n1 == 256 ? 16 : 8; }
 // self-paired columns 0 and N2/2: q < N1/2 pairs (N2 q, N2 (N1-q)) in column 0
 // (q = 0 pairs bin 0 with the Nyquist bin M), q >= N1/2 pairs within column
 // N2/2.  The one bin left over, M/2 (column 0, k1 = N1/2), enters the c2r
-// pre-pass at row 0, k1 = N1/2, hence B[j][0][n1] as C_j (-1)^{n1}: block 0
+// pre-pass at row 0, k1 = N1/2, hence B(j, k2 = 0, n1) as C_j (-1)^{n1}: block 0
 // evaluates C_j for every rate up front into yhalf[j] and k3 adds it, which
 // keeps its registers out of the rate loop.  The window weight and output
 // scale are folded into the Vandermonde coefficients a_t.
@@ -721,9 +721,13 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
       constexpr int RL = sizeof(v) / sizeof(c32);
       const int row = wv * RPW + f;
-      c32* dst = a.B + (static_cast<int64_t>(j0 + (row >> 1)) * N2 + ((row & 1) ? colB : colA)) * N1 + jx;
+      // B is blocked by 16 n1: B[j][n1/16][k2][n1%16]
+      c32* dst = a.B + static_cast<int64_t>(j0 + (row >> 1)) * (N1 * N2) + ((row & 1) ? colB : colA) * 16;
 #pragma unroll
-      for (int r = 0; r < RL; ++r) dst[r * (N1 / RL)] = v[r];
+      for (int r = 0; r < RL; ++r) {
+        const int n1 = jx + r * (N1 / RL);
+        dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
+      }
     });
     c32* t = cur;
     cur = nxt;
@@ -788,7 +792,7 @@ struct FastPlan {
   int N1 = 0, logN1 = 0, N2 = 0, logN2 = 0, NT = 0;
   float dr = 0;
   c32* G = nullptr;     // [Nt][N2][N1]
-  c32* B = nullptr;     // [Nr][N2][N1]
+  c32* B = nullptr;     // [Nr][N1/16][N2][16]
   c32* tw1 = nullptr;   // W_N1^m
   c32* tw2 = nullptr;   // W_N2^m
   c32* tabq = nullptr;  // e^{i pi dr j}
