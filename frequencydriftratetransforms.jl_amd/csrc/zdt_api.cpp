@@ -155,6 +155,9 @@ struct zdt_ws {
 
   // fused fast path (power-of-two Nf)
   FastPlan* fast = nullptr;
+  // fused output for the generic chirp-Z (power-of-two Nf, 16 < Nt <= 64)
+  FastPlan* fout = nullptr;
+  bool ys_fout = false;  // last run left its spectra in fout
 
   bool have_input = false;
   const c32* ys_src = nullptr;  // post-processed spectra for zdt_output
@@ -272,6 +275,7 @@ int ensure_V(zdt_ws* ws) {
 
 int output_impl(zdt_ws* ws, float* dest) {
   if (ws->ys_fast) return fast_output(ws->fast, dest, out_scale(ws), ws->stream);
+  if (ws->ys_fout) return fast_output(ws->fout, dest, out_scale(ws), ws->stream);
   if (ws->ys_src == nullptr) return fail(ZDT_ESTATE, "output!: no post-processed spectra (run zdtfdr! or the steps first)");
   ZDT_TRY(ensure_scratch(ws, real_scratch_bytes(ws)));
   return fft_c2r(ws->Nf, ws->ys_src, ws->ys_ld, dest, ws->Nf, ws->Nr, out_scale(ws), ws->s[0], ws->s[1], ws->s[2],
@@ -284,6 +288,14 @@ int run_impl(zdt_ws* ws, float r0, int wkind, const float* window, float* dest) 
   if (use_fast(ws)) {
     ZDT_TRY(fast_run(ws->fast, r0, wkind, window, dest, out_scale(ws), ws->stream));
     ws->ys_fast = (dest == nullptr);
+    ws->ys_src = nullptr;
+    return ZDT_OK;
+  }
+  if (ws->fout != nullptr) {
+    ZDT_TRY(fast_out_run(ws->fout, ws->F, ws->Nk, static_cast<int>(ws->Nt), r0, ws->dr, wkind, window, dest,
+                         out_scale(ws), ws->stream));
+    ws->ys_fast = false;
+    ws->ys_fout = (dest == nullptr);
     ws->ys_src = nullptr;
     return ZDT_OK;
   }
@@ -305,6 +317,7 @@ int run_impl(zdt_ws* ws, float r0, int wkind, const float* window, float* dest) 
   ws->ys_src = ws->Ys;
   ws->ys_ld = ws->Nk;
   ws->ys_fast = false;
+  ws->ys_fout = false;
   if (dest != nullptr) ZDT_TRY(output_impl(ws, dest));
   return ZDT_OK;
 }
@@ -342,6 +355,10 @@ int64_t zdt_workspace_bytes(int64_t Nf, int64_t Nt, int64_t Nr, const int* facto
   if (Nl < 0 || Nf < 1) return -1;
   const int64_t Nk = Nf / 2 + 1;
   if (!(flags & ZDT_FLAG_GENERIC) && fast_supported(Nf, Nt, Nr)) return fast_bytes(Nf, Nt, Nr);
+  if (!(flags & ZDT_FLAG_GENERIC) && fast_out_supported(Nf, Nt, Nr)) {
+    const int64_t sc = static_cast<int64_t>(fft_real_scratch_elems(Nf, Nt) * sizeof(c32));
+    return Nk * Nt * 8 + fast_out_bytes(Nf, Nr) + 3 * sc;
+  }
   const int64_t b = std::max(Nt, Nr);
   const int64_t sc = static_cast<int64_t>(fft_real_scratch_elems(Nf, b) * sizeof(c32));
   return Nk * Nt * 8 + Nk * Nr * 8 + 3 * sc;
@@ -386,9 +403,17 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
     rc = fast_create(Nf, Nt, Nr, dr, n2, &ws->fast);
     if (rc == ZDT_OK) ws->bytes += fast_bytes(Nf, Nt, Nr);
   } else if (rc == ZDT_OK) {
+    const bool fout = !(flags & ZDT_FLAG_GENERIC) && fast_out_supported(Nf, Nt, Nr);
     rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->F), static_cast<size_t>(ws->Nk * Nt) * sizeof(c32));
-    if (rc == ZDT_OK) rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->Ys), static_cast<size_t>(ws->Nk * Nr) * sizeof(c32));
-    if (rc == ZDT_OK) rc = ensure_scratch(ws, real_scratch_bytes(ws));
+    if (rc == ZDT_OK && fout) {
+      rc = fast_out_create(Nf, Nr, &ws->fout);
+      if (rc == ZDT_OK) ws->bytes += fast_out_bytes(Nf, Nr);
+      if (rc == ZDT_OK) rc = ensure_scratch(ws, fft_real_scratch_elems(Nf, Nt) * sizeof(c32));
+    } else {
+      if (rc == ZDT_OK)
+        rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->Ys), static_cast<size_t>(ws->Nk * Nr) * sizeof(c32));
+      if (rc == ZDT_OK) rc = ensure_scratch(ws, real_scratch_bytes(ws));
+    }
     if (rc == ZDT_OK) {
       const FftPlan* p = nullptr;
       rc = fft_plan_get(Nf, false, &p, maxr);
@@ -407,6 +432,11 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
 }
 
 bool ws_is_fused(const zdt_ws* ws) { return ws != nullptr && ws->fast != nullptr; }
+
+int ws_path(const zdt_ws* ws) {
+  if (ws == nullptr) return 0;
+  return ws->fast ? 1 : (ws->fout ? 2 : 0);
+}
 
 }  // namespace zdt
 
@@ -435,6 +465,7 @@ int zdt_destroy(zdt_ws* ws) {
   dev_free(ws->V);
   for (int i = 0; i < 3; ++i) dev_free(ws->s[i]);
   if (ws->fast) fast_destroy(ws->fast);
+  if (ws->fout) fast_destroy(ws->fout);
   delete ws->prof;
   delete ws;
   return ZDT_OK;
@@ -560,6 +591,7 @@ int zdt_postprocess(zdt_ws* ws, int window_kind, const float* window) {
   ws->ys_src = ws->Y;
   ws->ys_ld = ws->Nk;
   ws->ys_fast = false;
+  ws->ys_fout = false;
   return ZDT_OK;
 }
 

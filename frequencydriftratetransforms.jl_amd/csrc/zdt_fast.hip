@@ -735,6 +735,201 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   }
 }
 
+// ---------------------------------------------------------------- fused output for Nt > 16
+//
+// For power-of-two Nf with Nt up to 64 (more coefficients than k2 keeps in
+// registers) the chirp-Z runs per bin (czt_horner_col, the Vandermonde form
+// of czt_horner) and the output irfft reuses the second half of k2 and k3:
+//   czt_horner_col  Ys[k, j] -> Yc[j][k2][k1] (k = k2 + N2 k1 < M), Ys[M, j] -> yM[j]
+//   k2_c2r          per Hermitian column pair: c2r pre-pass, inverse DFT_N1
+//                   over k1 (one wavefront per row) -> B, bin M/2 -> yhalf
+//   k3_output       as in the fused path
+// which moves the spectra through HBM twice instead of once per FFT pass.
+
+struct CztColArgs {
+  const c32* F;  // Nk x Nt, natural layout
+  int64_t ldF;
+  c32* Yc;       // [Nr][N2][N1]
+  c32* yM;       // [Nr]
+  int64_t M, Nf;
+  int logN1, logN2;
+  int Nt, Nr;
+  double hd2;    // dr/Nf
+  double h0;     // r0/Nf
+  int wkind;
+  const float* wcustom;
+  float scale;
+};
+
+// one thread per bin; a block covers 16 columns k2 x 16 rows k1 (so the Nr
+// stores of a wavefront are four 128-byte rows of Yc); the last block's
+// thread 0 takes the Nyquist bin M
+template <int NT>
+__global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
+  constexpr int B = NT < 16 ? NT : 16;
+  constexpr int NB = NT / B;
+  const int ntk1 = (1 << a.logN1) >> 4;
+  const int tile = blockIdx.x;
+  int64_t k;
+  c32* dst;
+  int64_t stride;
+  if (tile * 256 >= a.M) {
+    if (threadIdx.x != 0) return;
+    k = a.M;
+    dst = a.yM;
+    stride = 1;
+  } else {
+    const int k2 = ((tile / ntk1) << 4) + (threadIdx.x >> 4);
+    const int k1 = ((tile % ntk1) << 4) + (threadIdx.x & 15);
+    k = k2 + (static_cast<int64_t>(k1) << a.logN2);
+    dst = a.Yc + (static_cast<int64_t>(k2) << a.logN1) + k1;
+    stride = a.M;
+  }
+  const double kd = static_cast<double>(k);
+  const float w = fast_window(a.wkind, a.wcustom, k, a.Nf) * a.scale;
+  c32 c[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (t < a.Nt) {
+      const c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
+      const c32 r = cmul(f, cis_nh(kd * static_cast<double>(t), a.h0));
+      c[t] = {r.x * w, r.y * w};
+    } else {
+      c[t] = {0.0f, 0.0f};
+    }
+  }
+  const c64 g = cis64f(kd * a.hd2);
+  const c64 gB = cis64f(static_cast<double>(B) * kd * a.hd2);
+  c64 z = {1.0, 0.0}, zB = {1.0, 0.0};
+  for (int j = 0; j < a.Nr; ++j) {
+    const c32 zf = {static_cast<float>(z.x), static_cast<float>(z.y)};
+    const c32 zBf = {static_cast<float>(zB.x), static_cast<float>(zB.y)};
+    c32 acc = {0.0f, 0.0f};
+#pragma unroll
+    for (int b = NB - 1; b >= 0; --b) {
+      c32 q = c[b * B + B - 1];
+#pragma unroll
+      for (int u = B - 2; u >= 0; --u) q = cfma(c[b * B + u], q, zf);
+      acc = (b == NB - 1) ? q : cfma(q, acc, zBf);
+    }
+    dst[static_cast<int64_t>(j) * stride] = acc;
+    z = cmul(z, g);
+    zB = cmul(zB, gB);
+  }
+}
+
+struct K2cArgs {
+  const c32* Yc;  // [Nr][N2][N1]
+  const c32* yM;  // [Nr]
+  c32* B;         // [Nr][N1/16][N2][16]
+  c32* yhalf;     // [Nr]
+  const c32* tw1;
+  int Nr;
+  int64_t M, Nf;
+  float oscale;   // output scale / M
+};
+
+// the rate loop of k2_czt with the spectra read from Yc instead of evaluated
+template <int N1, int N2, int ROWS>
+__global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c2r(K2cArgs a) {
+  constexpr int T = N1 < 512 ? N1 : 512;
+  static_assert(T == N1, "one Hermitian pair per thread");
+  constexpr int P = N1 + N1 / 8;
+  constexpr int HALF = N1 / 2;
+  __shared__ c32 buf[ROWS * P];
+  __shared__ c32 buf2[ROWS * P];
+  __shared__ c32 twl[N1];
+  for (int i = threadIdx.x; i < N1; i += T) twl[i] = a.tw1[i];
+  __syncthreads();
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x;
+  const bool special = (p == 0);
+  const int colA = special ? 0 : p;
+  const int colB = special ? N2 / 2 : N2 - p;
+  int rowA, rowB, k1A, k1B, cA, cB;
+  bool nyq = false;
+  if (!special) {
+    rowA = 0;
+    rowB = 1;
+    k1A = tid;
+    k1B = N1 - 1 - tid;
+    cA = colA;
+    cB = colB;
+  } else if (tid < HALF) {
+    rowA = rowB = 0;
+    k1A = tid;
+    k1B = N1 - tid;
+    nyq = (tid == 0);
+    cA = cB = 0;
+  } else {
+    rowA = rowB = 1;
+    k1A = tid - HALF;
+    k1B = N1 - 1 - k1A;
+    cA = cB = N2 / 2;
+  }
+  const int64_t kA = cA + static_cast<int64_t>(N2) * k1A;
+  const c32 WA = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));
+  const float h = 0.5f * a.oscale;
+  if (special) {
+    for (int j = tid; j < a.Nr; j += T) {
+      const c32 y = a.Yc[static_cast<int64_t>(j) * a.M + HALF];  // bin M/2: k2 = 0, k1 = N1/2
+      a.yhalf[j] = {y.x * a.oscale, -y.y * a.oscale};
+    }
+  }
+  const c32* srcA = a.Yc + static_cast<int64_t>(cA) * N1 + k1A;
+  const c32* srcB = a.Yc + static_cast<int64_t>(cB) * N1 + (nyq ? 0 : k1B);
+  constexpr int JG = ROWS / 2;
+  constexpr int NW = T / 64;
+  constexpr int RPW = ROWS / NW;
+  static_assert(RPW * NW == ROWS && RPW * fft_max_nr<N1>() <= 64, "one butterfly per lane and pass");
+  const int wv = tid >> 6, lane = tid & 63;
+  c32* cur = buf;
+  c32* nxt = buf2;
+  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
+    const int nj = min(JG, a.Nr - j0);
+    c32 ya[JG], yb[JG];
+#pragma unroll
+    for (int jj = 0; jj < JG; ++jj) {
+      if (jj < nj) {
+        const int64_t off = static_cast<int64_t>(j0 + jj) * a.M;
+        ya[jj] = srcA[off];
+        yb[jj] = nyq ? a.yM[j0 + jj] : srcB[off];
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < JG; ++jj) {
+      if (jj < nj) {
+        const c32 YA = {ya[jj].x * h, ya[jj].y * h}, YB = {yb[jj].x * h, yb[jj].y * h};
+        if (nyq) {
+          cur[(2 * jj + rowA) * P] = {YA.x + YB.x, YA.x - YB.x};
+          cur[(2 * jj + rowA) * P + lsw(HALF)] = {0.0f, 0.0f};
+        } else {
+          const c32 E = {YA.x + YB.x, YA.y - YB.y};
+          const c32 D = {YA.x - YB.x, YA.y + YB.y};
+          const c32 O = cmulc(D, WA);
+          cur[(2 * jj + rowA) * P + lsw(k1A)] = {E.x - O.y, E.y + O.x};
+          cur[(2 * jj + rowB) * P + lsw(k1B)] = {E.x + O.y, -E.y + O.x};
+        }
+      }
+    }
+    __syncthreads();
+    const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));
+    wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+      constexpr int RL = sizeof(v) / sizeof(c32);
+      const int row = wv * RPW + f;
+      c32* dst = a.B + static_cast<int64_t>(j0 + (row >> 1)) * (N1 * N2) + ((row & 1) ? colB : colA) * 16;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int n1 = jx + r * (N1 / RL);
+        dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
+      }
+    });
+    c32* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+}
+
 // ---------------------------------------------------------------- F unpack
 
 // Gt[t][k2][n1] = G[t][k2][n1] W_M^{n1 k2} (the four-step twiddle k2 applies
@@ -797,6 +992,8 @@ struct FastPlan {
   c32* tw2 = nullptr;   // W_N2^m
   c32* tabq = nullptr;  // e^{i pi dr j}
   c32* yhalf = nullptr;  // [Nr] bin M/2 term of the c2r (k2 -> k3)
+  c32* Yc = nullptr;     // fused output only: [Nr][N2][N1] spectra
+  c32* yM = nullptr;     // fused output only: [Nr] Nyquist bin
   bool have_B = false;
   int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
 };
@@ -840,7 +1037,8 @@ static std::vector<c32> twiddles(int N) {
 void fast_destroy(FastPlan* p) {
   if (p == nullptr) return;
   for (void* q : {static_cast<void*>(p->G), static_cast<void*>(p->B), static_cast<void*>(p->tw1),
-                  static_cast<void*>(p->tw2), static_cast<void*>(p->tabq), static_cast<void*>(p->yhalf)})
+                  static_cast<void*>(p->tw2), static_cast<void*>(p->tabq), static_cast<void*>(p->yhalf),
+                  static_cast<void*>(p->Yc), static_cast<void*>(p->yM)})
     if (q) (void)hipFree(q);
   delete p;
 }
@@ -1026,6 +1224,134 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st) {
     k3_output<16><<<grid, CF<16>::THREADS, 0, st>>>(a);
   }
   ZDT_CHECK_LAUNCH("k3_output");
+  return ZDT_OK;
+}
+
+// ---------------------------------------------------------------- fused output (host)
+
+bool fast_out_supported(int64_t Nf, int64_t Nt, int64_t Nr) {
+  if (Nf < (1 << 16) || Nf > (1 << 20) || (Nf & (Nf - 1)) != 0) return false;
+  return Nt >= 1 && Nt <= 64 && Nr >= 1 && Nr <= 65535;
+}
+
+int64_t fast_out_bytes(int64_t Nf, int64_t Nr) {
+  const int64_t M = Nf / 2;
+  const int64_t N2 = fast_n2(Nf);
+  return 16 * M * Nr + 8 * (M / N2 + N2 + 2 * Nr);
+}
+
+int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out) {
+  *out = nullptr;
+  if (!fast_out_supported(Nf, 1, Nr)) return fail(ZDT_ENOTSUP, "fused output: unsupported shape");
+  FastPlan* p = new FastPlan();
+  p->Nf = Nf;
+  p->Nt = 0;
+  p->Nr = Nr;
+  p->M = Nf / 2;
+  p->N2 = fast_n2(Nf);
+  p->logN2 = ilog2i(p->N2);
+  p->N1 = static_cast<int>(p->M / p->N2);
+  p->logN1 = ilog2i(p->N1);
+  {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    p->resident_blocks = 2 * static_cast<int64_t>(ncu);
+  }
+  int rc = ZDT_OK;
+  auto alloc = [&](c32** q, size_t n) {
+    if (rc != ZDT_OK) return;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(q), n * sizeof(c32));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      rc = fail(ZDT_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(c32), hipGetErrorString(e));
+    }
+  };
+  alloc(&p->Yc, static_cast<size_t>(Nr * p->M));
+  alloc(&p->B, static_cast<size_t>(Nr * p->M));
+  alloc(&p->yM, static_cast<size_t>(Nr));
+  alloc(&p->yhalf, static_cast<size_t>(Nr));
+  if (rc == ZDT_OK) rc = upload_table(&p->tw1, twiddles(p->N1));
+  if (rc == ZDT_OK) rc = upload_table(&p->tw2, twiddles(p->N2));
+  if (rc != ZDT_OK) {
+    fast_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return ZDT_OK;
+}
+
+template <int N1, int N2>
+static int launch_k2c(const K2cArgs& a, hipStream_t st) {
+  k2_c2r<N1, N2, k2_rows(N1)><<<N2 / 2, N1, 0, st>>>(a);
+  ZDT_CHECK_LAUNCH("k2_c2r");
+  return ZDT_OK;
+}
+
+int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, int Nt, float r0, float dr, int wkind, const float* window,
+                 float* dest, double scale, hipStream_t st) {
+  if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
+  CztColArgs c;
+  c.F = F;
+  c.ldF = ldF;
+  c.Yc = p->Yc;
+  c.yM = p->yM;
+  c.M = p->M;
+  c.Nf = p->Nf;
+  c.logN1 = p->logN1;
+  c.logN2 = p->logN2;
+  c.Nt = Nt;
+  c.Nr = static_cast<int>(p->Nr);
+  c.hd2 = static_cast<double>(dr) / static_cast<double>(p->Nf);
+  c.h0 = static_cast<double>(r0) / static_cast<double>(p->Nf);
+  c.wkind = wkind;
+  c.wcustom = window;
+  c.scale = 1.0f;
+  {
+    const unsigned grid = static_cast<unsigned>(p->M / 256 + 1);
+    ProfScope ps(st, "czt_horner_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
+    if (Nt <= 16) {
+      czt_horner_col<16><<<grid, 256, 0, st>>>(c);
+    } else if (Nt <= 32) {
+      czt_horner_col<32><<<grid, 256, 0, st>>>(c);
+    } else {
+      czt_horner_col<64><<<grid, 256, 0, st>>>(c);
+    }
+    ZDT_CHECK_LAUNCH("czt_horner_col");
+  }
+  K2cArgs a;
+  a.Yc = p->Yc;
+  a.yM = p->yM;
+  a.B = p->B;
+  a.yhalf = p->yhalf;
+  a.tw1 = p->tw1;
+  a.Nr = static_cast<int>(p->Nr);
+  a.M = p->M;
+  a.Nf = p->Nf;
+  a.oscale = static_cast<float>(scale / static_cast<double>(p->M));
+  {
+    ProfScope ps(st, "k2_c2r", 16.0 * p->M * p->Nr);
+    switch (p->N1 * 2048 + p->N2) {
+      case 64 * 2048 + 512:
+        ZDT_TRY((launch_k2c<64, 512>(a, st)));
+        break;
+      case 64 * 2048 + 1024:
+        ZDT_TRY((launch_k2c<64, 1024>(a, st)));
+        break;
+      case 128 * 2048 + 1024:
+        ZDT_TRY((launch_k2c<128, 1024>(a, st)));
+        break;
+      case 256 * 2048 + 1024:
+        ZDT_TRY((launch_k2c<256, 1024>(a, st)));
+        break;
+      case 512 * 2048 + 1024:
+        ZDT_TRY((launch_k2c<512, 1024>(a, st)));
+        break;
+      default:
+        return fail(ZDT_ENOTSUP, "fused output: N1=%d N2=%d", p->N1, p->N2);
+    }
+  }
+  p->have_B = true;
+  if (dest != nullptr) ZDT_TRY(fast_output(p, dest, scale, st));
   return ZDT_OK;
 }
 

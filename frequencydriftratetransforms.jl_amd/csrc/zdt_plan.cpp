@@ -127,8 +127,7 @@ int autotune(Shape& s, int flags) {
   for (const Cand& c : cands) {
     if (rc != ZDT_OK) break;
     float ms = INFINITY;
-    rc = time_candidate(s, c.fused ? flags : (flags | ZDT_FLAG_GENERIC), c.n2, c.maxr, static_cast<float*>(spec),
-                        static_cast<float*>(dest), &ms);
+    rc = time_candidate(s, flags, c.n2, c.maxr, static_cast<float*>(spec), static_cast<float*>(dest), &ms);
     if (rc == ZDT_OK && ms < best) {
       best = ms;
       s.fused = c.fused;
@@ -258,10 +257,10 @@ int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* 
     s.fused = (!(wflags & ZDT_FLAG_GENERIC) && fast_supported(s.Nf, s.Nt, s.Nr)) ? 1 : 0;
     if (flags & ZDT_PLAN_AUTOTUNE) rc = autotune(s, wflags);
     if (rc != ZDT_OK) break;
-    rc = create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, s.fused ? wflags : (wflags | ZDT_FLAG_GENERIC),
-                         s.st, s.fused ? s.n2 : 0, s.maxr, &s.ws);
+    rc = create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, wflags, s.st, s.fused ? s.n2 : 0, s.maxr,
+                         &s.ws);
     if (rc != ZDT_OK) break;
-    s.fused = ws_is_fused(s.ws) ? 1 : 0;
+    s.fused = ws_path(s.ws);
     rc = hip_alloc(reinterpret_cast<void**>(&s.stats), stats_doubles(s) * sizeof(double));
     if (rc == ZDT_OK) rc = hip_alloc(reinterpret_cast<void**>(&s.counts), (hits_tiles(s.Nf, s.Nr) + 1) * sizeof(int64_t));
   }
@@ -282,8 +281,8 @@ int zdt_plan_info(const zdt_plan* p, int ch, int* shape, int* path, int* n2, int
   const Shape& s = p->shapes[si];
   if (shape) *shape = si;
   if (path) *path = s.fused;
-  if (n2) *n2 = s.fused ? (s.n2 ? s.n2 : (s.Nf >= (int64_t(1) << 17) ? 1024 : 512)) : 0;
-  if (maxr) *maxr = s.fused ? 0 : s.maxr;
+  if (n2) *n2 = s.fused == 1 ? (s.n2 ? s.n2 : (s.Nf >= (int64_t(1) << 17) ? 1024 : 512)) : 0;
+  if (maxr) *maxr = s.fused == 1 ? 0 : s.maxr;
   return ZDT_OK;
 }
 

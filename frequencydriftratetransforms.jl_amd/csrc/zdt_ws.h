@@ -13,5 +13,7 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
 
 // true when the workspace runs the fused kernels
 bool ws_is_fused(const zdt_ws* ws);
+// 1: fused kernels; 2: per-bin chirp-Z with the fused output; 0: generic
+int ws_path(const zdt_ws* ws);
 
 }  // namespace zdt

@@ -79,8 +79,8 @@ class RaggedPlan:
         """Tuned choices for channel c: shape index, path, n2 (fused), maxr (generic)."""
         v = [C.c_int() for _ in range(4)]
         _lib.check(_lib.lib().zdt_plan_info(self._h, int(c), *[C.byref(x) for x in v]))
-        return {"shape": v[0].value, "path": "fused" if v[1].value else "generic", "n2": v[2].value,
-                "maxr": v[3].value}
+        return {"shape": v[0].value, "path": {1: "fused", 2: "fused-output"}.get(v[1].value, "generic"),
+                "n2": v[2].value, "maxr": v[3].value}
 
     def bind(self, thresholds=10.0, *, snr: bool = True, cap: int = 4096, keep_fdrs: bool = True) -> None:
         """Captures the graph.  thresholds: one value or one per channel (SNRs

@@ -225,7 +225,9 @@ int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* 
 int zdt_plan_destroy(zdt_plan* p);
 
 /* The choices made for channel c: shape index (channels with one shape share
- * a workspace), path (1 = fused, 0 = generic), n2 (fused) and maxr (generic). */
+ * a workspace), path (1 = fused; 2 = per-bin chirp-Z with the fused output,
+ * power-of-two Nf with 16 < Nt <= 64; 0 = generic), n2 (fused) and maxr
+ * (the radix cap of the other paths' FFTs). */
 int zdt_plan_info(const zdt_plan* p, int ch, int* shape, int* path, int* n2, int* maxr);
 
 /* Binds device buffers and captures the graph (no work is done yet).

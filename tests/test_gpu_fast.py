@@ -126,3 +126,32 @@ def test_fast_unnormalized_and_hits(F):
     ws2 = F.ZDTWorkspace(s, -4.0, 0.25, 33)
     fdr = F.zdtfdr(ws2)
     np.testing.assert_array_equal(F.findprotohits(fdr, 10, snr=True), H.findprotohits(host(fdr), 10, snr=True))
+
+
+# ---- per-bin chirp-Z with the fused output (power-of-two Nf, 16 < Nt <= 64)
+
+@pytest.mark.parametrize("Nf,Nt", [(1 << 16, 17), (1 << 17, 32), (1 << 19, 60), (1 << 16, 64)])
+def test_fused_output_vs_fp64(F, Nf, Nt):
+    Nr = 33
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.37, 1.25, 25.0), (Nf * 0.81, -2.0, 30.0)], seed=Nf % 991 + Nt)
+    say(f"fused output Nf={Nf} Nt={Nt}")
+    out = host(F.zdtfdr(F.ZDTWorkspace(s, -2.0, 0.125, Nr)))
+    assert normwise(out, Z.fdr64(s, -2.0, 0.125, Nr)) <= 2e-6
+
+
+def test_fused_output_window_nodest_unnormalized(F):
+    Nf, Nt = 1 << 17, 40
+    s = synth.spectrogram(Nf, Nt, [(40000.0, 0.75, 30.0)], seed=17)
+    say("fused output: hamming, no-dest + output!, unnormalized")
+    ws = F.ZDTWorkspace(s, -1.0, 0.0625, 33)
+    ref = Z.fdr64(s, -1.0, 0.0625, 33, window="hamming")
+    assert normwise(host(F.zdtfdr("hamming", ws)), ref) <= 2e-6
+    assert F.zdtfdr_(ws, r0=-1.0) is None
+    dest = F.create_fdr(s, 33)
+    F.output_(dest, ws)
+    F.output_(dest, ws)
+    assert normwise(host(dest), Z.fdr64(s, -1.0, 0.0625, 33)) <= 2e-6
+    wu = F.ZDTWorkspace(s, -1.0, 0.0625, 33, unnormalized=True)
+    assert normwise(host(F.zdtfdr(wu)) / (Nf * wu.Nl), Z.fdr64(s, -1.0, 0.0625, 33)) <= 2e-6
+    g = host(F.zdtfdr(F.ZDTWorkspace(s, -1.0, 0.0625, 33, generic=True)))
+    assert normwise(host(F.zdtfdr(ws)), g.astype(np.float64)) <= 2e-6

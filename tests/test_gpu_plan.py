@@ -76,7 +76,7 @@ def test_plan_replays_match_oracle(F):
     infos = [plan.info(c) for c in range(len(CHANNELS))]
     say(f"plan: {infos}")
     assert infos[0]["shape"] == infos[2]["shape"] and infos[0]["shape"] != infos[1]["shape"]
-    assert [i["path"] for i in infos] == ["fused", "fused", "fused", "generic", "generic"]
+    assert [i["path"] for i in infos] == ["fused", "fused", "fused", "generic", "fused-output"]
     assert infos[0]["n2"] == 512 and infos[1]["n2"] in (512, 1024)
     assert infos[3]["maxr"] in (8, 16) and infos[4]["maxr"] in (8, 16)
     plan.bind(8.0, snr=True, cap=4096)
