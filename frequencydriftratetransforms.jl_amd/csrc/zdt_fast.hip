@@ -440,8 +440,18 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
       const unsigned mp = r + R1 * hh;
       c32 y[R1];
       cf_stage2<R1>(y, ex, c, mp, true);
+      {
+        // the FDR is not read again: nontemporal stores keep it from evicting
+        // the next step's inputs (spectrogram, G) from the last-level cache
+        typedef float f2v __attribute__((ext_vector_type(2)));
+        f2v* q = reinterpret_cast<f2v*>(out + (mp << a.logN1) + o);
+        const int64_t step = int64_t(32) << a.logN1;
 #pragma unroll
-      for (int rp = 0; rp < R1; ++rp) out[((mp + 32 * rp) << a.logN1) + o] = y[rp];
+        for (int rp = 0; rp < R1; ++rp) {
+          const f2v v = {y[rp].x, y[rp].y};
+          __builtin_nontemporal_store(v, q + rp * step);
+        }
+      }
     }
     __syncthreads();
     tile = next;
