@@ -52,69 +52,6 @@ constexpr int fft_radix() {
   return (N / NS == 16 || (NS == 1 && ilog2c(N) % 3 != 0)) ? 16 : (N / NS >= 8 ? 8 : N / NS);
 }
 
-// One Stockham radix-R pass over nrows length-N FFTs held in LDS (row f at
-// buf + f*P).  All threads read their butterflies' inputs, a barrier
-// separates reads from writes, and a barrier ends the pass.
-template <int N, int R, int T, int MAXROWS, bool INV>
-__device__ __forceinline__ void lds_pass(c32* buf, int P, int nrows, int Ns, const c32* __restrict__ tw) {
-  constexpr int NR = N / R;
-  constexpr int LOGNR = ilog2c(NR);
-  constexpr int LOGN = ilog2c(N);
-  constexpr int LOGR = ilog2c(R);
-  constexpr int MAXB = (MAXROWS * NR + T - 1) / T;
-  const int logNs = 31 - __builtin_clz(Ns);
-  const int total = nrows * NR;
-  c32 v[MAXB][R];
-  int dst[MAXB];
-#pragma unroll
-  for (int i = 0; i < MAXB; ++i) {
-    const int b = threadIdx.x + i * T;
-    if (b < total) {
-      const int f = b >> LOGNR;
-      const int j = b & (NR - 1);
-      const c32* x = buf + f * P;
-      const int k = j & (Ns - 1);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        v[i][r] = x[j + r * NR];
-        if (INV) v[i][r].y = -v[i][r].y;
-      }
-      if (Ns > 1) {
-        const int sh = LOGN - logNs - LOGR;
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[(r * k) << sh]);
-      }
-      dft<float, R>(v[i]);
-      dst[i] = f * P + ((j >> logNs) << (logNs + LOGR)) + k;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < MAXB; ++i) {
-    const int b = threadIdx.x + i * T;
-    if (b < total) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        c32 y = v[i][r];
-        if (INV) y.y = -y.y;
-        buf[dst[i] + r * Ns] = y;
-      }
-    }
-  }
-  __syncthreads();
-}
-
-// radix schedule: 16s first, then the remainder
-template <int N, int T, int MAXROWS, bool INV, int NS = 1>
-__device__ __forceinline__ void lds_fft(c32* buf, int P, int nrows, const c32* __restrict__ tw) {
-  if constexpr (NS < N) {
-    constexpr int REM = N / NS;
-    constexpr int R = REM >= 16 ? 16 : REM;
-    lds_pass<N, R, T, MAXROWS, INV>(buf, P, nrows, NS, tw);
-    lds_fft<N, T, MAXROWS, INV, NS * R>(buf, P, nrows, tw);
-  }
-}
-
 // LDS row swizzle: one pad entry after every 8, so the strided Stockham
 // writes of the early passes spread over the banks.  Rows of length N occupy
 // N + N/8 entries.
