@@ -877,6 +877,102 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   }
 }
 
+// k2_c2r for N1 = 1024, 2048 (Nf = 2^21, 2^22): 1024 threads, PP = N1/1024
+// pairs each, the inverse DFT_N1 as block-level ping-pong passes (the rows are
+// too long for one wavefront), twiddles read from the global table
+template <int N1, int N2, int ROWS>
+__global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
+  constexpr int T = 1024;
+  constexpr int PP = N1 / T;
+  constexpr int P = N1 + N1 / 8;
+  constexpr int HALF = N1 / 2;
+  __shared__ c32 buf[ROWS * P];
+  __shared__ c32 buf2[ROWS * P];
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x;
+  const bool special = (p == 0);
+  const int colA = special ? 0 : p;
+  const int colB = special ? N2 / 2 : N2 - p;
+  int rowA[PP], rowB[PP], k1A[PP], k1B[PP];
+  bool nyq[PP];
+  c32 WA[PP];
+  const c32* srcA[PP];
+  const c32* srcB[PP];
+#pragma unroll
+  for (int i = 0; i < PP; ++i) {
+    const int q = tid + i * T;
+    int cA, cB;
+    nyq[i] = false;
+    if (!special) {
+      rowA[i] = 0;
+      rowB[i] = 1;
+      k1A[i] = q;
+      k1B[i] = N1 - 1 - q;
+      cA = colA;
+      cB = colB;
+    } else if (q < HALF) {
+      rowA[i] = rowB[i] = 0;
+      k1A[i] = q;
+      k1B[i] = N1 - q;
+      nyq[i] = (q == 0);
+      cA = cB = 0;
+    } else {
+      rowA[i] = rowB[i] = 1;
+      k1A[i] = q - HALF;
+      k1B[i] = N1 - 1 - k1A[i];
+      cA = cB = N2 / 2;
+    }
+    const int64_t kA = cA + static_cast<int64_t>(N2) * k1A[i];
+    WA[i] = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));
+    srcA[i] = a.Yc + static_cast<int64_t>(cA) * N1 + k1A[i];
+    srcB[i] = a.Yc + static_cast<int64_t>(cB) * N1 + (nyq[i] ? 0 : k1B[i]);
+  }
+  const float h = 0.5f * a.oscale;
+  if (special) {
+    for (int j = tid; j < a.Nr; j += T) {
+      const c32 y = a.Yc[static_cast<int64_t>(j) * a.M + HALF];
+      a.yhalf[j] = {y.x * a.oscale, -y.y * a.oscale};
+    }
+  }
+  constexpr int JG = ROWS / 2;
+  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
+    const int nj = min(JG, a.Nr - j0);
+#pragma unroll
+    for (int jj = 0; jj < JG; ++jj) {
+      if (jj < nj) {
+        const int64_t off = static_cast<int64_t>(j0 + jj) * a.M;
+#pragma unroll
+        for (int i = 0; i < PP; ++i) {
+          const c32 ya = srcA[i][off];
+          const c32 yb = nyq[i] ? a.yM[j0 + jj] : srcB[i][off];
+          const c32 YA = {ya.x * h, ya.y * h}, YB = {yb.x * h, yb.y * h};
+          if (nyq[i]) {
+            buf[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
+            buf[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
+          } else {
+            const c32 E = {YA.x + YB.x, YA.y - YB.y};
+            const c32 D = {YA.x - YB.x, YA.y + YB.y};
+            const c32 O = cmulc(D, WA[i]);
+            buf[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
+            buf[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, a.tw1);
+    for (int r = 0; r < 2 * nj; ++r) {
+      c32* dst = a.B + static_cast<int64_t>(j0 + (r >> 1)) * (N1 * N2) + ((r & 1) ? colB : colA) * 16;
+#pragma unroll
+      for (int i = 0; i < PP; ++i) {
+        const int n1 = tid + i * T;
+        dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = fo[r * P + lsw(n1)];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- F unpack
 
 // Gt[t][k2][n1] = G[t][k2][n1] W_M^{n1 k2} (the four-step twiddle k2 applies
@@ -1177,7 +1273,7 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st) {
 // ---------------------------------------------------------------- fused output (host)
 
 bool fast_out_supported(int64_t Nf, int64_t Nt, int64_t Nr) {
-  if (Nf < (1 << 16) || Nf > (1 << 20) || (Nf & (Nf - 1)) != 0) return false;
+  if (Nf < (1 << 16) || Nf > (1 << 22) || (Nf & (Nf - 1)) != 0) return false;
   return Nt >= 1 && Nt <= 64 && Nr >= 1 && Nr <= 65535;
 }
 
@@ -1229,7 +1325,12 @@ int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out) {
 
 template <int N1, int N2>
 static int launch_k2c(const K2cArgs& a, hipStream_t st) {
-  k2_c2r<N1, N2, k2_rows(N1)><<<N2 / 2, N1, 0, st>>>(a);
+  if constexpr (N1 <= 512) {
+    k2_c2r<N1, N2, k2_rows(N1)><<<N2 / 2, N1, 0, st>>>(a);
+  } else {
+    // rows of 2^10 or 2^11 entries: 8 or 4 rows per ping-pong buffer
+    k2_c2r_big<N1, N2, (N1 == 1024 ? 8 : 4)><<<N2 / 2, 1024, 0, st>>>(a);
+  }
   ZDT_CHECK_LAUNCH("k2_c2r");
   return ZDT_OK;
 }
@@ -1292,6 +1393,12 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, int Nt, float r0, float
         break;
       case 512 * 2048 + 1024:
         ZDT_TRY((launch_k2c<512, 1024>(a, st)));
+        break;
+      case 1024 * 2048 + 1024:
+        ZDT_TRY((launch_k2c<1024, 1024>(a, st)));
+        break;
+      case 2048 * 2048 + 1024:
+        ZDT_TRY((launch_k2c<2048, 1024>(a, st)));
         break;
       default:
         return fail(ZDT_ENOTSUP, "fused output: N1=%d N2=%d", p->N1, p->N2);

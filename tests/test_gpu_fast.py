@@ -155,3 +155,11 @@ def test_fused_output_window_nodest_unnormalized(F):
     assert normwise(host(F.zdtfdr(wu)) / (Nf * wu.Nl), Z.fdr64(s, -1.0, 0.0625, 33)) <= 2e-6
     g = host(F.zdtfdr(F.ZDTWorkspace(s, -1.0, 0.0625, 33, generic=True)))
     assert normwise(host(F.zdtfdr(ws)), g.astype(np.float64)) <= 2e-6
+
+
+@pytest.mark.parametrize("Nf,Nt", [(1 << 21, 17), (1 << 22, 20)])
+def test_fused_output_large_nf(F, Nf, Nt):
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.62, 3.0, 30.0), (Nf * 0.05, -1.5, 25.0)], seed=Nt)
+    say(f"fused output Nf={Nf} Nt={Nt}")
+    out = host(F.zdtfdr(F.ZDTWorkspace(s, -1.5, 1.5, 3)))
+    assert normwise(out, Z.fdr64(s, -1.5, 1.5, 3)) <= 2e-6
