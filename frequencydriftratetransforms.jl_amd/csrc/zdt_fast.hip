@@ -639,7 +639,9 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   c32* nxt = buf2;
   for (int j0 = 0; j0 < a.Nr; j0 += JG) {
     const int nj = min(JG, a.Nr - j0);
-    for (int jj = 0; jj < nj; ++jj) {
+#pragma unroll
+    for (int jj = 0; jj < JG; ++jj) {
+      if (jj >= nj) break;  // uniform
       const c32 qj = a.tabq[j0 + jj];
 #pragma unroll
       for (int i = 0; i < PP; ++i) {
