@@ -40,6 +40,19 @@ PKG = ROOT / "frequencydriftratetransforms.jl_amd"
 sys.path.insert(0, str(PKG))
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (the rate with packed FP32; 78.6 without)
+
+
+def k2_flops(Nf: int, Nt: int, Nr: int, N2: int = 1024) -> float:
+    """Algorithmic FP32 flops of one k2_czt launch: Horner (Nt-1 complex
+    multiply-adds, 8 flops, per bin and rate), the inverse DFT_N1 of every
+    column and rate and the forward one of every column and time step
+    (5 N log2 N each), and ~30 flops per bin and rate of c2r pre-pass,
+    power recurrence and twiddles."""
+    M = Nf // 2
+    N1 = M // N2
+    fft = 5.0 * N1 * np.log2(N1)
+    return (M + 1) * Nr * (Nt - 1) * 8.0 + N2 * (Nr + Nt) * fft + (M + 1) * Nr * 30.0
 NF, NT = 1 << 20, 16
 R0, DR, NR = -4.0, 0.0625, 129
 
@@ -318,6 +331,13 @@ def main(argv=None) -> int:
         print("bench.py: result check failed", file=sys.stderr)
         return 1
 
+    # the VALU-bound kernel against the FP32 vector peak (algorithmic flops)
+    compute = None
+    k2_us = [ms / n_ * 1e3 for k, ms, n_, by in kern if k == "k2_czt"]
+    if k2_us:
+        tf = k2_flops(NF, NT, NR) / (k2_us[0] * 1e-6) / 1e12
+        compute = {"kernel": "k2_czt", "bound": "valu", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "frac": tf / FP32_PEAK_TFLOPS, "flops_per_launch": k2_flops(NF, NT, NR)}
     out = {
         "metric": "FDR cells/s (Nf x Nrates) at 2^20ch x 16t x 129 rates, FP32 ZDT",
         "value": NF * NR / dt,
@@ -339,6 +359,7 @@ def main(argv=None) -> int:
                      "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": dbytes,
                      "timing": f"HIP events around each launch on the workspace stream, {a.steps} steps "
                                f"right before the timed ones ({dt_prof * 1e3:.4f} ms/step with the events)"},
+        "compute": compute,
         "kernels": [{"name": k, "ms_per_step": ms / a.steps, "launches_per_step": n_ / a.steps,
                      "alg_GBps": (by / (ms / n_ / 1e3) / 1e9) if ms > 0 else None} for k, ms, n_, by in kern],
     }
