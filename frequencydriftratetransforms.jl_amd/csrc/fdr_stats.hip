@@ -21,9 +21,16 @@ namespace zdt {
 constexpr int STAT_THREADS = 256;
 constexpr int STAT_ROWS = 16384;  // rows per partial-sum block
 
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// one block per (chunk of STAT_ROWS rows, column); vec: the column is
+// 16-byte aligned (float4 loads)
 __global__ __launch_bounds__(STAT_THREADS) void colstats_partial(const float* __restrict__ fdr, int64_t Nf,
                                                                  int64_t ld, double* __restrict__ part,
-                                                                 int nchunk) {
+                                                                 int nchunk, int vec) {
   const int chunk = blockIdx.x;
   const int64_t j = blockIdx.y;
   const float* col = fdr + j * ld;
@@ -32,25 +39,42 @@ __global__ __launch_bounds__(STAT_THREADS) void colstats_partial(const float* __
   int64_t r1 = r0 + STAT_ROWS;
   if (r1 > Nf) r1 = Nf;
   double s1 = 0.0, s2 = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += STAT_THREADS) {
-    const double d = static_cast<double>(col[i]) - c;
-    s1 += d;
-    s2 += d * d;
-  }
-  __shared__ double sh1[STAT_THREADS], sh2[STAT_THREADS];
-  sh1[threadIdx.x] = s1;
-  sh2[threadIdx.x] = s2;
-  __syncthreads();
-  for (int w = STAT_THREADS / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      sh1[threadIdx.x] += sh1[threadIdx.x + w];
-      sh2[threadIdx.x] += sh2[threadIdx.x + w];
+  if (vec && r1 - r0 == STAT_ROWS) {
+    const float4* p = reinterpret_cast<const float4*>(col + r0);
+    constexpr int NV = STAT_ROWS / 4 / STAT_THREADS;
+#pragma unroll 8
+    for (int k = 0; k < NV; ++k) {
+      const float4 v = p[threadIdx.x + k * STAT_THREADS];
+      const double d0 = static_cast<double>(v.x) - c, d1 = static_cast<double>(v.y) - c;
+      const double d2 = static_cast<double>(v.z) - c, d3 = static_cast<double>(v.w) - c;
+      s1 += (d0 + d1) + (d2 + d3);
+      s2 += (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
     }
-    __syncthreads();
+  } else {
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += STAT_THREADS) {
+      const double d = static_cast<double>(col[i]) - c;
+      s1 += d;
+      s2 += d * d;
+    }
   }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  constexpr int NW = STAT_THREADS / 64;
+  __shared__ double sh1[NW], sh2[NW];
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh1[wv] = s1;
+    sh2[wv] = s2;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    part[(j * nchunk + chunk) * 2 + 0] = sh1[0];
-    part[(j * nchunk + chunk) * 2 + 1] = sh2[0];
+    double t1 = 0.0, t2 = 0.0;
+    for (int w = 0; w < NW; ++w) {
+      t1 += sh1[w];
+      t2 += sh2[w];
+    }
+    part[(j * nchunk + chunk) * 2 + 0] = t1;
+    part[(j * nchunk + chunk) * 2 + 1] = t2;
   }
 }
 
@@ -78,7 +102,8 @@ int launch_colstats2(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, doubl
                      double* colvar, hipStream_t st) {
   const int nchunk = colstats_nchunk(Nf);
   dim3 grid(nchunk, static_cast<unsigned>(Nr));
-  colstats_partial<<<grid, STAT_THREADS, 0, st>>>(fdr, Nf, ld, part, nchunk);
+  const int vec = (ld % 4 == 0 && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
+  colstats_partial<<<grid, STAT_THREADS, 0, st>>>(fdr, Nf, ld, part, nchunk, vec);
   ZDT_CHECK_LAUNCH("colstats_partial");
   colstats_final<<<static_cast<unsigned>((Nr + 255) / 256), 256, 0, st>>>(fdr, Nf, Nr, ld, part, nchunk,
                                                                           colmean, colvar);
@@ -121,29 +146,104 @@ __device__ __forceinline__ float load_lin(const float* fdr, int64_t Nf, int64_t 
   return fdr[(g % Nf) + (g / Nf) * ld];
 }
 
+// vec: the matrix is contiguous (ld == Nf) and 16-byte aligned
 __global__ __launch_bounds__(HIT_THREADS) void hits_count(const float* __restrict__ fdr, int64_t Nf, int64_t n,
                                                           int64_t ld, double thr_h, const double* thr_d,
-                                                          int64_t* __restrict__ counts) {
+                                                          int64_t* __restrict__ counts, int vec) {
   const double thr = thr_d ? *thr_d : thr_h;
   const int64_t tile = blockIdx.x;
   const int64_t base = tile * HIT_TILE;
   int c = 0;
+  if (vec && base + HIT_TILE <= n) {
+    const float4* p = reinterpret_cast<const float4*>(fdr + base);
+    float4 v[HIT_PER_THREAD / 4];
+#pragma unroll
+    for (int i = 0; i < HIT_PER_THREAD / 4; ++i) v[i] = p[threadIdx.x + i * HIT_THREADS];
+#pragma unroll
+    for (int i = 0; i < HIT_PER_THREAD / 4; ++i)
+      c += (static_cast<double>(v[i].x) >= thr) + (static_cast<double>(v[i].y) >= thr) +
+           (static_cast<double>(v[i].z) >= thr) + (static_cast<double>(v[i].w) >= thr);
+  } else {
 #pragma unroll 4
-  for (int i = 0; i < HIT_PER_THREAD; ++i) {
-    const int64_t g = base + static_cast<int64_t>(i) * HIT_THREADS + threadIdx.x;
-    if (g < n && static_cast<double>(load_lin(fdr, Nf, ld, g)) >= thr) ++c;
+    for (int i = 0; i < HIT_PER_THREAD; ++i) {
+      const int64_t g = base + static_cast<int64_t>(i) * HIT_THREADS + threadIdx.x;
+      if (g < n && static_cast<double>(load_lin(fdr, Nf, ld, g)) >= thr) ++c;
+    }
   }
-  __shared__ int sh[HIT_THREADS];
-  sh[threadIdx.x] = c;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  constexpr int NW = HIT_THREADS / 64;
+  __shared__ int sh[NW];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
   __syncthreads();
-  for (int w = HIT_THREADS / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < NW; ++w) t += sh[w];
+    counts[tile] = t;
   }
-  if (threadIdx.x == 0) counts[tile] = sh[0];
 }
 
-// exclusive scan of counts[0..n) in place, total in *total (one block)
+// Exclusive scan of the tile counts in three steps: each block of
+// SCAN_THREADS scans SCAN_CHUNK counts locally and writes its total
+// (hits_scan_local), one block scans the totals (hits_scan), and the totals'
+// offsets are added back (hits_scan_add).
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_PER_THREAD = 8;
+constexpr int64_t SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
+
+int64_t hits_scan_scratch(int64_t n) { return (n + SCAN_CHUNK - 1) / SCAN_CHUNK; }
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v, int lane) {
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void hits_scan_local(int64_t* __restrict__ counts, int64_t n,
+                                                                int64_t* __restrict__ btot) {
+  constexpr int NW = SCAN_THREADS / 64;
+  __shared__ int64_t sh[NW + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+  int64_t v[SCAN_PER_THREAD];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+    v[i] = base + i < n ? counts[base + i] : 0;
+    s += v[i];
+  }
+  const int64_t incl = wave_incl_scan(s, lane);
+  if (lane == 63) sh[wv] = incl;
+  __syncthreads();
+  int64_t w = 0, wi = 0;
+  if (wv == 0) {
+    w = lane < NW ? sh[lane] : 0;
+    wi = wave_incl_scan(w, lane);
+  }
+  __syncthreads();
+  if (wv == 0) {
+    if (lane < NW) sh[lane] = wi - w;  // exclusive offsets of the wavefronts
+    if (lane == NW - 1) sh[NW] = wi;   // block total
+  }
+  __syncthreads();
+  int64_t run = sh[wv] + incl - s;
+#pragma unroll
+  for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+    if (base + i < n) counts[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 0) btot[blockIdx.x] = sh[NW];
+}
+
+__global__ __launch_bounds__(256) void hits_scan_add(int64_t* __restrict__ counts, int64_t n,
+                                                     const int64_t* __restrict__ boff) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (g < n) counts[g] += boff[g / SCAN_CHUNK];
+}
+
+// exclusive scan of counts[0..n) in place, total in *total (one block; used
+// on the per-chunk totals)
 __global__ __launch_bounds__(1024) void hits_scan(int64_t* __restrict__ counts, int64_t n,
                                                   int64_t* __restrict__ total) {
   __shared__ int64_t sh[1024];
@@ -172,7 +272,7 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restric
                                                           int64_t ld, double thr_h, const double* thr_d,
                                                           const int64_t* __restrict__ offsets, int64_t ntiles,
                                                           const int64_t* __restrict__ total, int64_t base_index,
-                                                          int64_t* __restrict__ hits, int64_t cap) {
+                                                          int64_t* __restrict__ hits, int64_t cap, int vec) {
   const double thr = thr_d ? *thr_d : thr_h;
   const int64_t tile = blockIdx.x;
   const int64_t start = offsets[tile];
@@ -182,10 +282,22 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restric
   // the per-thread counts gives each hit its ordered slot
   const int64_t base = tile * HIT_TILE + static_cast<int64_t>(threadIdx.x) * HIT_PER_THREAD;
   unsigned mask = 0;
+  if (vec && tile * HIT_TILE + HIT_TILE <= n) {
+    const float4* p = reinterpret_cast<const float4*>(fdr + base);
 #pragma unroll
-  for (int i = 0; i < HIT_PER_THREAD; ++i) {
-    const int64_t g = base + i;
-    if (g < n && static_cast<double>(load_lin(fdr, Nf, ld, g)) >= thr) mask |= 1u << i;
+    for (int i = 0; i < HIT_PER_THREAD / 4; ++i) {
+      const float4 v = p[i];
+      mask |= (static_cast<double>(v.x) >= thr ? 1u : 0u) << (4 * i);
+      mask |= (static_cast<double>(v.y) >= thr ? 1u : 0u) << (4 * i + 1);
+      mask |= (static_cast<double>(v.z) >= thr ? 1u : 0u) << (4 * i + 2);
+      mask |= (static_cast<double>(v.w) >= thr ? 1u : 0u) << (4 * i + 3);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < HIT_PER_THREAD; ++i) {
+      const int64_t g = base + i;
+      if (g < n && static_cast<double>(load_lin(fdr, Nf, ld, g)) >= thr) mask |= 1u << i;
+    }
   }
   __shared__ int sh[HIT_THREADS];
   const int c = __popc(mask);
@@ -210,14 +322,24 @@ int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, doub
                       int64_t* counts, hipStream_t st) {
   const int64_t nt = hits_tiles(Nf, Nr);
   if (nt == 0) return ZDT_OK;
-  hits_count<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev, counts);
+  const int vec = (ld == Nf && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
+  hits_count<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev, counts, vec);
   ZDT_CHECK_LAUNCH("hits_count");
   return ZDT_OK;
 }
 
-int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, hipStream_t st) {
-  hits_scan<<<1, 1024, 0, st>>>(counts, n, total);
+int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, int64_t* scratch, hipStream_t st) {
+  const int64_t nb = hits_scan_scratch(n);
+  if (nb > 0) {
+    hits_scan_local<<<static_cast<unsigned>(nb), SCAN_THREADS, 0, st>>>(counts, n, scratch);
+    ZDT_CHECK_LAUNCH("hits_scan_local");
+  }
+  hits_scan<<<1, 1024, 0, st>>>(scratch, nb, total);
   ZDT_CHECK_LAUNCH("hits_scan");
+  if (nb > 1) {
+    hits_scan_add<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(counts, n, scratch);
+    ZDT_CHECK_LAUNCH("hits_scan_add");
+  }
   return ZDT_OK;
 }
 
@@ -229,9 +351,10 @@ int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld,
   if (nt == 0) return ZDT_OK;
   // offsets of this matrix's tiles start at tile0 of the global scan; the end
   // of its last tile is the next matrix's first offset (or the total)
+  const int vec = (ld == Nf && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
   hits_write<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev,
                                                                 offsets + tile0, ntiles_all - tile0, total_dev,
-                                                                base_index, hits, cap);
+                                                                base_index, hits, cap, vec);
   ZDT_CHECK_LAUNCH("hits_write");
   return ZDT_OK;
 }

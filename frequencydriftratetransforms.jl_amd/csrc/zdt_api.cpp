@@ -814,13 +814,14 @@ int zdt_findprotohits(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr, 
   const int64_t nt = hits_tiles(Nf, Nr);
   const int64_t ntall = nt * nb;
   int64_t* buf = nullptr;
-  ZDT_TRY_HIP(hipMallocAsync(reinterpret_cast<void**>(&buf), (ntall + 1) * sizeof(int64_t), st));
+  ZDT_TRY_HIP(hipMallocAsync(reinterpret_cast<void**>(&buf), (ntall + 1 + hits_scan_scratch(ntall)) * sizeof(int64_t),
+                              st));
   int64_t* counts = buf;
   int64_t* total_dev = buf + ntall;
   int rc = ZDT_OK;
   for (int b = 0; b < nb && rc == ZDT_OK; ++b)
     rc = launch_hits_count(fdrs[b], Nf, Nr, ld, threshold, nullptr, counts + b * nt, st);
-  if (rc == ZDT_OK) rc = launch_hits_scan(counts, ntall, total_dev, st);
+  if (rc == ZDT_OK) rc = launch_hits_scan(counts, ntall, total_dev, total_dev + 1, st);
   int64_t total = 0;
   if (rc == ZDT_OK) {
     hipError_t e = hipMemcpyAsync(&total, total_dev, sizeof(int64_t), hipMemcpyDeviceToHost, st);

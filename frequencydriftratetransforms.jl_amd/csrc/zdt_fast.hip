@@ -834,17 +834,23 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   const int wv = tid >> 6, lane = tid & 63;
   c32* cur = buf;
   c32* nxt = buf2;
-  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
-    const int nj = min(JG, a.Nr - j0);
-    c32 ya[JG], yb[JG];
+  // the next group's spectra are loaded right after this group's barrier, so
+  // their latency overlaps the inverse DFT
+  c32 ya[JG], yb[JG];
+  auto load = [&](int j0n) {
+    const int njn = min(JG, a.Nr - j0n);
 #pragma unroll
     for (int jj = 0; jj < JG; ++jj) {
-      if (jj < nj) {
-        const int64_t off = static_cast<int64_t>(j0 + jj) * a.M;
+      if (jj < njn) {
+        const int64_t off = static_cast<int64_t>(j0n + jj) * a.M;
         ya[jj] = srcA[off];
-        yb[jj] = nyq ? a.yM[j0 + jj] : srcB[off];
+        yb[jj] = nyq ? a.yM[j0n + jj] : srcB[off];
       }
     }
+  };
+  load(0);
+  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
+    const int nj = min(JG, a.Nr - j0);
 #pragma unroll
     for (int jj = 0; jj < JG; ++jj) {
       if (jj < nj) {
@@ -862,6 +868,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       }
     }
     __syncthreads();
+    if (j0 + JG < a.Nr) load(j0 + JG);
     const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));
     wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
       constexpr int RL = sizeof(v) / sizeof(c32);
@@ -937,17 +944,32 @@ __global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
     }
   }
   constexpr int JG = ROWS / 2;
+  // the next group's spectra are loaded into registers before this group's
+  // DFT, so their latency overlaps the LDS passes
+  c32 ya[JG][PP], yb[JG][PP];
+  auto load = [&](int j0n) {
+    const int njn = min(JG, a.Nr - j0n);
+#pragma unroll
+    for (int jj = 0; jj < JG; ++jj) {
+      if (jj < njn) {
+        const int64_t off = static_cast<int64_t>(j0n + jj) * a.M;
+#pragma unroll
+        for (int i = 0; i < PP; ++i) {
+          ya[jj][i] = srcA[i][off];
+          yb[jj][i] = nyq[i] ? a.yM[j0n + jj] : srcB[i][off];
+        }
+      }
+    }
+  };
+  load(0);
   for (int j0 = 0; j0 < a.Nr; j0 += JG) {
     const int nj = min(JG, a.Nr - j0);
 #pragma unroll
     for (int jj = 0; jj < JG; ++jj) {
       if (jj < nj) {
-        const int64_t off = static_cast<int64_t>(j0 + jj) * a.M;
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
-          const c32 ya = srcA[i][off];
-          const c32 yb = nyq[i] ? a.yM[j0 + jj] : srcB[i][off];
-          const c32 YA = {ya.x * h, ya.y * h}, YB = {yb.x * h, yb.y * h};
+          const c32 YA = {ya[jj][i].x * h, ya[jj][i].y * h}, YB = {yb[jj][i].x * h, yb[jj][i].y * h};
           if (nyq[i]) {
             buf[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
             buf[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
@@ -962,6 +984,7 @@ __global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
       }
     }
     __syncthreads();
+    if (j0 + JG < a.Nr) load(j0 + JG);
     const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, a.tw1);
     for (int r = 0; r < 2 * nj; ++r) {
       c32* dst = a.B + static_cast<int64_t>(j0 + (r >> 1)) * (N1 * N2) + ((r & 1) ? colB : colA) * 16;

@@ -51,7 +51,9 @@ int64_t hits_tiles(int64_t Nf, int64_t Nr);
 // thr_dev, when not NULL, holds the threshold on the device (thr is ignored)
 int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, const double* thr_dev,
                       int64_t* counts, hipStream_t st);
-int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, hipStream_t st);
+// scratch: hits_scan_scratch(n) int64 words
+int64_t hits_scan_scratch(int64_t n);
+int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, int64_t* scratch, hipStream_t st);
 int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr,
                            const double* thr_dev, const int64_t* offsets, int64_t ntiles_all, int64_t tile0,
                            const int64_t* total_dev, int64_t base_index, int64_t* hits, int64_t cap,

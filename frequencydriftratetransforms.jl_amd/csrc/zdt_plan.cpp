@@ -169,7 +169,7 @@ int enqueue_channel(Shape& s, int c, const float* spec, float* dest, double thr,
   }
   const int64_t nt = hits_tiles(s.Nf, s.Nr);
   ZDT_TRY(launch_hits_count(dest, s.Nf, s.Nr, ld, thr_h, thr_dev, s.counts, s.st));
-  ZDT_TRY(launch_hits_scan(s.counts, nt, nhits + c, s.st));
+  ZDT_TRY(launch_hits_scan(s.counts, nt, nhits + c, s.counts + nt, s.st));
   if (cap > 0)
     ZDT_TRY(launch_hits_write_impl(dest, s.Nf, s.Nr, ld, thr_h, thr_dev, s.counts, nt, 0, nhits + c, 0,
                                    hits + static_cast<int64_t>(c) * cap, cap, s.st));
@@ -262,7 +262,9 @@ int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* 
     if (rc != ZDT_OK) break;
     s.fused = ws_path(s.ws);
     rc = hip_alloc(reinterpret_cast<void**>(&s.stats), stats_doubles(s) * sizeof(double));
-    if (rc == ZDT_OK) rc = hip_alloc(reinterpret_cast<void**>(&s.counts), (hits_tiles(s.Nf, s.Nr) + 1) * sizeof(int64_t));
+    const int64_t nt = hits_tiles(s.Nf, s.Nr);
+    if (rc == ZDT_OK)
+      rc = hip_alloc(reinterpret_cast<void**>(&s.counts), (nt + hits_scan_scratch(nt)) * sizeof(int64_t));
   }
   if (rc != ZDT_OK) {
     std::string msg = zdt_last_error();
