@@ -157,6 +157,7 @@ struct zdt_ws {
   FastPlan* fast = nullptr;
   // fused output for the generic chirp-Z (power-of-two Nf, 16 < Nt <= 64)
   FastPlan* fout = nullptr;
+  c32* Ft = nullptr;  // its input spectrum, Nk x Nt entries in the transposed layout
   bool ys_fout = false;  // last run left its spectra in fout
 
   bool have_input = false;
@@ -292,7 +293,7 @@ int run_impl(zdt_ws* ws, float r0, int wkind, const float* window, float* dest) 
     return ZDT_OK;
   }
   if (ws->fout != nullptr) {
-    ZDT_TRY(fast_out_run(ws->fout, ws->F, ws->Nk, static_cast<int>(ws->Nt), r0, ws->dr, wkind, window, dest,
+    ZDT_TRY(fast_out_run(ws->fout, ws->Ft, ws->Nk, true, static_cast<int>(ws->Nt), r0, ws->dr, wkind, window, dest,
                          out_scale(ws), ws->stream));
     ws->ys_fast = false;
     ws->ys_fout = (dest == nullptr);
@@ -355,10 +356,7 @@ int64_t zdt_workspace_bytes(int64_t Nf, int64_t Nt, int64_t Nr, const int* facto
   if (Nl < 0 || Nf < 1) return -1;
   const int64_t Nk = Nf / 2 + 1;
   if (!(flags & ZDT_FLAG_GENERIC) && fast_supported(Nf, Nt, Nr)) return fast_bytes(Nf, Nt, Nr);
-  if (!(flags & ZDT_FLAG_GENERIC) && fast_out_supported(Nf, Nt, Nr)) {
-    const int64_t sc = static_cast<int64_t>(fft_real_scratch_elems(Nf, Nt) * sizeof(c32));
-    return Nk * Nt * 8 + fast_out_bytes(Nf, Nr) + 3 * sc;
-  }
+  if (!(flags & ZDT_FLAG_GENERIC) && fast_out_supported(Nf, Nt, Nr)) return Nk * Nt * 8 + fast_out_bytes(Nf, Nr);
   const int64_t b = std::max(Nt, Nr);
   const int64_t sc = static_cast<int64_t>(fft_real_scratch_elems(Nf, b) * sizeof(c32));
   return Nk * Nt * 8 + Nk * Nr * 8 + 3 * sc;
@@ -404,12 +402,13 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
     if (rc == ZDT_OK) ws->bytes += fast_bytes(Nf, Nt, Nr);
   } else if (rc == ZDT_OK) {
     const bool fout = !(flags & ZDT_FLAG_GENERIC) && fast_out_supported(Nf, Nt, Nr);
-    rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->F), static_cast<size_t>(ws->Nk * Nt) * sizeof(c32));
-    if (rc == ZDT_OK && fout) {
-      rc = fast_out_create(Nf, Nr, &ws->fout);
+    if (fout) {
+      // the natural-layout F is unpacked from Ft only for the step functions
+      rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->Ft), static_cast<size_t>(ws->Nk * Nt) * sizeof(c32));
+      if (rc == ZDT_OK) rc = fast_out_create(Nf, Nr, &ws->fout);
       if (rc == ZDT_OK) ws->bytes += fast_out_bytes(Nf, Nr);
-      if (rc == ZDT_OK) rc = ensure_scratch(ws, fft_real_scratch_elems(Nf, Nt) * sizeof(c32));
     } else {
+      rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->F), static_cast<size_t>(ws->Nk * Nt) * sizeof(c32));
       if (rc == ZDT_OK)
         rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->Ys), static_cast<size_t>(ws->Nk * Nr) * sizeof(c32));
       if (rc == ZDT_OK) rc = ensure_scratch(ws, real_scratch_bytes(ws));
@@ -460,6 +459,7 @@ int zdt_destroy(zdt_ws* ws) {
   if (ws->stream) (void)hipStreamSynchronize(ws->stream);
   else (void)hipDeviceSynchronize();
   dev_free(ws->F);
+  dev_free(ws->Ft);
   dev_free(ws->Ys);
   dev_free(ws->Y);
   dev_free(ws->V);
@@ -509,6 +509,9 @@ int zdt_input(zdt_ws* ws, const float* spec) {
   if (use_fast(ws)) {
     ZDT_TRY(fast_input(ws->fast, spec, ws->stream));
     if (ws->F != nullptr) ZDT_TRY(fast_unpack_F(ws->fast, ws->F, ws->stream));
+  } else if (ws->fout != nullptr) {
+    ZDT_TRY(fast_out_input(ws->fout, spec, ws->Nt, ws->Ft, ws->stream));
+    if (ws->F != nullptr) ZDT_TRY(fast_out_unpack_F(ws->fout, ws->Ft, ws->Nt, ws->F, ws->stream));
   } else {
     ZDT_TRY(fft_r2c(ws->Nf, spec, ws->Nf, ws->F, ws->Nk, ws->Nt, ws->s[0], ws->s[1], ws->s[2], ws->stream, ws->maxr));
   }
@@ -544,13 +547,14 @@ int zdt_output(zdt_ws* ws, float* dest) {
   return output_impl(ws, dest);
 }
 
-// F in natural layout for the step functions (the fast path keeps its own
-// layout and unpacks on demand)
+// F in natural layout for the step functions (the fused paths keep their own
+// layouts and unpack on demand; input! refreshes an unpacked F)
 static int ensure_F_natural(zdt_ws* ws) {
-  if (ws->F != nullptr && !use_fast(ws)) return ZDT_OK;
-  if (ws->F == nullptr) {
-    ZDT_TRY(dev_alloc(ws, reinterpret_cast<void**>(&ws->F), static_cast<size_t>(ws->Nk * ws->Nt) * sizeof(c32)));
-    if (ws->have_input) ZDT_TRY(fast_unpack_F(ws->fast, ws->F, ws->stream));
+  if (ws->F != nullptr) return ZDT_OK;
+  ZDT_TRY(dev_alloc(ws, reinterpret_cast<void**>(&ws->F), static_cast<size_t>(ws->Nk * ws->Nt) * sizeof(c32)));
+  if (ws->have_input) {
+    if (use_fast(ws)) ZDT_TRY(fast_unpack_F(ws->fast, ws->F, ws->stream));
+    else if (ws->fout != nullptr) ZDT_TRY(fast_out_unpack_F(ws->fout, ws->Ft, ws->Nt, ws->F, ws->stream));
   }
   return ZDT_OK;
 }

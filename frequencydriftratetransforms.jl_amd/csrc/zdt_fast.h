@@ -21,13 +21,19 @@ int fast_unpack_F(FastPlan* p, c32* F, hipStream_t st);
 int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest, double scale, hipStream_t st);
 int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st);
 
-// Fused output for power-of-two Nf in [2^16, 2^20] and Nt <= 64: the chirp-Z
-// per bin from the natural-layout input spectrum F (Nk x Nt, column stride
-// ldF), then the c2r through k2_c2r and k3 (fast_output).
+// Fused output for power-of-two Nf in [2^16, 2^22] and Nt <= 64: the chirp-Z
+// per bin from the input spectrum F, then the c2r through k2_c2r and k3
+// (fast_output).  F is either natural (Nk x Nt, column stride ldF) or, with
+// ftrans, the layout fast_out_input writes: (Nk-1) x Nt entries [t][k2][k1]
+// (bin k2 + N2 k1) followed by the Nt Nyquist bins.
 bool fast_out_supported(int64_t Nf, int64_t Nt, int64_t Nr);
 int64_t fast_out_bytes(int64_t Nf, int64_t Nr);
 int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out);
-int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, int Nt, float r0, float dr, int wkind, const float* window,
-                 float* dest, double scale, hipStream_t st);
+// input!: spectrogram (device Nf x Nt) -> Ft (Nk x Nt entries, transposed layout)
+int fast_out_input(FastPlan* p, const float* spec, int64_t Nt, c32* Ft, hipStream_t st);
+// natural-layout F from Ft (for the step functions)
+int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_t st);
+int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
+                 const float* window, float* dest, double scale, hipStream_t st);
 
 }  // namespace zdt

@@ -696,8 +696,10 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
 // which moves the spectra through HBM twice instead of once per FFT pass.
 
 struct CztColArgs {
-  const c32* F;  // Nk x Nt, natural layout
+  const c32* F;  // Nk x Nt, natural layout (column stride ldF), or with ftrans
+                 // [Nt][N2][N1] (bin k2 + N2 k1 at [t][k2][k1]) then [Nt] bin M
   int64_t ldF;
+  int ftrans;
   c32* Yc;       // [Nr][N2][N1]
   c32* yM;       // [Nr]
   int64_t M, Nf;
@@ -722,17 +724,22 @@ __global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
   int64_t k;
   c32* dst;
   int64_t stride;
+  int64_t fo, fs;  // F offset of the bin and its step in t
   if (tile * 256 >= a.M) {
     if (threadIdx.x != 0) return;
     k = a.M;
     dst = a.yM;
     stride = 1;
+    fo = a.ftrans ? a.M * a.Nt : a.M;
+    fs = a.ftrans ? 1 : a.ldF;
   } else {
     const int k2 = ((tile / ntk1) << 4) + (threadIdx.x >> 4);
     const int k1 = ((tile % ntk1) << 4) + (threadIdx.x & 15);
     k = k2 + (static_cast<int64_t>(k1) << a.logN2);
     dst = a.Yc + (static_cast<int64_t>(k2) << a.logN1) + k1;
     stride = a.M;
+    fo = a.ftrans ? (static_cast<int64_t>(k2) << a.logN1) + k1 : k;
+    fs = a.ftrans ? a.M : a.ldF;
   }
   const double kd = static_cast<double>(k);
   const float w = fast_window(a.wkind, a.wcustom, k, a.Nf) * a.scale;
@@ -740,7 +747,7 @@ __global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     if (t < a.Nt) {
-      const c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
+      const c32 f = a.F[fo + static_cast<int64_t>(t) * fs];
       const c32 r = cmul(f, cis_nh(kd * static_cast<double>(t), a.h0));
       c[t] = {r.x * w, r.y * w};
     } else {
@@ -998,6 +1005,131 @@ __global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- fused-output input
+
+struct K2iArgs {
+  c32* G;          // in: [Nt][N2][N1] column FFTs of k1_input; out: X_t(k2 + N2 k1) at [t][k2][k1]
+  c32* XM;         // [Nt] X_t(M)
+  const c32* tw1;  // W_N1^m
+  int Nt;
+  int64_t M, Nf;
+};
+
+// input! for the fused output: the second half of the four-step r2c, in
+// place over k1_input's G.  Block per Hermitian column pair as in k2_czt
+// (regular block p: columns p and N2-p; block 0: columns 0 and N2/2); for each
+// group of TG = ROWS/2 time steps the 2 TG rows get the twiddle W_M^{n1 k2}
+// and a forward DFT_N1, then each thread untangles its pairs (k, M-k) into
+// the r2c spectrum and writes both bins back over the rows it read.
+template <int N1, int N2, int ROWS>
+__global__ __launch_bounds__(N1 < 1024 ? N1 : 1024) void k2_input(K2iArgs a) {
+  constexpr int T = N1 < 1024 ? N1 : 1024;
+  constexpr int PP = N1 / T;
+  constexpr int P = N1 + N1 / 8;
+  constexpr int HALF = N1 / 2;
+  constexpr int TG = ROWS / 2;
+  __shared__ c32 buf[ROWS * P];
+  __shared__ c32 buf2[ROWS * P];
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x;
+  const bool special = (p == 0);
+  const int colA = special ? 0 : p;
+  const int colB = special ? N2 / 2 : N2 - p;
+
+  int rowA[PP], rowB[PP], k1A[PP], k1B[PP];
+  bool nyq[PP];
+  c32 WA[PP], twS0[PP], twS1[PP];
+  const float invM = 1.0f / static_cast<float>(a.M);
+#pragma unroll
+  for (int i = 0; i < PP; ++i) {
+    const int q = tid + i * T;
+    int cA;
+    nyq[i] = false;
+    if (!special) {
+      rowA[i] = 0;
+      rowB[i] = 1;
+      k1A[i] = q;
+      k1B[i] = N1 - 1 - q;
+      cA = colA;
+    } else if (q < HALF) {
+      rowA[i] = rowB[i] = 0;
+      k1A[i] = q;
+      k1B[i] = N1 - q;  // q == 0: partner is the Nyquist bin M
+      nyq[i] = (q == 0);
+      cA = 0;
+    } else {
+      rowA[i] = rowB[i] = 1;
+      k1A[i] = q - HALF;
+      k1B[i] = N1 - 1 - k1A[i];
+      cA = N2 / 2;
+    }
+    const int64_t kA = cA + static_cast<int64_t>(N2) * k1A[i];
+    WA[i] = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));  // W_Nf^{kA}
+    const int64_t n1 = q;
+    twS0[i] = cis_rev(static_cast<float>((n1 * colA) & (a.M - 1)) * invM);
+    twS1[i] = cis_rev(static_cast<float>((n1 * colB) & (a.M - 1)) * invM);
+  }
+
+  for (int t0 = 0; t0 < a.Nt; t0 += TG) {
+    const int nt = min(TG, a.Nt - t0);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      if (r < 2 * nt) {
+        const c32* src = a.G + (static_cast<int64_t>(t0 + (r >> 1)) * N2 + ((r & 1) ? colB : colA)) * N1;
+#pragma unroll
+        for (int i = 0; i < PP; ++i)
+          buf[r * P + lsw(tid + i * T)] = cmulc(src[tid + i * T], (r & 1) ? twS1[i] : twS0[i]);
+      }
+    }
+    __syncthreads();
+    const c32* fz = lds_fft_pp<N1, T, false>(buf, buf2, P, 2 * nt, a.tw1);
+#pragma unroll
+    for (int uu = 0; uu < TG; ++uu) {
+      if (uu < nt) {
+        const int64_t t = t0 + uu;
+        c32* rA = a.G + (t * N2 + colA) * N1;
+        c32* rB = a.G + (t * N2 + colB) * N1;
+#pragma unroll
+        for (int i = 0; i < PP; ++i) {
+          const c32 Za = fz[(2 * uu + rowA[i]) * P + lsw(k1A[i])];
+          c32* dA = rowA[i] ? rB : rA;
+          c32* dB = rowB[i] ? rB : rA;
+          if (nyq[i]) {
+            dA[0] = {Za.x + Za.y, 0.0f};
+            a.XM[t] = {Za.x - Za.y, 0.0f};
+          } else {
+            const c32 Zb = fz[(2 * uu + rowB[i]) * P + lsw(k1B[i])];
+            const c32 E = {0.5f * (Za.x + Zb.x), 0.5f * (Za.y - Zb.y)};
+            const c32 O = {0.5f * (Za.y + Zb.y), -0.5f * (Za.x - Zb.x)};
+            const c32 WO = cmul(WA[i], O);
+            dA[k1A[i]] = cadd(E, WO);
+            dB[k1B[i]] = {E.x - WO.x, -(E.y - WO.y)};
+          }
+        }
+        if (special && tid == 0) {
+          // bin M/2 (column 0, k1 = N1/2) is its own partner: X = conj(Z)
+          const c32 Zh = fz[(2 * uu) * P + lsw(HALF)];
+          rA[HALF] = cconj(Zh);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// F (natural Nk x Nt) from the fused-output input layout
+__global__ __launch_bounds__(256) void ft_to_natural(const c32* __restrict__ Ft, c32* __restrict__ F, int64_t Nk,
+                                                     int Nt, int logN1, int logN2) {
+  const int64_t M = Nk - 1;
+  const int64_t total = Nk * Nt;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t t = g / Nk, k = g - t * Nk;
+    const int64_t k2 = k & ((int64_t(1) << logN2) - 1), k1 = k >> logN2;
+    F[g] = (k == M) ? Ft[M * Nt + t] : Ft[t * M + (k2 << logN1) + k1];
+  }
+}
+
 // ---------------------------------------------------------------- F unpack
 
 // Gt[t][k2][n1] = G[t][k2][n1] W_M^{n1 k2} (the four-step twiddle k2 applies
@@ -1172,16 +1304,16 @@ int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, FastPlan**
   return ZDT_OK;
 }
 
-int fast_input(FastPlan* p, const float* spec, hipStream_t st) {
+static int launch_k1(const FastPlan* p, const float* spec, c32* G, int64_t Nt, hipStream_t st) {
   K1Args a;
   a.spec = spec;
   a.ldspec = p->Nf;
-  a.G = p->G;
+  a.G = G;
   a.tw2 = p->tw2;
   a.logN1 = p->logN1;
   a.M = p->M;
-  a.ntiles = static_cast<int>(p->Nt * (p->N1 / 16));
-  ProfScope ps(st, "k1_input", 4.0 * p->Nf * p->Nt + 8.0 * p->M * p->Nt);
+  a.ntiles = static_cast<int>(Nt * (p->N1 / 16));
+  ProfScope ps(st, "k1_input", 4.0 * p->Nf * Nt + 8.0 * p->M * Nt);
   if (p->N2 == 1024) {
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
     k1_input<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
@@ -1192,6 +1324,8 @@ int fast_input(FastPlan* p, const float* spec, hipStream_t st) {
   ZDT_CHECK_LAUNCH("k1_input");
   return ZDT_OK;
 }
+
+int fast_input(FastPlan* p, const float* spec, hipStream_t st) { return launch_k1(p, spec, p->G, p->Nt, st); }
 
 template <int N1, int N2>
 static int launch_k2(const K2Args& a, int NT, hipStream_t st) {
@@ -1360,12 +1494,69 @@ static int launch_k2c(const K2cArgs& a, hipStream_t st) {
   return ZDT_OK;
 }
 
-int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, int Nt, float r0, float dr, int wkind, const float* window,
-                 float* dest, double scale, hipStream_t st) {
+template <int N1, int N2>
+static void launch_k2i(const K2iArgs& a, hipStream_t st) {
+  // two ping-pong buffers of ~36 KB each: 8 rows up to N1 = 512, then 4, 2
+  constexpr int ROWS = N1 <= 512 ? 8 : (N1 == 1024 ? 4 : 2);
+  k2_input<N1, N2, ROWS><<<N2 / 2, (N1 < 1024 ? N1 : 1024), 0, st>>>(a);
+}
+
+int fast_out_input(FastPlan* p, const float* spec, int64_t Nt, c32* Ft, hipStream_t st) {
+  if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%lld", (long long)Nt);
+  ZDT_TRY(launch_k1(p, spec, Ft, Nt, st));
+  K2iArgs a;
+  a.G = Ft;
+  a.XM = Ft + p->M * Nt;
+  a.tw1 = p->tw1;
+  a.Nt = static_cast<int>(Nt);
+  a.M = p->M;
+  a.Nf = p->Nf;
+  ProfScope ps(st, "k2_input", 16.0 * p->M * Nt);
+  switch (p->N1 * 2048 + p->N2) {
+    case 64 * 2048 + 512:
+      launch_k2i<64, 512>(a, st);
+      break;
+    case 64 * 2048 + 1024:
+      launch_k2i<64, 1024>(a, st);
+      break;
+    case 128 * 2048 + 1024:
+      launch_k2i<128, 1024>(a, st);
+      break;
+    case 256 * 2048 + 1024:
+      launch_k2i<256, 1024>(a, st);
+      break;
+    case 512 * 2048 + 1024:
+      launch_k2i<512, 1024>(a, st);
+      break;
+    case 1024 * 2048 + 1024:
+      launch_k2i<1024, 1024>(a, st);
+      break;
+    case 2048 * 2048 + 1024:
+      launch_k2i<2048, 1024>(a, st);
+      break;
+    default:
+      return fail(ZDT_ENOTSUP, "fused output input: N1=%d N2=%d", p->N1, p->N2);
+  }
+  ZDT_CHECK_LAUNCH("k2_input");
+  return ZDT_OK;
+}
+
+int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_t st) {
+  const int64_t n = (p->M + 1) * Nt;
+  int64_t g = (n + 255) / 256;
+  if (g > (1 << 20)) g = 1 << 20;
+  ft_to_natural<<<static_cast<unsigned>(g), 256, 0, st>>>(Ft, F, p->M + 1, static_cast<int>(Nt), p->logN1, p->logN2);
+  ZDT_CHECK_LAUNCH("ft_to_natural");
+  return ZDT_OK;
+}
+
+int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
+                 const float* window, float* dest, double scale, hipStream_t st) {
   if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
   CztColArgs c;
   c.F = F;
   c.ldF = ldF;
+  c.ftrans = ftrans ? 1 : 0;
   c.Yc = p->Yc;
   c.yM = p->yM;
   c.M = p->M;

@@ -130,7 +130,8 @@ def test_fast_unnormalized_and_hits(F):
 
 # ---- per-bin chirp-Z with the fused output (power-of-two Nf, 16 < Nt <= 64)
 
-@pytest.mark.parametrize("Nf,Nt", [(1 << 16, 17), (1 << 17, 32), (1 << 19, 60), (1 << 16, 64)])
+@pytest.mark.parametrize("Nf,Nt", [(1 << 16, 17), (1 << 17, 32), (1 << 18, 23), (1 << 19, 60), (1 << 20, 33),
+                                   (1 << 16, 64)])
 def test_fused_output_vs_fp64(F, Nf, Nt):
     Nr = 33
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.37, 1.25, 25.0), (Nf * 0.81, -2.0, 30.0)], seed=Nf % 991 + Nt)
@@ -163,3 +164,26 @@ def test_fused_output_large_nf(F, Nf, Nt):
     say(f"fused output Nf={Nf} Nt={Nt}")
     out = host(F.zdtfdr(F.ZDTWorkspace(s, -1.5, 1.5, 3)))
     assert normwise(out, Z.fdr64(s, -1.5, 1.5, 3)) <= 2e-6
+
+
+def test_fused_output_F_reinput_and_step_functions(F):
+    Nf, Nt = 1 << 18, 24
+    s1 = synth.spectrogram(Nf, Nt, [(90000.0, 1.25, 30.0)], seed=23)
+    s2 = synth.spectrogram(Nf, Nt, [(150000.0, -2.5, 25.0)], seed=24)
+    say("fused output: F buffer, input! again, step functions")
+    ws = F.ZDTWorkspace(s1, -2.0, 0.125, 33)
+    ref1 = np.fft.rfft(s1.astype(np.float64), axis=0)
+    assert normwise(host(ws.F), ref1) <= 1e-6
+    # a new input! refreshes the unpacked F and the transform
+    F.input_(ws, s2)
+    assert normwise(host(ws.F), np.fft.rfft(s2.astype(np.float64), axis=0)) <= 1e-6
+    assert normwise(host(F.zdtfdr(ws)), Z.fdr64(s2, -2.0, 0.125, 33)) <= 2e-6
+    # the step functions read the natural-layout F
+    dest = F.create_fdr(s2, 33)
+    F.preprocess_(ws, -2.0)
+    F.convolve_(ws)
+    F.postprocess_(ws)
+    F.output_(dest, ws)
+    assert normwise(host(dest), Z.fdr64(s2, -2.0, 0.125, 33)) <= 2e-6
+    # and the fused output again after them
+    assert normwise(host(F.zdtfdr(ws, s1)), Z.fdr64(s1, -2.0, 0.125, 33)) <= 2e-6
