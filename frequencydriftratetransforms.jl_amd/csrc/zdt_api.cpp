@@ -432,6 +432,35 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
 
 bool ws_is_fused(const zdt_ws* ws) { return ws != nullptr && ws->fast != nullptr; }
 
+int64_t devstats_scratch_doubles(int64_t Nf, int64_t Nr) { return 2 * static_cast<int64_t>(colstats_nchunk(Nf)) * Nr; }
+
+int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* colvar, double* part) {
+  ZDT_TRY(check_ws(ws));
+  ProfInstall prof_(ws);
+  if (dest == nullptr || colmean == nullptr || colvar == nullptr)
+    return fail(ZDT_EINVAL, "zdtfdr! with statistics: dest and statistics buffers are required");
+  ZDT_TRY(set_device(ws));
+  if (!ws->have_input) return fail(ZDT_ESTATE, "zdtfdr!: no input (call input! or pass a spectrogram)");
+  if (use_fast(ws)) {
+    ZDT_TRY(fast_run(ws->fast, r0, ZDT_WINDOW_RECT, nullptr, dest, out_scale(ws), ws->stream, colmean, colvar));
+    ws->ys_fast = false;
+    ws->ys_fout = false;
+    ws->ys_src = nullptr;
+    return ZDT_OK;
+  }
+  if (ws->fout != nullptr) {
+    ZDT_TRY(fast_out_run(ws->fout, ws->Ft, ws->Nk, true, static_cast<int>(ws->Nt), r0, ws->dr, ZDT_WINDOW_RECT,
+                         nullptr, dest, out_scale(ws), ws->stream, colmean, colvar));
+    ws->ys_fast = false;
+    ws->ys_fout = false;
+    ws->ys_src = nullptr;
+    return ZDT_OK;
+  }
+  if (part == nullptr) return fail(ZDT_EINVAL, "zdtfdr! with statistics: the generic path needs scratch");
+  ZDT_TRY(run_impl(ws, r0, ZDT_WINDOW_RECT, nullptr, dest));
+  return launch_colstats2(dest, ws->Nf, ws->Nr, ws->Nf, part, colmean, colvar, ws->stream);
+}
+
 int ws_path(const zdt_ws* ws) {
   if (ws == nullptr) return 0;
   return ws->fast ? 1 : (ws->fout ? 2 : 0);

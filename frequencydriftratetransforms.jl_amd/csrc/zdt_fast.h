@@ -17,9 +17,15 @@ void fast_destroy(FastPlan* p);
 int fast_input(FastPlan* p, const float* spec, hipStream_t st);
 // the input spectrum F in natural Nk x Nt layout (for the step functions)
 int fast_unpack_F(FastPlan* p, c32* F, hipStream_t st);
-// one batch; dest NULL keeps the result for fast_output
-int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest, double scale, hipStream_t st);
-int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st);
+// one batch; dest NULL keeps the result for fast_output.  colmean/colvar
+// (device, Nr doubles each), when given with a dest, receive the FDR's column
+// statistics, computed by k3 as it writes the FDR.
+int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest, double scale, hipStream_t st,
+             double* colmean = nullptr, double* colvar = nullptr);
+int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* colmean = nullptr,
+                double* colvar = nullptr);
+// doubles of k3's statistics partials (held by every fused plan)
+int64_t fast_stats_doubles(int64_t Nf, int64_t Nr);
 
 // Fused output for power-of-two Nf in [2^16, 2^22] and Nt <= 64: the chirp-Z
 // per bin from the input spectrum F, then the c2r through k2_c2r and k3
@@ -34,6 +40,7 @@ int fast_out_input(FastPlan* p, const float* spec, int64_t Nt, c32* Ft, hipStrea
 // natural-layout F from Ft (for the step functions)
 int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_t st);
 int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
-                 const float* window, float* dest, double scale, hipStream_t st);
+                 const float* window, float* dest, double scale, hipStream_t st, double* colmean = nullptr,
+                 double* colvar = nullptr);
 
 }  // namespace zdt

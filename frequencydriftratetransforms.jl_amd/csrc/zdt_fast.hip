@@ -322,7 +322,29 @@ struct K3Args {
   int logN1;
   int Nr;
   int ntiles;    // Nr * N1/16
+  double* part;  // NULL, or per-wavefront FDR statistics [Nr][N1/16][waves][3] (see k3_stats)
 };
+
+// Each wavefront of a k3 tile writes (S1, S2, c): the FP64 sums of (x - c) and
+// (x - c)^2 over its 64 x 64 FDR values, c = lane 0's first value.  Each lane
+// sums with its own first value as shift and is re-centred to c before the
+// (fixed-order) wavefront reduction, so no sum cancels.
+__device__ __forceinline__ void k3_stats(double* dst, double s1, double s2, double ct) {
+  constexpr double n = 64.0;  // values per lane
+  const double c = __shfl(ct, 0, 64);
+  const double d = ct - c;
+  double S1 = s1 + n * d;
+  double S2 = s2 + d * (2.0 * s1 + n * d);
+  for (int off = 32; off > 0; off >>= 1) {
+    S1 += __shfl_xor(S1, off, 64);
+    S2 += __shfl_xor(S2, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    dst[0] = S1;
+    dst[1] = S2;
+    dst[2] = c;
+  }
+}
 
 template <int R1>
 __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_output(K3Args a) {
@@ -372,11 +394,21 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
     if (next < a.ntiles) load(next, x, h);
     c32* out = reinterpret_cast<c32*>(a.dest + static_cast<int64_t>(j) * a.ld);
     const unsigned o = (nb << 4) + c;
+    double s1 = 0.0, s2 = 0.0, ct = 0.0;
 #pragma unroll
     for (int hh = 0; hh < C::MPER; ++hh) {
       const unsigned mp = r + R1 * hh;
       c32 y[R1];
       cf_stage2<R1>(y, ex, c, mp, true);
+      if (a.part != nullptr) {  // uniform
+        if (hh == 0) ct = static_cast<double>(y[0].x);
+#pragma unroll
+        for (int rp = 0; rp < R1; ++rp) {
+          const double d0 = static_cast<double>(y[rp].x) - ct, d1 = static_cast<double>(y[rp].y) - ct;
+          s1 += d0 + d1;
+          s2 += d0 * d0 + d1 * d1;
+        }
+      }
       {
         // the FDR is not read again: nontemporal stores keep it from evicting
         // the next step's inputs (spectrogram, G) from the last-level cache
@@ -389,6 +421,10 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
           __builtin_nontemporal_store(v, q + rp * step);
         }
       }
+    }
+    if (a.part != nullptr) {
+      constexpr int NW = C::THREADS / 64;
+      k3_stats(a.part + ((static_cast<int64_t>(j) * nblk + nb) * NW + (threadIdx.x >> 6)) * 3, s1, s2, ct);
     }
     __syncthreads();
     tile = next;
@@ -1194,6 +1230,7 @@ struct FastPlan {
   c32* yhalf = nullptr;  // [Nr] bin M/2 term of the c2r (k2 -> k3)
   c32* Yc = nullptr;     // fused output only: [Nr][N2][N1] spectra
   c32* yM = nullptr;     // fused output only: [Nr] Nyquist bin
+  double* part = nullptr;  // k3's per-wavefront FDR statistics (fast_stats_doubles)
   bool have_B = false;
   int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
 };
@@ -1215,7 +1252,7 @@ int64_t fast_bytes(int64_t Nf, int64_t Nt, int64_t Nr) {
   const int64_t M = Nf / 2;
   const int64_t N2 = fast_n2(Nf);
   const int64_t N1 = M / N2;
-  return 8 * M * (Nt + Nr) + 8 * (N1 + N2 + 2 * Nr);
+  return 8 * M * (Nt + Nr) + 8 * (N1 + N2 + 2 * Nr) + 8 * fast_stats_doubles(Nf, Nr);
 }
 
 static int upload_table(c32** dst, const std::vector<c32>& h) {
@@ -1234,11 +1271,22 @@ static std::vector<c32> twiddles(int N) {
   return t;
 }
 
+static void alloc_part(FastPlan* p, int& rc) {
+  if (rc != ZDT_OK) return;
+  const size_t n = static_cast<size_t>(fast_stats_doubles(p->Nf, p->Nr));
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->part), n * sizeof(double));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    p->part = nullptr;
+    rc = fail(ZDT_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(double), hipGetErrorString(e));
+  }
+}
+
 void fast_destroy(FastPlan* p) {
   if (p == nullptr) return;
   for (void* q : {static_cast<void*>(p->G), static_cast<void*>(p->B), static_cast<void*>(p->tw1),
                   static_cast<void*>(p->tw2), static_cast<void*>(p->tabq), static_cast<void*>(p->yhalf),
-                  static_cast<void*>(p->Yc), static_cast<void*>(p->yM)})
+                  static_cast<void*>(p->Yc), static_cast<void*>(p->yM), static_cast<void*>(p->part)})
     if (q) (void)hipFree(q);
   delete p;
 }
@@ -1282,6 +1330,7 @@ int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, FastPlan**
   };
   alloc(&p->G, static_cast<size_t>(Nt * p->M));
   alloc(&p->B, static_cast<size_t>(Nr * p->M));
+  alloc_part(p, rc);
   alloc(&p->yhalf, static_cast<size_t>(Nr));
   if (rc == ZDT_OK) rc = upload_table(&p->tw1, twiddles(p->N1));
   if (rc == ZDT_OK) rc = upload_table(&p->tw2, twiddles(p->N2));
@@ -1350,7 +1399,8 @@ static int launch_k2(const K2Args& a, int NT, hipStream_t st) {
   return ZDT_OK;
 }
 
-int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest, double scale, hipStream_t st) {
+int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest, double scale, hipStream_t st,
+             double* colmean, double* colvar) {
   K2Args a;
   a.G = p->G;
   a.B = p->B;
@@ -1401,13 +1451,55 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
     }
   }
   p->have_B = true;
-  if (dest != nullptr) ZDT_TRY(fast_output(p, dest, scale, st));
+  if (dest != nullptr) ZDT_TRY(fast_output(p, dest, scale, st, colmean, colvar));
   return ZDT_OK;
 }
 
-int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st) {
+// Column mean and corrected variance from k3's per-wavefront partials, one
+// block per column, merged in a fixed order around the column's first
+// partial shift (the same statistics as colstats, fdrstats of
+// src/fdrutils.jl:39-50).
+__global__ __launch_bounds__(256) void k3_stats_merge(const double* __restrict__ part, int np, int64_t Nf,
+                                                      double* __restrict__ colmean, double* __restrict__ colvar) {
+  const int64_t j = blockIdx.x;
+  const double* pj = part + j * np * 3;
+  const double C0 = pj[2];
+  const double n = static_cast<double>(Nf) / np;  // values per partial
+  double S1 = 0.0, S2 = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    const double s1 = pj[3 * i], s2 = pj[3 * i + 1], d = pj[3 * i + 2] - C0;
+    S1 += s1 + n * d;
+    S2 += s2 + d * (2.0 * s1 + n * d);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    S1 += __shfl_xor(S1, off, 64);
+    S2 += __shfl_xor(S2, off, 64);
+  }
+  __shared__ double sh[2][4];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = S1;
+    sh[1][threadIdx.x >> 6] = S2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double T1 = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    const double T2 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+    const double N = static_cast<double>(Nf);
+    colmean[j] = C0 + T1 / N;
+    colvar[j] = Nf > 1 ? (T2 - T1 * T1 / N) / (N - 1.0) : 0.0 / 0.0;
+  }
+}
+
+static int k3_waves(const FastPlan* p) { return (p->N2 == 1024 ? CF<32>::THREADS : CF<16>::THREADS) / 64; }
+
+// (N1/16) tiles x k3 wavefronts per rate is M/2048 for both N2 = 1024
+// (8 wavefronts) and N2 = 512 (4 wavefronts)
+int64_t fast_stats_doubles(int64_t Nf, int64_t Nr) { return Nr * (Nf / 2 / 2048) * 3; }
+
+int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* colmean, double* colvar) {
   (void)scale;  // folded into k2
   if (!p->have_B) return fail(ZDT_ESTATE, "output!: no post-processed spectra");
+  if (colmean != nullptr && p->part == nullptr) return fail(ZDT_ESTATE, "fused statistics: no partials buffer");
   K3Args a;
   a.B = p->B;
   a.yhalf = p->yhalf;
@@ -1417,15 +1509,23 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st) {
   a.logN1 = p->logN1;
   a.Nr = static_cast<int>(p->Nr);
   a.ntiles = static_cast<int>(p->Nr * (p->N1 / 16));
-  ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
-  if (p->N2 == 1024) {
-    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
-    k3_output<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
-  } else {
-    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
-    k3_output<16><<<grid, CF<16>::THREADS, 0, st>>>(a);
+  a.part = colmean != nullptr ? p->part : nullptr;
+  {
+    ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
+    if (p->N2 == 1024) {
+      const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
+      k3_output<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
+    } else {
+      const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
+      k3_output<16><<<grid, CF<16>::THREADS, 0, st>>>(a);
+    }
+    ZDT_CHECK_LAUNCH("k3_output");
   }
-  ZDT_CHECK_LAUNCH("k3_output");
+  if (colmean != nullptr) {
+    const int np = (p->N1 / 16) * k3_waves(p);
+    k3_stats_merge<<<static_cast<unsigned>(p->Nr), 256, 0, st>>>(p->part, np, p->Nf, colmean, colvar);
+    ZDT_CHECK_LAUNCH("k3_stats_merge");
+  }
   return ZDT_OK;
 }
 
@@ -1439,7 +1539,7 @@ bool fast_out_supported(int64_t Nf, int64_t Nt, int64_t Nr) {
 int64_t fast_out_bytes(int64_t Nf, int64_t Nr) {
   const int64_t M = Nf / 2;
   const int64_t N2 = fast_n2(Nf);
-  return 16 * M * Nr + 8 * (M / N2 + N2 + 2 * Nr);
+  return 16 * M * Nr + 8 * (M / N2 + N2 + 2 * Nr) + 8 * fast_stats_doubles(Nf, Nr);
 }
 
 int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out) {
@@ -1470,6 +1570,7 @@ int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out) {
   };
   alloc(&p->Yc, static_cast<size_t>(Nr * p->M));
   alloc(&p->B, static_cast<size_t>(Nr * p->M));
+  alloc_part(p, rc);
   alloc(&p->yM, static_cast<size_t>(Nr));
   alloc(&p->yhalf, static_cast<size_t>(Nr));
   if (rc == ZDT_OK) rc = upload_table(&p->tw1, twiddles(p->N1));
@@ -1551,7 +1652,7 @@ int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_
 }
 
 int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
-                 const float* window, float* dest, double scale, hipStream_t st) {
+                 const float* window, float* dest, double scale, hipStream_t st, double* colmean, double* colvar) {
   if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
   CztColArgs c;
   c.F = F;
@@ -1621,7 +1722,7 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, fl
     }
   }
   p->have_B = true;
-  if (dest != nullptr) ZDT_TRY(fast_output(p, dest, scale, st));
+  if (dest != nullptr) ZDT_TRY(fast_output(p, dest, scale, st, colmean, colvar));
   return ZDT_OK;
 }
 

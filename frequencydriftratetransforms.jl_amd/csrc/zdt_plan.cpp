@@ -69,7 +69,7 @@ int hip_alloc(void** p, size_t bytes) {
 }
 
 size_t stats_doubles(const Shape& s) {
-  return 2 * static_cast<size_t>(colstats_nchunk(s.Nf)) * s.Nr + 2 * static_cast<size_t>(s.Nr) + 1;
+  return static_cast<size_t>(devstats_scratch_doubles(s.Nf, s.Nr)) + 2 * static_cast<size_t>(s.Nr) + 1;
 }
 
 // Times input! + one zdtfdr! batch of a candidate workspace (ms, best of 3).
@@ -152,20 +152,22 @@ void destroy_graph(zdt_plan* p) {
 int enqueue_channel(Shape& s, int c, const float* spec, float* dest, double thr, int snr, int64_t* hits, int64_t cap,
                     int64_t* nhits, double* thr_out) {
   ZDT_TRY(zdt_input(s.ws, spec));
-  ZDT_TRY(zdt_run(s.ws, s.r0, ZDT_WINDOW_RECT, nullptr, dest));
   const int64_t ld = s.Nf;
   const double* thr_dev = nullptr;
   double thr_h = thr;
   if (snr) {
-    const size_t part_n = 2 * static_cast<size_t>(colstats_nchunk(s.Nf)) * s.Nr;
+    // the transform and the FDR statistics (fused into the output kernel
+    // where the workspace runs the fused kernels)
+    const size_t part_n = static_cast<size_t>(devstats_scratch_doubles(s.Nf, s.Nr));
     double* colmean = s.stats + part_n;
     double* colvar = colmean + s.Nr;
     double* t = thr_out ? thr_out + c : colvar + s.Nr;
-    ZDT_TRY(launch_colstats2(dest, s.Nf, s.Nr, ld, s.stats, colmean, colvar, s.st));
+    ZDT_TRY(run_devstats(s.ws, s.r0, dest, colmean, colvar, s.stats));
     ZDT_TRY(launch_hits_threshold(colmean, colvar, s.Nr, thr, t, s.st));
     thr_dev = t;
-  } else if (thr_out) {
-    ZDT_TRY(launch_set_double(thr_out + c, thr_h, s.st));
+  } else {
+    ZDT_TRY(zdt_run(s.ws, s.r0, ZDT_WINDOW_RECT, nullptr, dest));
+    if (thr_out) ZDT_TRY(launch_set_double(thr_out + c, thr_h, s.st));
   }
   const int64_t nt = hits_tiles(s.Nf, s.Nr);
   ZDT_TRY(launch_hits_count(dest, s.Nf, s.Nr, ld, thr_h, thr_dev, s.counts, s.st));

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "zdt_internal.h"
+#include "zdt_ws.h"
 
 struct zdt_stream {
   int device = 0;
@@ -36,6 +37,7 @@ struct zdt_stream {
   std::vector<float*> fdr;  // resident batch FDRs (or one reused buffer)
   int64_t* dhits = nullptr;  // device hit staging
   int64_t dcap = 0;
+  double* dstats = nullptr;  // colmean, colvar (Nrb each), then generic-path scratch
 };
 
 namespace zdt {
@@ -59,7 +61,7 @@ int64_t stream_bytes(int64_t Nf, int64_t Nt, int64_t Nr, int64_t Nrb, bool resid
   if (ws < 0) return -1;
   const int64_t nb = (Nr + Nrb - 1) / Nrb;
   const int64_t fdrs = (resident ? nb : 1) * Nf * Nrb * 4;
-  return ws + fdrs + 2 * Nf * Nt * 4 + dcap * 8;
+  return ws + fdrs + 2 * Nf * Nt * 4 + dcap * 8 + (2 * Nrb + devstats_scratch_doubles(Nf, Nrb)) * 8;
 }
 
 }  // namespace
@@ -95,6 +97,7 @@ int zdt_stream_destroy(zdt_stream* s) {
     if (s->used[i]) (void)hipEventDestroy(s->used[i]);
   }
   if (s->dhits) (void)hipFree(s->dhits);
+  if (s->dstats) (void)hipFree(s->dstats);
   if (s->copy) (void)hipStreamDestroy(s->copy);
   if (s->compute) (void)hipStreamDestroy(s->compute);
   delete s;
@@ -176,6 +179,9 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
     if (rc == ZDT_OK) s->fdr.push_back(f);
   }
   if (rc == ZDT_OK) rc = salloc(s, reinterpret_cast<void**>(&s->dhits), static_cast<size_t>(dcap) * 8);
+  if (rc == ZDT_OK)
+    rc = salloc(s, reinterpret_cast<void**>(&s->dstats),
+                static_cast<size_t>(2 * s->Nrb + devstats_scratch_doubles(Nf, s->Nrb)) * sizeof(double));
   if (rc != ZDT_OK) {
     std::string msg = zdt_last_error();
     zdt_stream_destroy(s);
@@ -228,15 +234,19 @@ int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, 
     // statistics over all batches: mean of the first column, least deviation
     double m = 0.0, sd = INFINITY;
     bool nan = false;
+    std::vector<double> hst(static_cast<size_t>(2 * Nrb));
     for (int64_t b = 0; b < s->nb; ++b) {
       float* f = s->fdr[s->resident ? b : 0];
-      ZDT_TRY(zdt_run(s->ws, r0b(b), ZDT_WINDOW_RECT, nullptr, f));
-      double mb = 0, sb = 0;
-      const float* fl[1] = {f};
-      ZDT_TRY(zdt_fdrstats(fl, 1, Nf, Nrb, Nf, s->compute, &mb, &sb));
-      if (b == 0) m = mb;
-      if (std::isnan(sb)) nan = true;
-      sd = std::min(sd, sb);
+      ZDT_TRY(run_devstats(s->ws, r0b(b), f, s->dstats, s->dstats + Nrb, s->dstats + 2 * Nrb));
+      ZDT_TRY_HIP(hipMemcpyAsync(hst.data(), s->dstats, hst.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 s->compute));
+      ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
+      if (b == 0) m = hst[0];
+      for (int64_t j = 0; j < Nrb; ++j) {
+        const double v = hst[static_cast<size_t>(Nrb + j)];
+        if (std::isnan(v)) nan = true;
+        sd = std::min(sd, std::sqrt(std::max(v, 0.0)));
+      }
     }
     const double t = snr * (nan ? NAN : sd) + m;  // fdrdenormalize (src/fdrutils.jl:104-107)
     if (thr) thr[i] = t;

@@ -16,4 +16,12 @@ bool ws_is_fused(const zdt_ws* ws);
 // 1: fused kernels; 2: per-bin chirp-Z with the fused output; 0: generic
 int ws_path(const zdt_ws* ws);
 
+// zdtfdr! (rectangular window) into dest plus the FDR's column mean and
+// corrected variance on the device (colmean, colvar: Nr doubles each), all
+// enqueued on the workspace stream.  The fused paths compute the statistics
+// inside the output kernel; the generic path runs colstats over dest with
+// part as scratch (devstats_scratch_doubles).
+int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* colvar, double* part);
+int64_t devstats_scratch_doubles(int64_t Nf, int64_t Nr);
+
 }  // namespace zdt
