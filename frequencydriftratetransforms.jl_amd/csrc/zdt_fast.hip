@@ -746,37 +746,50 @@ struct CztColArgs {
   int wkind;
   const float* wcustom;
   float scale;
+  int pair;      // czt_reinsch_col: rates pair as r_j = -r_{pair-j}
 };
 
-// one thread per bin; a block covers 16 columns k2 x 16 rows k1 (so the Nr
-// stores of a wavefront are four 128-byte rows of Yc); the last block's
-// thread 0 takes the Nyquist bin M
+// The bin of a thread: one thread per bin; a block covers 16 columns k2 x 16
+// rows k1 (so the Nr stores of a wavefront are four 128-byte rows of Yc); the
+// last block's thread 0 takes the Nyquist bin M.  Returns false for the idle
+// threads of the last block.
+struct ColBin {
+  int64_t k;
+  c32* dst;
+  int64_t stride;  // between rates in dst
+  int64_t fo, fs;  // F offset of the bin and its step in t
+};
+
+__device__ __forceinline__ bool col_bin(const CztColArgs& a, ColBin& b) {
+  const int ntk1 = (1 << a.logN1) >> 4;
+  const int tile = blockIdx.x;
+  if (tile * 256 >= a.M) {
+    if (threadIdx.x != 0) return false;
+    b.k = a.M;
+    b.dst = a.yM;
+    b.stride = 1;
+    b.fo = a.ftrans ? a.M * a.Nt : a.M;
+    b.fs = a.ftrans ? 1 : a.ldF;
+  } else {
+    const int k2 = ((tile / ntk1) << 4) + (threadIdx.x >> 4);
+    const int k1 = ((tile % ntk1) << 4) + (threadIdx.x & 15);
+    b.k = k2 + (static_cast<int64_t>(k1) << a.logN2);
+    b.dst = a.Yc + (static_cast<int64_t>(k2) << a.logN1) + k1;
+    b.stride = a.M;
+    b.fo = a.ftrans ? (static_cast<int64_t>(k2) << a.logN1) + k1 : b.k;
+    b.fs = a.ftrans ? a.M : a.ldF;
+  }
+  return true;
+}
+
 template <int NT>
 __global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
   constexpr int B = NT < 16 ? NT : 16;
   constexpr int NB = NT / B;
-  const int ntk1 = (1 << a.logN1) >> 4;
-  const int tile = blockIdx.x;
-  int64_t k;
-  c32* dst;
-  int64_t stride;
-  int64_t fo, fs;  // F offset of the bin and its step in t
-  if (tile * 256 >= a.M) {
-    if (threadIdx.x != 0) return;
-    k = a.M;
-    dst = a.yM;
-    stride = 1;
-    fo = a.ftrans ? a.M * a.Nt : a.M;
-    fs = a.ftrans ? 1 : a.ldF;
-  } else {
-    const int k2 = ((tile / ntk1) << 4) + (threadIdx.x >> 4);
-    const int k1 = ((tile % ntk1) << 4) + (threadIdx.x & 15);
-    k = k2 + (static_cast<int64_t>(k1) << a.logN2);
-    dst = a.Yc + (static_cast<int64_t>(k2) << a.logN1) + k1;
-    stride = a.M;
-    fo = a.ftrans ? (static_cast<int64_t>(k2) << a.logN1) + k1 : k;
-    fs = a.ftrans ? a.M : a.ldF;
-  }
+  ColBin bin;
+  if (!col_bin(a, bin)) return;
+  const int64_t k = bin.k, fo = bin.fo, fs = bin.fs, stride = bin.stride;
+  c32* dst = bin.dst;
   const double kd = static_cast<double>(k);
   const float w = fast_window(a.wkind, a.wcustom, k, a.Nf) * a.scale;
   c32 c[NT];
@@ -807,6 +820,65 @@ __global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
     dst[static_cast<int64_t>(j) * stride] = acc;
     z = cmul(z, g);
     zB = cmul(zB, gB);
+  }
+}
+
+// Rates in mirrored pairs r_j = -r_{P-j} (P = pair): the bin's sums at
+// z = e^{i theta} and conj(z), theta = 2 pi k r_j/Nf, come from one
+// Reinsch-modified Goertzel recurrence (C. Reinsch, 1967; J. Oliver, "An
+// error analysis of the modified Clenshaw method", 1977) over the real and
+// the imaginary parts of the coefficients.  With s = sign(cos theta) and
+// lambda = 2 (cos theta - s),
+//   D_t = a_t + lambda U_{t+1} + s D_{t+1},  U_t = D_t + s U_{t+1}  (t = Nt-1 .. 1),
+//   sum a_t cos(t theta) = a_0 + (lambda/2) U_1 + s D_1,  sum a_t sin(t theta) = U_1 sin theta:
+// three FMAs per term and part, six per term for both rates against eight
+// for two complex Horner evaluations.  lambda comes from the FP64 cos theta,
+// so no accuracy is lost near theta = 0 or pi (where Goertzel's rounded
+// 2 cos theta would lose it).  Rates without a partner are evaluated the same
+// way and only the sum at z is stored.
+template <int NT>
+__global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
+  ColBin bin;
+  if (!col_bin(a, bin)) return;
+  const double kd = static_cast<double>(bin.k);
+  const float w = fast_window(a.wkind, a.wcustom, bin.k, a.Nf) * a.scale;
+  float cr[NT], ci[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (t < a.Nt) {
+      const c32 f = a.F[bin.fo + static_cast<int64_t>(t) * bin.fs];
+      cr[t] = f.x * w;
+      ci[t] = f.y * w;
+    } else {
+      cr[t] = 0.0f;
+      ci[t] = 0.0f;
+    }
+  }
+  const c64 g = cis64f(kd * a.hd2);
+  c64 z = cis64f(kd * a.h0);  // e^{2 pi i k r_0/Nf}
+  for (int j = 0; j < a.Nr; ++j) {
+    const int jp = a.pair - j;
+    const bool mirror = jp >= 0 && jp < a.Nr;
+    if (!mirror || jp >= j) {  // uniform: the partner's sum is stored with j's
+      const float s = z.x >= 0.0 ? 1.0f : -1.0f;
+      const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
+      const float sn = static_cast<float>(z.y);
+      float Dr = cr[NT - 1], Ur = cr[NT - 1], Di = ci[NT - 1], Ui = ci[NT - 1];
+#pragma unroll
+      for (int t = NT - 2; t >= 1; --t) {
+        Dr = fmaf(s, Dr, fmaf(lam, Ur, cr[t]));
+        Ur = fmaf(s, Ur, Dr);
+        Di = fmaf(s, Di, fmaf(lam, Ui, ci[t]));
+        Ui = fmaf(s, Ui, Di);
+      }
+      const float hl = 0.5f * lam;
+      const float Cr = fmaf(s, Dr, fmaf(hl, Ur, cr[0]));
+      const float Ci = fmaf(s, Di, fmaf(hl, Ui, ci[0]));
+      const float Sr = sn * Ur, Si = sn * Ui;
+      bin.dst[static_cast<int64_t>(j) * bin.stride] = {Cr - Si, Sr + Ci};
+      if (mirror && jp > j) bin.dst[static_cast<int64_t>(jp) * bin.stride] = {Cr + Si, Ci - Sr};
+    }
+    z = cmul(z, g);
   }
 }
 
@@ -1651,6 +1723,20 @@ int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_
   return ZDT_OK;
 }
 
+// P when the rates r0 + j dr (j < Nr) pair up as r_j = -r_{P-j} for more than
+// a third of them (P = -2 r0/dr, an integer; the Reinsch evaluation of a
+// pair costs 3/4 of two Horner evaluations, of a lone rate 3/2 of one), else -1
+static int mirror_pairs(float r0, float dr, int64_t Nr) {
+  if (dr == 0.0f) return -1;
+  const double P = -2.0 * static_cast<double>(r0) / static_cast<double>(dr);
+  if (!std::isfinite(P) || P != std::rint(P) || P < 1.0 || P > 2.0 * static_cast<double>(Nr)) return -1;
+  const int64_t Pi = static_cast<int64_t>(P);
+  int64_t npairs = 0;
+  for (int64_t j = 0; j < Nr; ++j)
+    if (Pi - j > j && Pi - j < Nr) ++npairs;
+  return 3 * npairs > Nr ? static_cast<int>(Pi) : -1;
+}
+
 int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
                  const float* window, float* dest, double scale, hipStream_t st, double* colmean, double* colvar) {
   if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
@@ -1671,17 +1757,30 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, fl
   c.wkind = wkind;
   c.wcustom = window;
   c.scale = 1.0f;
+  c.pair = mirror_pairs(r0, dr, p->Nr);
   {
     const unsigned grid = static_cast<unsigned>(p->M / 256 + 1);
-    ProfScope ps(st, "czt_horner_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
-    if (Nt <= 16) {
-      czt_horner_col<16><<<grid, 256, 0, st>>>(c);
-    } else if (Nt <= 32) {
-      czt_horner_col<32><<<grid, 256, 0, st>>>(c);
+    if (c.pair >= 0) {
+      ProfScope ps(st, "czt_reinsch_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
+      if (Nt <= 16) {
+        czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);
+      } else if (Nt <= 32) {
+        czt_reinsch_col<32><<<grid, 256, 0, st>>>(c);
+      } else {
+        czt_reinsch_col<64><<<grid, 256, 0, st>>>(c);
+      }
+      ZDT_CHECK_LAUNCH("czt_reinsch_col");
     } else {
-      czt_horner_col<64><<<grid, 256, 0, st>>>(c);
+      ProfScope ps(st, "czt_horner_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
+      if (Nt <= 16) {
+        czt_horner_col<16><<<grid, 256, 0, st>>>(c);
+      } else if (Nt <= 32) {
+        czt_horner_col<32><<<grid, 256, 0, st>>>(c);
+      } else {
+        czt_horner_col<64><<<grid, 256, 0, st>>>(c);
+      }
+      ZDT_CHECK_LAUNCH("czt_horner_col");
     }
-    ZDT_CHECK_LAUNCH("czt_horner_col");
   }
   K2cArgs a;
   a.Yc = p->Yc;

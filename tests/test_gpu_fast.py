@@ -133,6 +133,7 @@ def test_fast_unnormalized_and_hits(F):
 @pytest.mark.parametrize("Nf,Nt", [(1 << 16, 17), (1 << 17, 32), (1 << 18, 23), (1 << 19, 60), (1 << 20, 33),
                                    (1 << 16, 64)])
 def test_fused_output_vs_fp64(F, Nf, Nt):
+    # rates -2:1/8:2 pair as r_j = -r_{32-j} (the paired-rate evaluation)
     Nr = 33
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.37, 1.25, 25.0), (Nf * 0.81, -2.0, 30.0)], seed=Nf % 991 + Nt)
     say(f"fused output Nf={Nf} Nt={Nt}")
@@ -156,6 +157,20 @@ def test_fused_output_window_nodest_unnormalized(F):
     assert normwise(host(F.zdtfdr(wu)) / (Nf * wu.Nl), Z.fdr64(s, -1.0, 0.0625, 33)) <= 2e-6
     g = host(F.zdtfdr(F.ZDTWorkspace(s, -1.0, 0.0625, 33, generic=True)))
     assert normwise(host(F.zdtfdr(ws)), g.astype(np.float64)) <= 2e-6
+
+
+@pytest.mark.parametrize("r0,dr,Nr", [(-2.0, 0.125, 45), (-1.0, 0.125, 40), (0.5, 0.25, 20), (3.0, -0.25, 25),
+                                     (-0.375, 0.25, 4), (-0.3, 0.25, 20)])
+def test_fused_output_rate_sets(F, r0, dr, Nr):
+    """Rate sets r_j = r0 + j dr with pairs r_j = -r_{P-j}, P = -2 r0/dr:
+    pairs and lone rates (P = 32 of 45), too few pairs for the paired
+    evaluation (P = 16 of 40), no pairs, descending rates (P = 24), all
+    paired (P = 3 of 4), and a P that is not an integer."""
+    Nf, Nt = 1 << 17, 24
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.41, 1.75, 25.0), (Nf * 0.77, -0.5, 30.0)], seed=Nr)
+    say(f"fused output rates r0={r0} dr={dr} Nr={Nr}")
+    out = host(F.zdtfdr(F.ZDTWorkspace(s, r0, dr, Nr)))
+    assert normwise(out, Z.fdr64(s, r0, dr, Nr)) <= 2e-6
 
 
 @pytest.mark.parametrize("Nf,Nt", [(1 << 21, 17), (1 << 22, 20)])

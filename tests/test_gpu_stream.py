@@ -77,3 +77,27 @@ def test_stream_auto_batch_and_truncation(F):
     with pytest.raises(ValueError):
         st.search([np.zeros((NF, NT + 1), np.float32)])
     st.close()
+
+
+@pytest.mark.parametrize("Nf,Nt", [(1 << 17, 20), (24576, 9)])
+def test_stream_fused_output_and_generic_paths(F, Nf, Nt):
+    """The statistics come from the output kernel (fused output, Nt > 16) or
+    from colstats over each batch (generic path, Nf not a power of two)."""
+    r0, dr, Nr, Nrb = -2.0, 0.125, 33, 11
+    specs = [synth.spectrogram(Nf, Nt, [(Nf * 0.3, 1.5 - i, 30.0), (Nf * 0.7, -1.0, 25.0)], seed=400 + i)
+             for i in range(2)]
+    st = F.StreamSearch(Nf, Nt, (r0, dr, Nr), Nrb=Nrb)
+    say(f"stream Nf={Nf} Nt={Nt}: Nrb={st.Nrb} nbatch={st.nbatch}")
+    assert (st.Nrb, st.nbatch) == (Nrb, 3)
+    hits, thr = st.search(specs, snr=7.0, cap=1 << 16)
+    for i, s in enumerate(specs):
+        ws = F.ZDTWorkspace(s, r0, dr, Nrb)
+        r0s = [r0 + b * Nrb * dr for b in range(3)]
+        dests = [F.create_fdr(s, Nrb) for _ in r0s]
+        F.zdtfdr_(dests, ws, r0=r0s)
+        fdrs = [d.cpu().numpy() for d in dests]
+        m, sd = H.fdrstats(fdrs)
+        t = H.fdrdenormalize(7.0, m, sd)
+        assert abs(thr[i] - t) <= 1e-9 * abs(t), (i, thr[i], t)
+        np.testing.assert_array_equal(hits[i], H.findprotohits(fdrs, t))
+    st.close()
