@@ -1001,17 +1001,144 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   }
 }
 
-// k2_c2r for N1 = 1024, 2048 (Nf = 2^21, 2^22): 1024 threads, PP = N1/1024
-// pairs each, the inverse DFT_N1 as block-level ping-pong passes (the rows are
-// too long for one wavefront), twiddles read from the global table
-template <int N1, int N2, int ROWS>
-__global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
-  constexpr int T = 1024;
+
+// In-place inverse/forward FFT of one row of N entries owned by one
+// wavefront, N/(64 R) radix-R butterflies per lane and pass (rows of 2^10 and
+// 2^11 entries, too long for wave_fft's one butterfly per lane).  A pass
+// issues all of its reads before any of its writes, as in wave_pass; the last
+// pass calls emit(j, v) for each butterfly j of the lane (elements
+// j + r N/R) instead of writing back.  Twiddles: the table tw of W_N^m.
+template <int N, bool INV, int NS = 1, typename Emit>
+__device__ __forceinline__ void wave_fft_row(c32* x, const c32* __restrict__ tw, int lane, Emit&& emit) {
+  constexpr int R = fft_radix<N, NS>();
+  constexpr int NR = N / R;
+  constexpr int BPL = NR / 64;
+  constexpr int LOGN = ilog2c(N), LOGR = ilog2c(R), LOGNS = ilog2c(NS);
+  static_assert(BPL * 64 == NR && NR % 8 == 0, "whole butterflies per lane");
+  c32 v[BPL][R];
+#pragma unroll
+  for (int b = 0; b < BPL; ++b) {
+    const c32* xr = x + lsw(lane + 64 * b);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b][r] = xr[r * (NR + NR / 8)];
+  }
+#pragma unroll
+  for (int b = 0; b < BPL; ++b) {
+    if constexpr (NS > 1) {
+      const int k = (lane + 64 * b) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const c32 w = tw[(r * k) << (LOGN - LOGNS - LOGR)];
+        v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
+      }
+    }
+    dftd<INV, float, R>(v[b]);
+  }
+  if constexpr (NS * R < N) {
+#pragma unroll
+    for (int b = 0; b < BPL; ++b) {
+      const int j = lane + 64 * b;
+      const int k = j & (NS - 1);
+      c32* yy = x + lsw(((j >> LOGNS) << (LOGNS + LOGR)) + k);
+      if constexpr (NS == 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[b][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) yy[r * (NS + (NS >> 3))] = v[b][r];
+      }
+    }
+    wave_sync();
+    wave_fft_row<N, INV, NS * R>(x, tw, lane, emit);
+  } else {
+#pragma unroll
+    for (int b = 0; b < BPL; ++b) emit(lane + 64 * b, v[b]);
+  }
+}
+
+// Inverse DFT of one 2048-entry row owned by one wavefront, in place and
+// with 32 row entries per lane but at most 16 live at a time (a Stockham row
+// FFT keeps all 32 live, which does not fit beside k2_c2r's state).  Stage 1
+// is a radix-16 Stockham pass from natural order (butterfly j reads entries
+// j + 128 r) whose outputs go to the digit-reversed positions of an in-place
+// decimation-in-time FFT (group g = j/16 + 8 (j mod 16), entries 16 g + r');
+// stages 2 (radix 8, span 16) and 3 (radix 16, span 128) are in-place DIT
+// butterflies on entries g L + k + r L', one butterfly at a time, and stage 3
+// calls emit(k, v) with v[r] = output k + 128 r.  Rows are stored with the
+// swizzle s2048 (bits 7-10 folded into bits 0-3), which keeps every stage's
+// accesses within two-way bank conflicts; tw is the LDS table of W_2048^m.
+__device__ __forceinline__ int s2048(int i) { return i ^ ((i >> 7) & 15); }
+
+template <typename Emit>
+__device__ __forceinline__ void wave_ifft_2048(c32* x, const c32* tw, int lane, Emit&& emit) {
+  // opaque to the optimizer: keeps the ~100 lane-dependent LDS addresses from
+  // being hoisted out of the caller's rate loop (where they would spill)
+  asm volatile("" : "+v"(lane));
+  {
+    c32 v[2][16];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[b][r] = x[s2048(lane + 64 * b + 128 * r)];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      dftd<true, float, 16>(v[b]);
+      const int j = lane + 64 * b;
+      const int g = (j >> 4) + 8 * (j & 15);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) x[s2048(16 * g + r)] = v[b][r];
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int bb = lane + 64 * b;
+    const int base = (bb >> 4) * 128 + (bb & 15);
+    const int k = bb & 15;
+    c32 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = x[s2048(base + 16 * r)];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = cmulc(v[r], tw[16 * r * k]);  // W_128^{-r k}
+    dftd<true, float, 8>(v);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) x[s2048(base + 16 * r)] = v[r];
+  }
+  wave_sync();
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int k = lane + 64 * b;
+    c32 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = x[s2048(k + 128 * r)];
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmulc(v[r], tw[r * k]);  // W_2048^{-r k}
+    dftd<true, float, 16>(v);
+    emit(k, v);
+  }
+}
+
+// k2_c2r for N1 = 1024, 2048 (Nf = 2^21, 2^22): 512 threads, PP = N1/512
+// pairs each, groups of 4 rates (8 rows, one per wavefront) in one LDS
+// buffer; each wavefront runs the inverse DFT_N1 of its row in place
+// (wave_fft_row) and stores it, so a group costs two barriers.  For N1 = 1024
+// the next group's spectra are loaded into registers before the DFTs; at
+// N1 = 2048 the 32 row entries per lane leave no registers for that.
+template <int N1, int N2>
+__global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
+  constexpr int T = 512;
   constexpr int PP = N1 / T;
-  constexpr int P = N1 + N1 / 8;
+  constexpr bool BIG = N1 == 2048;  // wave_ifft_2048 layout (no row padding)
+  constexpr int P = BIG ? N1 : N1 + N1 / 8;
   constexpr int HALF = N1 / 2;
+  constexpr int ROWS = 8;
+  constexpr int JG = ROWS / 2;
   __shared__ c32 buf[ROWS * P];
-  __shared__ c32 buf2[ROWS * P];
+  __shared__ c32 twl[BIG ? N1 : 1];
+  if constexpr (BIG) {
+    for (int i = threadIdx.x; i < N1; i += T) twl[i] = a.tw1[i];
+  }
+  auto pos = [](int i) { return BIG ? s2048(i) : lsw(i); };
   const int tid = threadIdx.x;
   const int p = blockIdx.x;
   const bool special = (p == 0);
@@ -1058,9 +1185,6 @@ __global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
       a.yhalf[j] = {y.x * a.oscale, -y.y * a.oscale};
     }
   }
-  constexpr int JG = ROWS / 2;
-  // the next group's spectra are loaded into registers before this group's
-  // DFT, so their latency overlaps the LDS passes
   c32 ya[JG][PP], yb[JG][PP];
   auto load = [&](int j0n) {
     const int njn = min(JG, a.Nr - j0n);
@@ -1076,9 +1200,12 @@ __global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
       }
     }
   };
-  load(0);
+  constexpr bool PF = PP <= 2;
+  if (PF) load(0);
+  const int wv = tid >> 6, lane = tid & 63;
   for (int j0 = 0; j0 < a.Nr; j0 += JG) {
     const int nj = min(JG, a.Nr - j0);
+    if (!PF) load(j0);
 #pragma unroll
     for (int jj = 0; jj < JG; ++jj) {
       if (jj < nj) {
@@ -1087,29 +1214,38 @@ __global__ __launch_bounds__(1024) void k2_c2r_big(K2cArgs a) {
           const c32 YA = {ya[jj][i].x * h, ya[jj][i].y * h}, YB = {yb[jj][i].x * h, yb[jj][i].y * h};
           if (nyq[i]) {
             buf[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
-            buf[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
+            buf[(2 * jj + rowA[i]) * P + pos(HALF)] = {0.0f, 0.0f};
           } else {
             const c32 E = {YA.x + YB.x, YA.y - YB.y};
             const c32 D = {YA.x - YB.x, YA.y + YB.y};
             const c32 O = cmulc(D, WA[i]);
-            buf[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
-            buf[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};
+            buf[(2 * jj + rowA[i]) * P + pos(k1A[i])] = {E.x - O.y, E.y + O.x};
+            buf[(2 * jj + rowB[i]) * P + pos(k1B[i])] = {E.x + O.y, -E.y + O.x};
           }
         }
       }
     }
     __syncthreads();
-    if (j0 + JG < a.Nr) load(j0 + JG);
-    const c32* fo = lds_fft_pp<N1, T, true>(buf, buf2, P, 2 * nj, a.tw1);
-    for (int r = 0; r < 2 * nj; ++r) {
-      c32* dst = a.B + static_cast<int64_t>(j0 + (r >> 1)) * (N1 * N2) + ((r & 1) ? colB : colA) * 16;
+    if (PF && j0 + JG < a.Nr) load(j0 + JG);
+    if (wv < 2 * nj) {  // uniform per wavefront: wavefront wv owns row wv
+      c32* dst = a.B + static_cast<int64_t>(j0 + (wv >> 1)) * (N1 * N2) + ((wv & 1) ? colB : colA) * 16;
+      auto emit = [&](int jx, auto& v) {
+        constexpr int RL = sizeof(v) / sizeof(c32);
 #pragma unroll
-      for (int i = 0; i < PP; ++i) {
-        const int n1 = tid + i * T;
-        dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = fo[r * P + lsw(n1)];
+        for (int r = 0; r < RL; ++r) {
+          const int n1 = jx + r * (N1 / RL);
+          dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
+        }
+      };
+      if constexpr (BIG) {
+        wave_ifft_2048(buf + wv * P, twl, lane, emit);
+      } else {
+        int ln = lane;
+        asm volatile("" : "+v"(ln));  // keeps the row FFT's LDS addresses inside the rate loop
+        wave_fft_row<N1, true>(buf + wv * P, a.tw1, ln, emit);
       }
     }
-    __syncthreads();
+    __syncthreads();  // the rows are rewritten by the next group
   }
 }
 
@@ -1660,8 +1796,7 @@ static int launch_k2c(const K2cArgs& a, hipStream_t st) {
   if constexpr (N1 <= 512) {
     k2_c2r<N1, N2, k2_rows(N1)><<<N2 / 2, N1, 0, st>>>(a);
   } else {
-    // rows of 2^10 or 2^11 entries: 8 or 4 rows per ping-pong buffer
-    k2_c2r_big<N1, N2, (N1 == 1024 ? 8 : 4)><<<N2 / 2, 1024, 0, st>>>(a);
+    k2_c2r_wave<N1, N2><<<N2 / 2, 512, 0, st>>>(a);
   }
   ZDT_CHECK_LAUNCH("k2_c2r");
   return ZDT_OK;
