@@ -323,7 +323,15 @@ struct K3Args {
   int Nr;
   int ntiles;    // Nr * N1/16
   double* part;  // NULL, or per-wavefront FDR statistics [Nr][N1/16][waves][3] (see k3_stats)
+  int paired;    // k2 wrote the rates in mirrored pairs from the middle outwards
 };
+
+// k3's i-th rate: the reverse of k2's write order, so the spectra written
+// last are read first (Nr-1, Nr-2, ... or, for paired rates, Nr-1, 0, Nr-2, 1, ...)
+__device__ __forceinline__ int k3_rate(int i, int Nr, int paired) {
+  if (!paired) return Nr - 1 - i;
+  return (i & 1) ? (i >> 1) : Nr - 1 - (i >> 1);
+}
 
 // Each wavefront of a k3 tile writes (S1, S2, c): the FP64 sums of (x - c) and
 // (x - c)^2 over its 64 x 64 FDR values, c = lane 0's first value.  Each lane
@@ -361,17 +369,18 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
   auto load = [&](int tl, c32(&x)[32], c32& h) {
     const int jr = tl / nblk, nb = tl - jr * nblk;
     // the tile's 16 columns are one contiguous [k2][16] block of B
-    const c32* b = a.B + static_cast<int64_t>(a.Nr - 1 - jr) * C::N2 * N1 + static_cast<int64_t>(nb) * C::N2 * 16;
+    const int jj = k3_rate(jr, a.Nr, a.paired);
+    const c32* b = a.B + static_cast<int64_t>(jj) * C::N2 * N1 + static_cast<int64_t>(nb) * C::N2 * 16;
 #pragma unroll
     for (int m = 0; m < 32; ++m) x[m] = b[static_cast<unsigned>(r + R1 * m) * 16 + c];
-    h = a.yhalf[a.Nr - 1 - jr];
+    h = a.yhalf[jj];
   };
   c32 x[32], h;
   load(tile, x, h);
   __syncthreads();
   for (;;) {
     const int jr = tile / nblk, nb = tile - jr * nblk;
-    const int j = a.Nr - 1 - jr;
+    const int j = k3_rate(jr, a.Nr, a.paired);
     if (r == 0) x[0] = (c & 1) ? csub(x[0], h) : cadd(x[0], h);
     {
       // four-step twiddle W_M^{-n1 k2}, k2 = r + R1 m, as A[m mod 8] B[m / 8]
@@ -439,6 +448,8 @@ struct K2Args {
   c32* B;            // [Nr][N1/16][N2][16]
   const c32* tw1;    // W_N1^m
   const c32* tabq;   // e^{i pi dr j}, j < Nr
+  const c64* tab64;  // the same in FP64 (paired rates)
+  c64 eir0;          // e^{i pi r0} (paired rates)
   c32* yhalf;        // [Nr] c2r pre-pass value of the bin M/2, added by k3
   int Nt, Nr;
   int logN1;
@@ -483,6 +494,30 @@ constexpr int k2_waves_per_eu(int n1) { return n1 >= 512 ? 4 : 2; }
 // wavefront-level passes (8 rows for 512 = 8*8*8 over 8 wavefronts, ...)
 constexpr int k2_rows(int n1) { return n1 == 256 ? 16 : 8; }
 
+// Sums of the coefficients c_t at z and conj(z), |z| = 1, by the
+// Reinsch-modified Goertzel recurrence over the real and imaginary parts
+// (see czt_reinsch_col); s, lambda and sin(theta) come from the FP64 z.
+template <int NT>
+__device__ __forceinline__ void reinsch2(const c32 (&c)[NT], c64 z, c32& Yz, c32& Yzc) {
+  const float s = z.x >= 0.0 ? 1.0f : -1.0f;
+  const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
+  const float sn = static_cast<float>(z.y);
+  float Dr = c[NT - 1].x, Ur = c[NT - 1].x, Di = c[NT - 1].y, Ui = c[NT - 1].y;
+#pragma unroll
+  for (int t = NT - 2; t >= 1; --t) {
+    Dr = fmaf(s, Dr, fmaf(lam, Ur, c[t].x));
+    Ur = fmaf(s, Ur, Dr);
+    Di = fmaf(s, Di, fmaf(lam, Ui, c[t].y));
+    Ui = fmaf(s, Ui, Di);
+  }
+  const float hl = 0.5f * lam;
+  const float Cr = fmaf(s, Dr, fmaf(hl, Ur, c[0].x));
+  const float Ci = fmaf(s, Di, fmaf(hl, Ui, c[0].y));
+  const float Sr = sn * Ur, Si = sn * Ui;
+  Yz = {Cr - Si, Sr + Ci};
+  Yzc = {Cr + Si, Ci - Sr};
+}
+
 // Block of T = min(N1, 512) threads; each thread owns PP = N1/T Hermitian
 // pairs q = tid + i*T (i < PP).  Pair q of a regular block (columns c, N2-c)
 // is (k = c + N2 q, M - k = (N2-c) + N2 (N1-1-q)); the block for p = 0 holds the
@@ -493,7 +528,13 @@ constexpr int k2_rows(int n1) { return n1 == 256 ? 16 : 8; }
 // evaluates C_j for every rate up front into yhalf[j] and k3 adds it, which
 // keeps its registers out of the rate loop.  The window weight and output
 // scale are folded into the Vandermonde coefficients a_t.
-template <int N1, int N2, int NT, int ROWS>
+//
+// PAIRED: the rates are symmetric, r_j = -r_{Nr-1-j}.  The coefficients are
+// then not pre-rotated; rates j and Nr-1-j of a bin are its sums at
+// z = e^{2 pi i k r_j/Nf} and conj(z) and come from one reinsch2 (partner
+// bin: z = e^{i pi r_j} conj(z_k)), and a group of 4 rates holds the pairs
+// (j, Nr-1-j), (j+1, Nr-2-j).
+template <int N1, int N2, int NT, int ROWS, bool PAIRED = false>
 __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_czt(K2Args a) {
   constexpr int T = N1 < 512 ? N1 : 512;
   constexpr int PP = N1 / T;
@@ -624,11 +665,16 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
             XA = cadd(E, WO);
             XB = {E.x - WO.x, -(E.y - WO.y)};
           }
-          // pre-rotation e^{2 pi i k u r0/Nf}; partner: e^{i pi u r0} conj(.)
-          const c32 rot = cis_nh(static_cast<double>(kA[i]) * ud, a.h0);
-          const c32 ra = cmul(XA, rot), rb = cmul(XB, cmulc(rq, rot));
-          aA[i][u] = {ra.x * wA[i], ra.y * wA[i]};
-          aB[i][u] = {rb.x * wB[i], rb.y * wB[i]};
+          if constexpr (PAIRED) {
+            aA[i][u] = {XA.x * wA[i], XA.y * wA[i]};
+            aB[i][u] = {XB.x * wB[i], XB.y * wB[i]};
+          } else {
+            // pre-rotation e^{2 pi i k u r0/Nf}; partner: e^{i pi u r0} conj(.)
+            const c32 rot = cis_nh(static_cast<double>(kA[i]) * ud, a.h0);
+            const c32 ra = cmul(XA, rot), rb = cmul(XB, cmulc(rq, rot));
+            aA[i][u] = {ra.x * wA[i], ra.y * wA[i]};
+            aB[i][u] = {rb.x * wB[i], rb.y * wB[i]};
+          }
         }
         if (special && tid == 0) {
           // bin M/2: X = conj(Z[M/2]); rotation e^{2 pi i (M/2) u r0/Nf} = e^{i pi u r0/2}
@@ -653,70 +699,150 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     }
   }
 
-  // ---- chirp-Z for every rate, c2r pre-pass, inverse DFT_N1, store
-  c64 gam[PP], z[PP];
+  if constexpr (PAIRED) {
+    constexpr int JG = ROWS / 2;
+    static_assert(JG == 4, "groups of two rate pairs");
+    constexpr int NW = T / 64;
+    constexpr int RPW = ROWS / NW;
+    static_assert(RPW * NW == ROWS && RPW * fft_max_nr<N1>() <= 64, "one butterfly per lane and pass");
+    const int wv = tid >> 6, lane = tid & 63;
+    const int Pm = a.Nr - 1;        // rate j pairs with Pm - j
+    const int npairs = Pm / 2 + 1;  // the last is the middle rate when Pm is even
+    c64 gam[PP], gamr2[PP], zp[PP];
 #pragma unroll
-  for (int i = 0; i < PP; ++i) {
-    gam[i] = cis64f(static_cast<double>(kA[i]) * a.hd2);
-    z[i] = {1.0, 0.0};
-  }
-
-  // Rate groups of JG rates (2 JG rows: columns A and B of each rate).  The
-  // rows alternate between buf and buf2 from group to group, so the one
-  // barrier per group (after the c2r pre-pass writes) also orders a group's
-  // writes after the previous-but-one group's FFT reads.  Each wavefront
-  // then runs the inverse DFT_N1 of its own rows in place and stores them.
-  constexpr int JG = ROWS / 2;
-  constexpr int NW = T / 64;      // wavefronts
-  constexpr int RPW = ROWS / NW;  // rows per wavefront
-  static_assert(RPW * NW == ROWS && RPW * fft_max_nr<N1>() <= 64, "one butterfly per lane and pass");
-  const int wv = tid >> 6, lane = tid & 63;
-  c32* cur = buf;
-  c32* nxt = buf2;
-  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
-    const int nj = min(JG, a.Nr - j0);
+    for (int i = 0; i < PP; ++i) {
+      gam[i] = cis64f(static_cast<double>(kA[i]) * a.hd2);
+      gamr2[i] = cis64f(-2.0 * static_cast<double>(kA[i]) * a.hd2);
+    }
+    // groups from the middle rates outwards, so that the rates k3 reads
+    // first (Nr-1, 0, Nr-2, 1, ...) are the ones written last
+    const int g0 = ((npairs - 1) / 2) * 2;
 #pragma unroll
-    for (int jj = 0; jj < JG; ++jj) {
-      if (jj >= nj) break;  // uniform
-      const c32 qj = a.tabq[j0 + jj];
+    for (int i = 0; i < PP; ++i)  // e^{2 pi i k r_g0/Nf}
+      zp[i] = cis64f(static_cast<double>(kA[i]) * (a.h0 + static_cast<double>(g0) * a.hd2));
+    c32* cur = buf;
+    c32* nxt = buf2;
+    for (int p0 = g0; p0 >= 0; p0 -= 2) {
+      const int np = min(2, npairs - p0);
 #pragma unroll
-      for (int i = 0; i < PP; ++i) {
-        const c32 zA = {static_cast<float>(z[i].x), static_cast<float>(z[i].y)};
-        z[i] = cmul(z[i], gam[i]);
-        const c32 zB = cmulc(qj, zA);  // e^{i pi dr j} conj(zA)
-        // YA, YB: Ys of bins kA and M - kA, halved
-        const c32 YA = horner<NT>(aA[i], zA);
-        const c32 YB = horner<NT>(aB[i], zB);
-        if (nyq[i]) {
-          // pair (0, M): only the real parts enter the c2r; the bin M/2 slot
-          // of the row is zero here (k3 adds it)
-          cur[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
-          cur[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
-        } else {
-          const c32 E = {YA.x + YB.x, YA.y - YB.y};  // (YA + conj YB)/2
-          const c32 D = {YA.x - YB.x, YA.y + YB.y};  // (YA - conj YB)/2
-          const c32 O = cmulc(D, WA[i]);  // W^{-k} D
-          cur[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
-          cur[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
+      for (int pp = 0; pp < 2; ++pp) {
+        if (pp >= np) break;  // uniform
+        const int jA = p0 + pp;
+        const bool twin = Pm - jA != jA;
+        const c64 q = cmul(a.eir0, a.tab64[jA]);  // e^{i pi r_jA}
+#pragma unroll
+        for (int i = 0; i < PP; ++i) {
+          const c64 zA = pp == 0 ? zp[i] : cmul(zp[i], gam[i]);
+          const c64 zB = cmul(q, c64{zA.x, -zA.y});
+          c32 YA[2], YB[2];  // [0]: rate jA, [1]: rate Pm - jA
+          reinsch2<NT>(aA[i], zA, YA[0], YA[1]);
+          reinsch2<NT>(aB[i], zB, YB[0], YB[1]);
+#pragma unroll
+          for (int m = 0; m < 2; ++m) {
+            if (m == 1 && !twin) break;
+            const int slot = 2 * pp + m;
+            if (nyq[i]) {
+              cur[(2 * slot + rowA[i]) * P] = {YA[m].x + YB[m].x, YA[m].x - YB[m].x};
+              cur[(2 * slot + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
+            } else {
+              const c32 E = {YA[m].x + YB[m].x, YA[m].y - YB[m].y};
+              const c32 D = {YA[m].x - YB[m].x, YA[m].y + YB[m].y};
+              const c32 O = cmulc(D, WA[i]);
+              cur[(2 * slot + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
+              cur[(2 * slot + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};
+            }
+          }
         }
       }
-    }
-    __syncthreads();
-    const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));  // uniform per wavefront
-    wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
-      constexpr int RL = sizeof(v) / sizeof(c32);
-      const int row = wv * RPW + f;
-      // B is blocked by 16 n1: B[j][n1/16][k2][n1%16]
-      c32* dst = a.B + static_cast<int64_t>(j0 + (row >> 1)) * (N1 * N2) + ((row & 1) ? colB : colA) * 16;
+      __syncthreads();
+      // slots 0..3 hold rates p0, Pm-p0, p0+1, Pm-p0-1 (the middle rate once)
+      const int nslots = (np == 2 ? 4 : 2) - ((Pm & 1) == 0 && p0 + np == npairs ? 1 : 0);
+      const int nrw = min(RPW, max(0, 2 * nslots - wv * RPW));  // uniform per wavefront
+      wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+        constexpr int RL = sizeof(v) / sizeof(c32);
+        const int row = wv * RPW + f;
+        const int slot = row >> 1;
+        const int rate = (slot & 1) ? Pm - p0 - (slot >> 1) : p0 + (slot >> 1);
+        c32* dst = a.B + static_cast<int64_t>(rate) * (N1 * N2) + ((row & 1) ? colB : colA) * 16;
 #pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const int n1 = jx + r * (N1 / RL);
-        dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
+        for (int r = 0; r < RL; ++r) {
+          const int n1 = jx + r * (N1 / RL);
+          dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
+        }
+      });
+      c32* t = cur;
+      cur = nxt;
+      nxt = t;
+#pragma unroll
+      for (int i = 0; i < PP; ++i) zp[i] = cmul(zp[i], gamr2[i]);  // back two pairs
+    }
+  } else {
+    // ---- chirp-Z for every rate, c2r pre-pass, inverse DFT_N1, store
+    c64 gam[PP], z[PP];
+  #pragma unroll
+    for (int i = 0; i < PP; ++i) {
+      gam[i] = cis64f(static_cast<double>(kA[i]) * a.hd2);
+      z[i] = {1.0, 0.0};
+    }
+
+    // Rate groups of JG rates (2 JG rows: columns A and B of each rate).  The
+    // rows alternate between buf and buf2 from group to group, so the one
+    // barrier per group (after the c2r pre-pass writes) also orders a group's
+    // writes after the previous-but-one group's FFT reads.  Each wavefront
+    // then runs the inverse DFT_N1 of its own rows in place and stores them.
+    constexpr int JG = ROWS / 2;
+    constexpr int NW = T / 64;      // wavefronts
+    constexpr int RPW = ROWS / NW;  // rows per wavefront
+    static_assert(RPW * NW == ROWS && RPW * fft_max_nr<N1>() <= 64, "one butterfly per lane and pass");
+    const int wv = tid >> 6, lane = tid & 63;
+    c32* cur = buf;
+    c32* nxt = buf2;
+    for (int j0 = 0; j0 < a.Nr; j0 += JG) {
+      const int nj = min(JG, a.Nr - j0);
+  #pragma unroll
+      for (int jj = 0; jj < JG; ++jj) {
+        if (jj >= nj) break;  // uniform
+        const c32 qj = a.tabq[j0 + jj];
+  #pragma unroll
+        for (int i = 0; i < PP; ++i) {
+          const c32 zA = {static_cast<float>(z[i].x), static_cast<float>(z[i].y)};
+          z[i] = cmul(z[i], gam[i]);
+          const c32 zB = cmulc(qj, zA);  // e^{i pi dr j} conj(zA)
+          // YA, YB: Ys of bins kA and M - kA, halved
+          const c32 YA = horner<NT>(aA[i], zA);
+          const c32 YB = horner<NT>(aB[i], zB);
+          if (nyq[i]) {
+            // pair (0, M): only the real parts enter the c2r; the bin M/2 slot
+            // of the row is zero here (k3 adds it)
+            cur[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
+            cur[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
+          } else {
+            const c32 E = {YA.x + YB.x, YA.y - YB.y};  // (YA + conj YB)/2
+            const c32 D = {YA.x - YB.x, YA.y + YB.y};  // (YA - conj YB)/2
+            const c32 O = cmulc(D, WA[i]);  // W^{-k} D
+            cur[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
+            cur[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
+          }
+        }
       }
-    });
-    c32* t = cur;
-    cur = nxt;
-    nxt = t;
+      __syncthreads();
+      const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));  // uniform per wavefront
+      wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+        constexpr int RL = sizeof(v) / sizeof(c32);
+        const int row = wv * RPW + f;
+        // B is blocked by 16 n1: B[j][n1/16][k2][n1%16]
+        c32* dst = a.B + static_cast<int64_t>(j0 + (row >> 1)) * (N1 * N2) + ((row & 1) ? colB : colA) * 16;
+  #pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const int n1 = jx + r * (N1 / RL);
+          dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
+        }
+      });
+      c32* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+
   }
 }
 
@@ -1435,10 +1561,12 @@ struct FastPlan {
   c32* tw1 = nullptr;   // W_N1^m
   c32* tw2 = nullptr;   // W_N2^m
   c32* tabq = nullptr;  // e^{i pi dr j}
+  c64* tab64 = nullptr;  // e^{i pi dr j} in FP64
   c32* yhalf = nullptr;  // [Nr] bin M/2 term of the c2r (k2 -> k3)
   c32* Yc = nullptr;     // fused output only: [Nr][N2][N1] spectra
   c32* yM = nullptr;     // fused output only: [Nr] Nyquist bin
   double* part = nullptr;  // k3's per-wavefront FDR statistics (fast_stats_doubles)
+  bool paired = false;     // the last k2 wrote B in mirrored rate pairs
   bool have_B = false;
   int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
 };
@@ -1493,7 +1621,7 @@ static void alloc_part(FastPlan* p, int& rc) {
 void fast_destroy(FastPlan* p) {
   if (p == nullptr) return;
   for (void* q : {static_cast<void*>(p->G), static_cast<void*>(p->B), static_cast<void*>(p->tw1),
-                  static_cast<void*>(p->tw2), static_cast<void*>(p->tabq), static_cast<void*>(p->yhalf),
+                  static_cast<void*>(p->tw2), static_cast<void*>(p->tabq), static_cast<void*>(p->tab64), static_cast<void*>(p->yhalf),
                   static_cast<void*>(p->Yc), static_cast<void*>(p->yM), static_cast<void*>(p->part)})
     if (q) (void)hipFree(q);
   delete p;
@@ -1552,6 +1680,23 @@ int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, FastPlan**
       q[j] = {static_cast<float>(cosl(2 * pi * rev)), static_cast<float>(sinl(2 * pi * rev))};
     }
     rc = upload_table(&p->tabq, q);
+    if (rc == ZDT_OK) {
+      std::vector<c64> q64(Nr);
+      for (int64_t j = 0; j < Nr; ++j) {
+        long double rev = 0.5L * static_cast<long double>(dr) * static_cast<long double>(j);
+        rev -= floorl(rev);
+        q64[j] = {static_cast<double>(cosl(2 * pi * rev)), static_cast<double>(sinl(2 * pi * rev))};
+      }
+      hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->tab64), q64.size() * sizeof(c64));
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p->tab64 = nullptr;
+        rc = fail(ZDT_ENOMEM, "hipMalloc(table) failed");
+      } else {
+        e = hipMemcpy(p->tab64, q64.data(), q64.size() * sizeof(c64), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(ZDT_EHIP, "table upload failed: %s", hipGetErrorString(e));
+      }
+    }
   }
   if (rc != ZDT_OK) {
     fast_destroy(p);
@@ -1585,10 +1730,17 @@ static int launch_k1(const FastPlan* p, const float* spec, c32* G, int64_t Nt, h
 int fast_input(FastPlan* p, const float* spec, hipStream_t st) { return launch_k1(p, spec, p->G, p->Nt, st); }
 
 template <int N1, int N2>
-static int launch_k2(const K2Args& a, int NT, hipStream_t st) {
+static int launch_k2(const K2Args& a, int NT, bool paired, hipStream_t st) {
   constexpr unsigned grid = N2 / 2;
   constexpr unsigned TB = N1 < 512 ? N1 : 512;
   constexpr int RW = k2_rows(N1);
+  if constexpr (RW == 8) {  // groups of two rate pairs
+    if (paired && NT == 16) {
+      k2_czt<N1, N2, 16, RW, true><<<grid, TB, 0, st>>>(a);
+      ZDT_CHECK_LAUNCH("k2_czt");
+      return ZDT_OK;
+    }
+  }
   switch (NT) {
     case 2:
       k2_czt<N1, N2, 2, RW><<<grid, TB, 0, st>>>(a);
@@ -1614,6 +1766,16 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
   a.B = p->B;
   a.tw1 = p->tw1;
   a.tabq = p->tabq;
+  a.tab64 = p->tab64;
+  {
+    const double h = 0.5 * static_cast<double>(r0);  // e^{i pi r0}
+    a.eir0 = {std::cos(2.0 * M_PI * (h - std::rint(h))), std::sin(2.0 * M_PI * (h - std::rint(h)))};
+  }
+  // symmetric rates (r0 = -(Nr-1) dr/2 exactly): the paired evaluation
+  // (compiled for NT = 16 with 8-row groups)
+  const bool paired = p->Nr > 1 && -2.0 * static_cast<double>(r0) == static_cast<double>(p->Nr - 1) * p->dr &&
+                      p->NT == 16 && k2_rows(p->N1) == 8;
+  p->paired = paired;
   a.yhalf = p->yhalf;
   a.Nt = static_cast<int>(p->Nt);
   a.Nr = static_cast<int>(p->Nr);
@@ -1631,28 +1793,28 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
     const int key = p->N1 * 2048 + p->N2;
     switch (key) {
       case 64 * 2048 + 512:
-        ZDT_TRY((launch_k2<64, 512>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<64, 512>(a, p->NT, paired, st)));
         break;
       case 64 * 2048 + 1024:
-        ZDT_TRY((launch_k2<64, 1024>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<64, 1024>(a, p->NT, paired, st)));
         break;
       case 128 * 2048 + 512:
-        ZDT_TRY((launch_k2<128, 512>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<128, 512>(a, p->NT, paired, st)));
         break;
       case 256 * 2048 + 512:
-        ZDT_TRY((launch_k2<256, 512>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<256, 512>(a, p->NT, paired, st)));
         break;
       case 512 * 2048 + 512:
-        ZDT_TRY((launch_k2<512, 512>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<512, 512>(a, p->NT, paired, st)));
         break;
       case 128 * 2048 + 1024:
-        ZDT_TRY((launch_k2<128, 1024>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<128, 1024>(a, p->NT, paired, st)));
         break;
       case 256 * 2048 + 1024:
-        ZDT_TRY((launch_k2<256, 1024>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<256, 1024>(a, p->NT, paired, st)));
         break;
       case 512 * 2048 + 1024:
-        ZDT_TRY((launch_k2<512, 1024>(a, p->NT, st)));
+        ZDT_TRY((launch_k2<512, 1024>(a, p->NT, paired, st)));
         break;
       default:
         return fail(ZDT_ENOTSUP, "fused path: N1=%d N2=%d", p->N1, p->N2);
@@ -1718,6 +1880,7 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* 
   a.Nr = static_cast<int>(p->Nr);
   a.ntiles = static_cast<int>(p->Nr * (p->N1 / 16));
   a.part = colmean != nullptr ? p->part : nullptr;
+  a.paired = p->paired ? 1 : 0;
   {
     ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
     if (p->N2 == 1024) {
