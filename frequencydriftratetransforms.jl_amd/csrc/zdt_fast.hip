@@ -532,8 +532,8 @@ __device__ __forceinline__ void reinsch2(const c32 (&c)[NT], c64 z, c32& Yz, c32
 // PAIRED: the rates are symmetric, r_j = -r_{Nr-1-j}.  The coefficients are
 // then not pre-rotated; rates j and Nr-1-j of a bin are its sums at
 // z = e^{2 pi i k r_j/Nf} and conj(z) and come from one reinsch2 (partner
-// bin: z = e^{i pi r_j} conj(z_k)), and a group of 4 rates holds the pairs
-// (j, Nr-1-j), (j+1, Nr-2-j).
+// bin: z = e^{i pi r_j} conj(z_k)), and a group of ROWS/2 rates holds the
+// pairs (j, Nr-1-j), (j+1, Nr-2-j), ...
 template <int N1, int N2, int NT, int ROWS, bool PAIRED = false>
 __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_czt(K2Args a) {
   constexpr int T = N1 < 512 ? N1 : 512;
@@ -701,38 +701,41 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
 
   if constexpr (PAIRED) {
     constexpr int JG = ROWS / 2;
-    static_assert(JG == 4, "groups of two rate pairs");
+    constexpr int NPG = JG / 2;  // rate pairs per group
     constexpr int NW = T / 64;
     constexpr int RPW = ROWS / NW;
     static_assert(RPW * NW == ROWS && RPW * fft_max_nr<N1>() <= 64, "one butterfly per lane and pass");
     const int wv = tid >> 6, lane = tid & 63;
     const int Pm = a.Nr - 1;        // rate j pairs with Pm - j
     const int npairs = Pm / 2 + 1;  // the last is the middle rate when Pm is even
-    c64 gam[PP], gamr2[PP], zp[PP];
+    c64 gam[PP], gamrg[PP], zp[PP], zq[PP];
 #pragma unroll
     for (int i = 0; i < PP; ++i) {
       gam[i] = cis64f(static_cast<double>(kA[i]) * a.hd2);
-      gamr2[i] = cis64f(-2.0 * static_cast<double>(kA[i]) * a.hd2);
+      gamrg[i] = cis64f(-static_cast<double>(NPG) * static_cast<double>(kA[i]) * a.hd2);
     }
     // groups from the middle rates outwards, so that the rates k3 reads
     // first (Nr-1, 0, Nr-2, 1, ...) are the ones written last
-    const int g0 = ((npairs - 1) / 2) * 2;
+    const int g0 = ((npairs - 1) / NPG) * NPG;
 #pragma unroll
     for (int i = 0; i < PP; ++i)  // e^{2 pi i k r_g0/Nf}
       zp[i] = cis64f(static_cast<double>(kA[i]) * (a.h0 + static_cast<double>(g0) * a.hd2));
     c32* cur = buf;
     c32* nxt = buf2;
-    for (int p0 = g0; p0 >= 0; p0 -= 2) {
-      const int np = min(2, npairs - p0);
+    for (int p0 = g0; p0 >= 0; p0 -= NPG) {
+      const int np = min(NPG, npairs - p0);
 #pragma unroll
-      for (int pp = 0; pp < 2; ++pp) {
+      for (int i = 0; i < PP; ++i) zq[i] = zp[i];
+#pragma unroll
+      for (int pp = 0; pp < NPG; ++pp) {
         if (pp >= np) break;  // uniform
         const int jA = p0 + pp;
         const bool twin = Pm - jA != jA;
         const c64 q = cmul(a.eir0, a.tab64[jA]);  // e^{i pi r_jA}
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
-          const c64 zA = pp == 0 ? zp[i] : cmul(zp[i], gam[i]);
+          const c64 zA = zq[i];
+          zq[i] = cmul(zq[i], gam[i]);
           const c64 zB = cmul(q, c64{zA.x, -zA.y});
           c32 YA[2], YB[2];  // [0]: rate jA, [1]: rate Pm - jA
           reinsch2<NT>(aA[i], zA, YA[0], YA[1]);
@@ -755,8 +758,8 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
         }
       }
       __syncthreads();
-      // slots 0..3 hold rates p0, Pm-p0, p0+1, Pm-p0-1 (the middle rate once)
-      const int nslots = (np == 2 ? 4 : 2) - ((Pm & 1) == 0 && p0 + np == npairs ? 1 : 0);
+      // slots 2pp, 2pp+1 hold rates p0+pp, Pm-p0-pp (the middle rate once, last)
+      const int nslots = 2 * np - ((Pm & 1) == 0 && p0 + np == npairs ? 1 : 0);
       const int nrw = min(RPW, max(0, 2 * nslots - wv * RPW));  // uniform per wavefront
       wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
         constexpr int RL = sizeof(v) / sizeof(c32);
@@ -774,7 +777,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       cur = nxt;
       nxt = t;
 #pragma unroll
-      for (int i = 0; i < PP; ++i) zp[i] = cmul(zp[i], gamr2[i]);  // back two pairs
+      for (int i = 0; i < PP; ++i) zp[i] = cmul(zp[i], gamrg[i]);  // back one group
     }
   } else {
     // ---- chirp-Z for every rate, c2r pre-pass, inverse DFT_N1, store
@@ -1734,12 +1737,10 @@ static int launch_k2(const K2Args& a, int NT, bool paired, hipStream_t st) {
   constexpr unsigned grid = N2 / 2;
   constexpr unsigned TB = N1 < 512 ? N1 : 512;
   constexpr int RW = k2_rows(N1);
-  if constexpr (RW == 8) {  // groups of two rate pairs
-    if (paired && NT == 16) {
-      k2_czt<N1, N2, 16, RW, true><<<grid, TB, 0, st>>>(a);
-      ZDT_CHECK_LAUNCH("k2_czt");
-      return ZDT_OK;
-    }
+  if (paired && NT == 16) {
+    k2_czt<N1, N2, 16, RW, true><<<grid, TB, 0, st>>>(a);
+    ZDT_CHECK_LAUNCH("k2_czt");
+    return ZDT_OK;
   }
   switch (NT) {
     case 2:
@@ -1772,9 +1773,9 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
     a.eir0 = {std::cos(2.0 * M_PI * (h - std::rint(h))), std::sin(2.0 * M_PI * (h - std::rint(h)))};
   }
   // symmetric rates (r0 = -(Nr-1) dr/2 exactly): the paired evaluation
-  // (compiled for NT = 16 with 8-row groups)
+  // (compiled for NT = 16)
   const bool paired = p->Nr > 1 && -2.0 * static_cast<double>(r0) == static_cast<double>(p->Nr - 1) * p->dr &&
-                      p->NT == 16 && k2_rows(p->N1) == 8;
+                      p->NT == 16;
   p->paired = paired;
   a.yhalf = p->yhalf;
   a.Nt = static_cast<int>(p->Nt);

@@ -51,13 +51,15 @@ def test_fast_sizes(F, Nf, Nt):
     assert normwise(out, ref) <= 2e-6
 
 
-@pytest.mark.parametrize("r0,dr,Nr,Nt", [(-3.75, 0.5, 16, 16), (-4.0, 0.25, 33, 16), (2.0, -0.25, 17, 11),
-                                        (-0.5, 1.0, 2, 16), (-1.0, 1.0, 3, 13), (-1.0, 0.5, 5, 16)])
-def test_fast_paired_rates(F, r0, dr, Nr, Nt):
+@pytest.mark.parametrize("Nf,r0,dr,Nr,Nt", [(1 << 17, -3.75, 0.5, 16, 16), (1 << 17, -4.0, 0.25, 33, 16),
+                                           (1 << 17, 2.0, -0.25, 17, 11), (1 << 17, -0.5, 1.0, 2, 16),
+                                           (1 << 17, -1.0, 1.0, 3, 13), (1 << 17, -1.0, 0.5, 5, 16),
+                                           (1 << 19, -2.0, 0.125, 33, 16), (1 << 19, -1.25, 0.25, 11, 14)])
+def test_fast_paired_rates(F, Nf, r0, dr, Nr, Nt):
     """Symmetric rate sets (r_j = -r_{Nr-1-j}) take k2's paired evaluation:
     even Nr, odd Nr (a middle rate), descending rates, one pair, one pair and
-    the middle, and a last group holding only the middle rate."""
-    Nf = 1 << 17
+    the middle, a last group holding only the middle rate, and at 2^19 the
+    16-row groups (4 pairs) with partial first groups."""
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.23, 0.75, 30.0), (Nf * 0.66, -1.5, 25.0)], seed=Nr + Nt)
     say(f"fast paired r0={r0} dr={dr} Nr={Nr} Nt={Nt}")
     out = host(F.zdtfdr(F.ZDTWorkspace(s, r0, dr, Nr)))
