@@ -494,6 +494,42 @@ constexpr int k2_waves_per_eu(int n1) { return n1 >= 512 ? 4 : 2; }
 // wavefront-level passes (8 rows for 512 = 8*8*8 over 8 wavefronts, ...)
 constexpr int k2_rows(int n1) { return n1 == 256 ? 16 : 8; }
 
+// The Hermitian pair q of column-pair block p (see k2_czt): bin
+// kA = cA + N2 k1A in row rowA of the block's two rows and its partner
+// M - kA = cB + N2 k1B in row rowB.  Block p > 0 holds columns p and N2-p;
+// block 0 holds the self-paired columns 0 (q < N1/2, pairs k1 and N1-k1, the
+// partner of q = 0 being the Nyquist bin M) and N2/2 (q >= N1/2).
+struct HPair {
+  int rowA, rowB, k1A, k1B, cA, cB;
+  bool nyq;
+};
+
+template <int N1, int N2>
+__device__ __forceinline__ HPair hpair(int p, int q) {
+  HPair h;
+  h.nyq = false;
+  if (p != 0) {
+    h.rowA = 0;
+    h.rowB = 1;
+    h.k1A = q;
+    h.k1B = N1 - 1 - q;
+    h.cA = p;
+    h.cB = N2 - p;
+  } else if (q < N1 / 2) {
+    h.rowA = h.rowB = 0;
+    h.k1A = q;
+    h.k1B = N1 - q;
+    h.nyq = (q == 0);
+    h.cA = h.cB = 0;
+  } else {
+    h.rowA = h.rowB = 1;
+    h.k1A = q - N1 / 2;
+    h.k1B = N1 - 1 - h.k1A;
+    h.cA = h.cB = N2 / 2;
+  }
+  return h;
+}
+
 // Sums of the coefficients c_t at z and conj(z), |z| = 1, by the
 // Reinsch-modified Goertzel recurrence over the real and imaginary parts
 // (see czt_reinsch_col); s, lambda and sin(theta) come from the FP64 z.
@@ -559,28 +595,13 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   const float invNf = 1.0f / static_cast<float>(a.Nf);
 #pragma unroll
   for (int i = 0; i < PP; ++i) {
-    const int q = tid + i * T;
-    int cA;
-    nyq[i] = false;
-    if (!special) {
-      rowA[i] = 0;
-      rowB[i] = 1;
-      k1A[i] = q;
-      k1B[i] = N1 - 1 - q;
-      cA = colA;
-    } else if (q < HALF) {
-      rowA[i] = rowB[i] = 0;
-      k1A[i] = q;
-      k1B[i] = N1 - q;  // q == 0: partner is the Nyquist bin M
-      nyq[i] = (q == 0);
-      cA = 0;
-    } else {
-      rowA[i] = rowB[i] = 1;
-      k1A[i] = q - HALF;
-      k1B[i] = N1 - 1 - k1A[i];
-      cA = N2 / 2;
-    }
-    kA[i] = cA + static_cast<int64_t>(N2) * k1A[i];
+    const HPair h = hpair<N1, N2>(p, tid + i * T);
+    rowA[i] = h.rowA;
+    rowB[i] = h.rowB;
+    k1A[i] = h.k1A;
+    k1B[i] = h.k1B;
+    nyq[i] = h.nyq;
+    kA[i] = h.cA + static_cast<int64_t>(N2) * k1A[i];
     WA[i] = cis_rev(-static_cast<float>(kA[i]) * invNf);  // W_Nf^{kA}
   }
   if (special && tid < NT) ahalf[tid] = {0.0f, 0.0f};  // rows u >= Nt stay zero
@@ -1039,27 +1060,9 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   const bool special = (p == 0);
   const int colA = special ? 0 : p;
   const int colB = special ? N2 / 2 : N2 - p;
-  int rowA, rowB, k1A, k1B, cA, cB;
-  bool nyq = false;
-  if (!special) {
-    rowA = 0;
-    rowB = 1;
-    k1A = tid;
-    k1B = N1 - 1 - tid;
-    cA = colA;
-    cB = colB;
-  } else if (tid < HALF) {
-    rowA = rowB = 0;
-    k1A = tid;
-    k1B = N1 - tid;
-    nyq = (tid == 0);
-    cA = cB = 0;
-  } else {
-    rowA = rowB = 1;
-    k1A = tid - HALF;
-    k1B = N1 - 1 - k1A;
-    cA = cB = N2 / 2;
-  }
+  const HPair hp = hpair<N1, N2>(p, tid);
+  const int rowA = hp.rowA, rowB = hp.rowB, k1A = hp.k1A, k1B = hp.k1B, cA = hp.cA, cB = hp.cB;
+  const bool nyq = hp.nyq;
   const int64_t kA = cA + static_cast<int64_t>(N2) * k1A;
   const c32 WA = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));
   const float h = 0.5f * a.oscale;
@@ -1280,28 +1283,13 @@ __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
   const c32* srcB[PP];
 #pragma unroll
   for (int i = 0; i < PP; ++i) {
-    const int q = tid + i * T;
-    int cA, cB;
-    nyq[i] = false;
-    if (!special) {
-      rowA[i] = 0;
-      rowB[i] = 1;
-      k1A[i] = q;
-      k1B[i] = N1 - 1 - q;
-      cA = colA;
-      cB = colB;
-    } else if (q < HALF) {
-      rowA[i] = rowB[i] = 0;
-      k1A[i] = q;
-      k1B[i] = N1 - q;
-      nyq[i] = (q == 0);
-      cA = cB = 0;
-    } else {
-      rowA[i] = rowB[i] = 1;
-      k1A[i] = q - HALF;
-      k1B[i] = N1 - 1 - k1A[i];
-      cA = cB = N2 / 2;
-    }
+    const HPair h = hpair<N1, N2>(p, tid + i * T);
+    rowA[i] = h.rowA;
+    rowB[i] = h.rowB;
+    k1A[i] = h.k1A;
+    k1B[i] = h.k1B;
+    nyq[i] = h.nyq;
+    const int cA = h.cA, cB = h.cB;
     const int64_t kA = cA + static_cast<int64_t>(N2) * k1A[i];
     WA[i] = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));
     srcA[i] = a.Yc + static_cast<int64_t>(cA) * N1 + k1A[i];
@@ -1416,26 +1404,13 @@ __global__ __launch_bounds__(N1 < 1024 ? N1 : 1024) void k2_input(K2iArgs a) {
 #pragma unroll
   for (int i = 0; i < PP; ++i) {
     const int q = tid + i * T;
-    int cA;
-    nyq[i] = false;
-    if (!special) {
-      rowA[i] = 0;
-      rowB[i] = 1;
-      k1A[i] = q;
-      k1B[i] = N1 - 1 - q;
-      cA = colA;
-    } else if (q < HALF) {
-      rowA[i] = rowB[i] = 0;
-      k1A[i] = q;
-      k1B[i] = N1 - q;  // q == 0: partner is the Nyquist bin M
-      nyq[i] = (q == 0);
-      cA = 0;
-    } else {
-      rowA[i] = rowB[i] = 1;
-      k1A[i] = q - HALF;
-      k1B[i] = N1 - 1 - k1A[i];
-      cA = N2 / 2;
-    }
+    const HPair h = hpair<N1, N2>(p, q);
+    rowA[i] = h.rowA;
+    rowB[i] = h.rowB;
+    k1A[i] = h.k1A;
+    k1B[i] = h.k1B;
+    nyq[i] = h.nyq;
+    const int cA = h.cA;
     const int64_t kA = cA + static_cast<int64_t>(N2) * k1A[i];
     WA[i] = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));  // W_Nf^{kA}
     const int64_t n1 = q;
