@@ -384,14 +384,26 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
     if (r == 0) x[0] = (c & 1) ? csub(x[0], h) : cadd(x[0], h);
     {
       // four-step twiddle W_M^{-n1 k2}, k2 = r + R1 m, as A[m mod 8] B[m / 8]
-      // (n1 k2 < M: every phase is an exact float)
+      // (n1 k2 < M: every phase is an exact float).  Four cis evaluations:
+      // A[0], A[4], the step s = e^{2 pi i n1 R1/M} and B[1]; the rest are
+      // products of at most three of them (a few ulp)
       const unsigned n1 = (nb << 4) + c;
       const float invM = 1.0f / static_cast<float>(N1 * C::N2);
       c32 A[8], Bq[4];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) A[i] = cis_rev(static_cast<float>(n1 * (r + R1 * i)) * invM);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) Bq[q] = cis_rev(static_cast<float>(n1 * (8 * R1 * q)) * invM);
+      A[0] = cis_rev(static_cast<float>(n1 * r) * invM);
+      A[4] = cis_rev(static_cast<float>(n1 * (r + 4 * R1)) * invM);
+      const c32 s1 = cis_rev(static_cast<float>(n1 * R1) * invM);
+      const c32 s2 = cmul(s1, s1), s3 = cmul(s2, s1);
+      A[1] = cmul(A[0], s1);
+      A[2] = cmul(A[0], s2);
+      A[3] = cmul(A[0], s3);
+      A[5] = cmul(A[4], s1);
+      A[6] = cmul(A[4], s2);
+      A[7] = cmul(A[4], s3);
+      Bq[0] = {1.0f, 0.0f};
+      Bq[1] = cis_rev(static_cast<float>(n1 * (8 * R1)) * invM);
+      Bq[2] = cmul(Bq[1], Bq[1]);
+      Bq[3] = cmul(Bq[2], Bq[1]);
 #pragma unroll
       for (int m = 1; m < 32; ++m) x[m] = cmul(x[m], (m < 8) ? A[m] : cmul(A[m & 7], Bq[m >> 3]));
       x[0] = cmul(x[0], A[0]);
