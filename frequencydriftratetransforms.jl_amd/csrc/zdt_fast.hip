@@ -1265,9 +1265,10 @@ __device__ __forceinline__ void wave_ifft_2048(c32* x, const c32* tw, int lane, 
 // k2_c2r for N1 = 1024, 2048 (Nf = 2^21, 2^22): 512 threads, PP = N1/512
 // pairs each, groups of 4 rates (8 rows, one per wavefront) in one LDS
 // buffer; each wavefront runs the inverse DFT_N1 of its row in place
-// (wave_fft_row) and stores it, so a group costs two barriers.  For N1 = 1024
-// the next group's spectra are loaded into registers before the DFTs; at
-// N1 = 2048 the 32 row entries per lane leave no registers for that.
+// (wave_fft_row) and stores it, so a group costs two barriers.  The next
+// group's spectra are loaded into registers before the DFTs (at N1 = 2048
+// this spills a few registers outside the DFTs, and is still 7% faster than
+// loading after the barrier).
 template <int N1, int N2>
 __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
   constexpr int T = 512;
@@ -1291,8 +1292,7 @@ __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
   int rowA[PP], rowB[PP], k1A[PP], k1B[PP];
   bool nyq[PP];
   c32 WA[PP];
-  const c32* srcA[PP];
-  const c32* srcB[PP];
+  int offA[PP], offB[PP];  // within a rate's M spectra
 #pragma unroll
   for (int i = 0; i < PP; ++i) {
     const HPair h = hpair<N1, N2>(p, tid + i * T);
@@ -1304,8 +1304,8 @@ __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
     const int cA = h.cA, cB = h.cB;
     const int64_t kA = cA + static_cast<int64_t>(N2) * k1A[i];
     WA[i] = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));
-    srcA[i] = a.Yc + static_cast<int64_t>(cA) * N1 + k1A[i];
-    srcB[i] = a.Yc + static_cast<int64_t>(cB) * N1 + (nyq[i] ? 0 : k1B[i]);
+    offA[i] = cA * N1 + k1A[i];
+    offB[i] = cB * N1 + (nyq[i] ? 0 : k1B[i]);
   }
   const float h = 0.5f * a.oscale;
   if (special) {
@@ -1320,21 +1320,19 @@ __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
 #pragma unroll
     for (int jj = 0; jj < JG; ++jj) {
       if (jj < njn) {
-        const int64_t off = static_cast<int64_t>(j0n + jj) * a.M;
+        const c32* y = a.Yc + static_cast<int64_t>(j0n + jj) * a.M;  // uniform
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
-          ya[jj][i] = srcA[i][off];
-          yb[jj][i] = nyq[i] ? a.yM[j0n + jj] : srcB[i][off];
+          ya[jj][i] = y[offA[i]];
+          yb[jj][i] = nyq[i] ? a.yM[j0n + jj] : y[offB[i]];
         }
       }
     }
   };
-  constexpr bool PF = PP <= 2;
-  if (PF) load(0);
+  load(0);
   const int wv = tid >> 6, lane = tid & 63;
   for (int j0 = 0; j0 < a.Nr; j0 += JG) {
     const int nj = min(JG, a.Nr - j0);
-    if (!PF) load(j0);
 #pragma unroll
     for (int jj = 0; jj < JG; ++jj) {
       if (jj < nj) {
@@ -1355,7 +1353,7 @@ __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
       }
     }
     __syncthreads();
-    if (PF && j0 + JG < a.Nr) load(j0 + JG);
+    if (j0 + JG < a.Nr) load(j0 + JG);
     if (wv < 2 * nj) {  // uniform per wavefront: wavefront wv owns row wv
       c32* dst = a.B + static_cast<int64_t>(j0 + (wv >> 1)) * (N1 * N2) + ((wv & 1) ? colB : colA) * 16;
       auto emit = [&](int jx, auto& v) {
