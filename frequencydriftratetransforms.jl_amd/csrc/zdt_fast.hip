@@ -24,236 +24,23 @@
 // by Horner's rule in FP32 at z = gamma_k^j, with z carried by an FP64
 // recurrence.  A thread owns a bin k and its Hermitian partner M-k, for which
 // gamma_{M-k}^j = e^{i pi dr j} conj(gamma_k^j), so the partner's powers cost
-// one multiply by a per-rate table.
+// one multiply by a per-rate table.  Symmetric rate sets take the paired
+// evaluation of k2_czt<..., PAIRED> (reinsch2, zdt_fast_dev.h).
+//
+// The device building blocks live in zdt_fast_dev.h, the plan in
+// zdt_fast_plan.h, and the fused output (Nt up to 64, Nf up to 2^22), which
+// shares k1 and k3, in zdt_fout.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
-#include "fft_butterflies.h"
 #include "fft_generic.h"
-#include "zdt_fast.h"
+#include "zdt_fast_dev.h"
+#include "zdt_fast_plan.h"
 
 namespace zdt {
-
-// strided FFT length N2 of k1/k3: 1024 (32 x 32) when that leaves N1 = M/N2 >= 64,
-// else 512 (32 x 16)
-constexpr int fast_n2(int64_t Nf) { return Nf >= (int64_t(1) << 17) ? 1024 : 512; }
-
-// ---------------------------------------------------------------- LDS FFT
-
-constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
-
-// radix of the pass at stride NS in the LDS FFTs: 1024 = 16*8*8,
-// 512 = 8*8*8, 256 = 16*16, 128 = 16*8, 64 = 8*8
-template <int N, int NS>
-constexpr int fft_radix() {
-  return (N / NS == 16 || (NS == 1 && ilog2c(N) % 3 != 0)) ? 16 : (N / NS >= 8 ? 8 : N / NS);
-}
-
-// LDS row swizzle: one pad entry after every 8, so the strided Stockham
-// writes of the early passes spread over the banks.  Rows of length N occupy
-// N + N/8 entries.
-__device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
-
-// Ping-pong variant: radix-R butterflies stream from buffer `in` to buffer
-// `out` (one butterfly live per thread), one barrier per pass.  Rows are
-// swizzled (lsw); since NR and Ns are multiples of 8 or the write base is,
-// every butterfly needs the swizzle only once for its reads and once for its
-// writes.  Twiddles come from the LDS table tw (W_N^m, m < N); INV selects the
-// inverse butterflies and conjugate twiddles.
-template <int N, int R, int T, bool INV>
-__device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int nrows, int Ns,
-                                            const c32* __restrict__ tw) {
-  constexpr int NR = N / R;
-  constexpr int LOGNR = ilog2c(NR);
-  constexpr int LOGN = ilog2c(N);
-  constexpr int LOGR = ilog2c(R);
-  static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
-  const int logNs = 31 - __builtin_clz(Ns);
-  const int total = nrows * NR;
-  const int sh = LOGN - logNs - LOGR;
-  for (int b = threadIdx.x; b < total; b += T) {
-    const int f = b >> LOGNR;
-    const int j = b & (NR - 1);
-    int k = j & (Ns - 1);
-    // Opaque to the optimizer: keeps the R-1 twiddle addresses from being
-    // hoisted out of the caller's rate loop, where they would be spilled.
-    asm volatile("" : "+v"(k));
-    const c32* x = in + f * P + lsw(j);
-    c32 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = x[r * (NR + NR / 8)];
-    if (Ns > 1) {
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const c32 w = tw[(r * k) << sh];
-        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
-      }
-    }
-    dftd<INV, float, R>(v);
-    const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
-    c32* y = out + f * P;
-    if (Ns == 1) {
-      // d0 = j*R is a multiple of 8 (R >= 8): lsw(d0 + r) = lsw(d0) + r + r/8
-      c32* yy = y + lsw(d0);
-#pragma unroll
-      for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
-    } else {
-      // Ns >= 8: the stride Ns keeps the swizzle offset linear
-      c32* yy = y + lsw(d0);
-#pragma unroll
-      for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
-    }
-  }
-  __syncthreads();
-}
-
-// radix schedule: 1024 = 16*8*8, 512 = 8*8*8, 256 = 16*16, 128 = 16*8,
-// 64 = 8*8; returns the buffer holding the result
-template <int N, int T, bool INV, int NS = 1>
-__device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, const c32* __restrict__ tw) {
-  if constexpr (NS < N) {
-    constexpr int R = fft_radix<N, NS>();
-    lds_pass_pp<N, R, T, INV>(a, b, P, nrows, NS, tw);
-    return lds_fft_pp<N, T, INV, NS * R>(b, a, P, nrows, tw);
-  } else {
-    return a;
-  }
-}
-
-// Wavefront-level passes.  The rows [0, nrw) at `rows` (pitch P, swizzled)
-// belong to one wavefront, and lane b < nrw*N/R owns butterfly b of each
-// pass.  A wavefront issues all its reads of a pass before any of its
-// writes and its LDS accesses complete in order, so the passes run in place
-// with no workgroup barrier; wave_sync only keeps the compiler from moving
-// LDS accesses across pass boundaries.  Requires nrw*N/R <= 64 for every
-// radix R of the schedule.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <int N, int R, bool INV>
-__device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, const c32* __restrict__ tw, int lane,
-                                          c32 (&v)[R], bool last) {
-  constexpr int NR = N / R;
-  constexpr int LOGNR = ilog2c(NR);
-  constexpr int LOGN = ilog2c(N);
-  constexpr int LOGR = ilog2c(R);
-  static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
-  if (lane < nrw * NR) {
-    const int logNs = 31 - __builtin_clz(Ns);
-    const int f = lane >> LOGNR;
-    const int j = lane & (NR - 1);
-    int k = j & (Ns - 1);
-    asm volatile("" : "+v"(k));  // keeps twiddle addresses out of the caller's loop
-    c32* x = rows + f * P;
-    const c32* xr = x + lsw(j);
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = xr[r * (NR + NR / 8)];
-    if (Ns > 1) {
-      const int sh = LOGN - logNs - LOGR;
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const c32 w = tw[(r * k) << sh];
-        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
-      }
-    }
-    dftd<INV, float, R>(v);
-    if (!last) {
-      const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
-      c32* yy = x + lsw(d0);
-      if (Ns == 1) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
-      } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
-      }
-    }
-  }
-  wave_sync();
-}
-
-template <int N, int NS = 1>
-constexpr int fft_max_nr() {
-  if constexpr (NS >= N) {
-    return 0;
-  } else {
-    constexpr int R = fft_radix<N, NS>();
-    constexpr int a = N / R, b = fft_max_nr<N, NS * R>();
-    return a > b ? a : b;
-  }
-}
-
-// In-place inverse/forward FFT of the wavefront's rows; the last pass leaves
-// its butterfly outputs in registers (lane (f, j): row f, elements
-// j + r*N/R) and calls emit(f, j, v) instead of writing them back.
-template <int N, bool INV, int NS = 1, typename Emit>
-__device__ __forceinline__ void wave_fft(c32* rows, int P, int nrw, const c32* __restrict__ tw, int lane, Emit&& emit) {
-  constexpr int R = fft_radix<N, NS>();
-  c32 v[R];
-  if constexpr (NS * R < N) {
-    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, false);
-    wave_fft<N, INV, NS * R>(rows, P, nrw, tw, lane, emit);
-  } else {
-    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, true);
-    constexpr int NR = N / R;
-    if (lane < nrw * NR) emit(lane / NR, lane % NR, v);
-  }
-}
-
-// ---------------------------------------------------------------- column FFT (N2 = 32*R1)
-//
-// k1 and k3 transform 16 adjacent columns (n1 or k1 fixed, 128 contiguous
-// bytes per row) along a strided axis of length N2 = 32*R1 (R1 = 16 or 32),
-// register resident: thread (c = tid&15, r = tid>>4) holds rows r + R1*m
-// (m < 32) of column c, does a DFT-32 over m, multiplies by W_N2^{r m'}, and
-// after one LDS exchange the thread (c, g) does the DFT-R1 over r for
-// m' = g + R1*h, giving output rows m' + 32 r'.  Inverse transforms conjugate
-// on entry and exit.  Blocks are persistent over tiles.
-
-template <int R1>
-struct CF {
-  static constexpr int N2 = 32 * R1;
-  static constexpr int THREADS = 16 * R1;
-  static constexpr int PITCH = R1 * 16 + 8;  // complex per m' slab ([r][c] + pad)
-  static constexpr int LDS = 32 * PITCH;
-  static constexpr int MPER = 32 / R1;       // m' values per thread in stage 2
-};
-
-template <int R1>
-__device__ __forceinline__ void cf_stage1(c32 (&x)[32], int r, const c32* twl, c32* ex, int c, bool inv) {
-  if (inv) {
-#pragma unroll
-    for (int m = 0; m < 32; ++m) x[m].y = -x[m].y;
-  }
-  dft32(x);
-#pragma unroll
-  for (int m = 1; m < 32; ++m) x[m] = cmul(x[m], twl[r * m]);  // W_N2^{r m'}
-#pragma unroll
-  for (int m = 0; m < 32; ++m) ex[m * CF<R1>::PITCH + r * 16 + c] = x[m];
-}
-
-// stage 2 for one m': y[r'] is output row m' + 32 r'
-template <int R1>
-__device__ __forceinline__ void cf_stage2(c32 (&y)[R1], const c32* ex, int c, int mp, bool inv) {
-#pragma unroll
-  for (int r = 0; r < R1; ++r) y[r] = ex[mp * CF<R1>::PITCH + r * 16 + c];
-  dft<float, R1>(y);
-  if (inv) {
-#pragma unroll
-    for (int r = 0; r < R1; ++r) y[r].y = -y[r].y;
-  }
-}
-
-template <int N>
-__device__ __forceinline__ void cf_load_twiddles(c32* twl, const c32* __restrict__ tw, int nthreads) {
-  for (int i = threadIdx.x; i < N; i += nthreads) twl[i] = tw[i];
-}
 
 // ---------------------------------------------------------------- k1: input pass A
 
@@ -473,98 +260,6 @@ struct K2Args {
   const float* wcustom;
   float oscale;      // output scale / M  (c2r normalization)
 };
-
-__device__ __forceinline__ float fast_window(int wkind, const float* wcustom, int64_t k, int64_t Nf) {
-  if (wkind == ZDT_WINDOW_HAMMING) {
-    const float p = static_cast<float>(static_cast<double>(k) / static_cast<double>(Nf));
-    return 0.53836f + 0.46164f * cis_rev(p).x;
-  }
-  if (wkind == ZDT_WINDOW_CUSTOM) return wcustom[k];
-  return 1.0f;
-}
-
-__device__ __forceinline__ c64 cis64f(double rev) {
-  const double f = rev - __builtin_rint(rev);
-  double s, c;
-  sincospi(2.0 * f, &s, &c);
-  return {c, s};
-}
-
-// Horner: sum_u a[u] z^u
-template <int NT>
-__device__ __forceinline__ c32 horner(const c32* a, c32 z) {
-  c32 acc = a[NT - 1];
-#pragma unroll
-  for (int u = NT - 2; u >= 0; --u) acc = cfma(a[u], acc, z);
-  return acc;
-}
-
-// k2 occupancy target: two 512-thread blocks per CU (4 waves per SIMD) for
-// N1 = 512, whose LDS (2 x 8 rows x 576 + 512 entries, 78 KB) allows it.
-constexpr int k2_waves_per_eu(int n1) { return n1 >= 512 ? 4 : 2; }
-// rows per group: as many as keep one butterfly per lane in the
-// wavefront-level passes (8 rows for 512 = 8*8*8 over 8 wavefronts, ...)
-constexpr int k2_rows(int n1) { return n1 == 256 ? 16 : 8; }
-
-// The Hermitian pair q of column-pair block p (see k2_czt): bin
-// kA = cA + N2 k1A in row rowA of the block's two rows and its partner
-// M - kA = cB + N2 k1B in row rowB.  Block p > 0 holds columns p and N2-p;
-// block 0 holds the self-paired columns 0 (q < N1/2, pairs k1 and N1-k1, the
-// partner of q = 0 being the Nyquist bin M) and N2/2 (q >= N1/2).
-struct HPair {
-  int rowA, rowB, k1A, k1B, cA, cB;
-  bool nyq;
-};
-
-template <int N1, int N2>
-__device__ __forceinline__ HPair hpair(int p, int q) {
-  HPair h;
-  h.nyq = false;
-  if (p != 0) {
-    h.rowA = 0;
-    h.rowB = 1;
-    h.k1A = q;
-    h.k1B = N1 - 1 - q;
-    h.cA = p;
-    h.cB = N2 - p;
-  } else if (q < N1 / 2) {
-    h.rowA = h.rowB = 0;
-    h.k1A = q;
-    h.k1B = N1 - q;
-    h.nyq = (q == 0);
-    h.cA = h.cB = 0;
-  } else {
-    h.rowA = h.rowB = 1;
-    h.k1A = q - N1 / 2;
-    h.k1B = N1 - 1 - h.k1A;
-    h.cA = h.cB = N2 / 2;
-  }
-  return h;
-}
-
-// Sums of the coefficients c_t at z and conj(z), |z| = 1, by the
-// Reinsch-modified Goertzel recurrence over the real and imaginary parts
-// (see czt_reinsch_col); s, lambda and sin(theta) come from the FP64 z.
-template <int NT>
-__device__ __forceinline__ void reinsch2(const c32 (&c)[NT], c64 z, c32& Yz, c32& Yzc) {
-  const float s = z.x >= 0.0 ? 1.0f : -1.0f;
-  const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
-  const float sn = static_cast<float>(z.y);
-  float Dr = c[NT - 1].x, Ur = c[NT - 1].x, Di = c[NT - 1].y, Ui = c[NT - 1].y;
-#pragma unroll
-  for (int t = NT - 2; t >= 1; --t) {
-    Dr = fmaf(s, Dr, fmaf(lam, Ur, c[t].x));
-    Ur = fmaf(s, Ur, Dr);
-    Di = fmaf(s, Di, fmaf(lam, Ui, c[t].y));
-    Ui = fmaf(s, Ui, Di);
-  }
-  const float hl = 0.5f * lam;
-  const float Cr = fmaf(s, Dr, fmaf(hl, Ur, c[0].x));
-  const float Ci = fmaf(s, Di, fmaf(hl, Ui, c[0].y));
-  const float Sr = sn * Ur, Si = sn * Ui;
-  Yz = {Cr - Si, Sr + Ci};
-  Yzc = {Cr + Si, Ci - Sr};
-}
 
 // Block of T = min(N1, 512) threads; each thread owns PP = N1/T Hermitian
 // pairs q = tid + i*T (i < PP).  Pair q of a regular block (columns c, N2-c)
@@ -882,612 +577,6 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   }
 }
 
-// ---------------------------------------------------------------- fused output for Nt > 16
-//
-// For power-of-two Nf with Nt up to 64 (more coefficients than k2 keeps in
-// registers) the chirp-Z runs per bin (czt_horner_col, the Vandermonde form
-// of czt_horner) and the output irfft reuses the second half of k2 and k3:
-//   czt_horner_col  Ys[k, j] -> Yc[j][k2][k1] (k = k2 + N2 k1 < M), Ys[M, j] -> yM[j]
-//   k2_c2r          per Hermitian column pair: c2r pre-pass, inverse DFT_N1
-//                   over k1 (one wavefront per row) -> B, bin M/2 -> yhalf
-//   k3_output       as in the fused path
-// which moves the spectra through HBM twice instead of once per FFT pass.
-
-struct CztColArgs {
-  const c32* F;  // Nk x Nt, natural layout (column stride ldF), or with ftrans
-                 // [Nt][N2][N1] (bin k2 + N2 k1 at [t][k2][k1]) then [Nt] bin M
-  int64_t ldF;
-  int ftrans;
-  c32* Yc;       // [Nr][N2][N1]
-  c32* yM;       // [Nr]
-  int64_t M, Nf;
-  int logN1, logN2;
-  int Nt, Nr;
-  double hd2;    // dr/Nf
-  double h0;     // r0/Nf
-  int wkind;
-  const float* wcustom;
-  float scale;
-  int pair;      // czt_reinsch_col: rates pair as r_j = -r_{pair-j}
-};
-
-// The bin of a thread: one thread per bin; a block covers 16 columns k2 x 16
-// rows k1 (so the Nr stores of a wavefront are four 128-byte rows of Yc); the
-// last block's thread 0 takes the Nyquist bin M.  Returns false for the idle
-// threads of the last block.
-struct ColBin {
-  int64_t k;
-  c32* dst;
-  int64_t stride;  // between rates in dst
-  int64_t fo, fs;  // F offset of the bin and its step in t
-};
-
-__device__ __forceinline__ bool col_bin(const CztColArgs& a, ColBin& b) {
-  const int ntk1 = (1 << a.logN1) >> 4;
-  const int tile = blockIdx.x;
-  if (tile * 256 >= a.M) {
-    if (threadIdx.x != 0) return false;
-    b.k = a.M;
-    b.dst = a.yM;
-    b.stride = 1;
-    b.fo = a.ftrans ? a.M * a.Nt : a.M;
-    b.fs = a.ftrans ? 1 : a.ldF;
-  } else {
-    const int k2 = ((tile / ntk1) << 4) + (threadIdx.x >> 4);
-    const int k1 = ((tile % ntk1) << 4) + (threadIdx.x & 15);
-    b.k = k2 + (static_cast<int64_t>(k1) << a.logN2);
-    b.dst = a.Yc + (static_cast<int64_t>(k2) << a.logN1) + k1;
-    b.stride = a.M;
-    b.fo = a.ftrans ? (static_cast<int64_t>(k2) << a.logN1) + k1 : b.k;
-    b.fs = a.ftrans ? a.M : a.ldF;
-  }
-  return true;
-}
-
-template <int NT>
-__global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
-  constexpr int B = NT < 16 ? NT : 16;
-  constexpr int NB = NT / B;
-  ColBin bin;
-  if (!col_bin(a, bin)) return;
-  const int64_t k = bin.k, fo = bin.fo, fs = bin.fs, stride = bin.stride;
-  c32* dst = bin.dst;
-  const double kd = static_cast<double>(k);
-  const float w = fast_window(a.wkind, a.wcustom, k, a.Nf) * a.scale;
-  c32 c[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    if (t < a.Nt) {
-      const c32 f = a.F[fo + static_cast<int64_t>(t) * fs];
-      const c32 r = cmul(f, cis_nh(kd * static_cast<double>(t), a.h0));
-      c[t] = {r.x * w, r.y * w};
-    } else {
-      c[t] = {0.0f, 0.0f};
-    }
-  }
-  const c64 g = cis64f(kd * a.hd2);
-  const c64 gB = cis64f(static_cast<double>(B) * kd * a.hd2);
-  c64 z = {1.0, 0.0}, zB = {1.0, 0.0};
-  for (int j = 0; j < a.Nr; ++j) {
-    const c32 zf = {static_cast<float>(z.x), static_cast<float>(z.y)};
-    const c32 zBf = {static_cast<float>(zB.x), static_cast<float>(zB.y)};
-    c32 acc = {0.0f, 0.0f};
-#pragma unroll
-    for (int b = NB - 1; b >= 0; --b) {
-      c32 q = c[b * B + B - 1];
-#pragma unroll
-      for (int u = B - 2; u >= 0; --u) q = cfma(c[b * B + u], q, zf);
-      acc = (b == NB - 1) ? q : cfma(q, acc, zBf);
-    }
-    dst[static_cast<int64_t>(j) * stride] = acc;
-    z = cmul(z, g);
-    zB = cmul(zB, gB);
-  }
-}
-
-// Rates in mirrored pairs r_j = -r_{P-j} (P = pair): the bin's sums at
-// z = e^{i theta} and conj(z), theta = 2 pi k r_j/Nf, come from one
-// Reinsch-modified Goertzel recurrence (C. Reinsch, 1967; J. Oliver, "An
-// error analysis of the modified Clenshaw method", 1977) over the real and
-// the imaginary parts of the coefficients.  With s = sign(cos theta) and
-// lambda = 2 (cos theta - s),
-//   D_t = a_t + lambda U_{t+1} + s D_{t+1},  U_t = D_t + s U_{t+1}  (t = Nt-1 .. 1),
-//   sum a_t cos(t theta) = a_0 + (lambda/2) U_1 + s D_1,  sum a_t sin(t theta) = U_1 sin theta:
-// three FMAs per term and part, six per term for both rates against eight
-// for two complex Horner evaluations.  lambda comes from the FP64 cos theta,
-// so no accuracy is lost near theta = 0 or pi (where Goertzel's rounded
-// 2 cos theta would lose it).  Rates without a partner are evaluated the same
-// way and only the sum at z is stored.
-template <int NT>
-__global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
-  ColBin bin;
-  if (!col_bin(a, bin)) return;
-  const double kd = static_cast<double>(bin.k);
-  const float w = fast_window(a.wkind, a.wcustom, bin.k, a.Nf) * a.scale;
-  float cr[NT], ci[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    if (t < a.Nt) {
-      const c32 f = a.F[bin.fo + static_cast<int64_t>(t) * bin.fs];
-      cr[t] = f.x * w;
-      ci[t] = f.y * w;
-    } else {
-      cr[t] = 0.0f;
-      ci[t] = 0.0f;
-    }
-  }
-  const c64 g = cis64f(kd * a.hd2);
-  c64 z = cis64f(kd * a.h0);  // e^{2 pi i k r_0/Nf}
-  for (int j = 0; j < a.Nr; ++j) {
-    const int jp = a.pair - j;
-    const bool mirror = jp >= 0 && jp < a.Nr;
-    if (!mirror || jp >= j) {  // uniform: the partner's sum is stored with j's
-      const float s = z.x >= 0.0 ? 1.0f : -1.0f;
-      const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
-      const float sn = static_cast<float>(z.y);
-      float Dr = cr[NT - 1], Ur = cr[NT - 1], Di = ci[NT - 1], Ui = ci[NT - 1];
-#pragma unroll
-      for (int t = NT - 2; t >= 1; --t) {
-        Dr = fmaf(s, Dr, fmaf(lam, Ur, cr[t]));
-        Ur = fmaf(s, Ur, Dr);
-        Di = fmaf(s, Di, fmaf(lam, Ui, ci[t]));
-        Ui = fmaf(s, Ui, Di);
-      }
-      const float hl = 0.5f * lam;
-      const float Cr = fmaf(s, Dr, fmaf(hl, Ur, cr[0]));
-      const float Ci = fmaf(s, Di, fmaf(hl, Ui, ci[0]));
-      const float Sr = sn * Ur, Si = sn * Ui;
-      bin.dst[static_cast<int64_t>(j) * bin.stride] = {Cr - Si, Sr + Ci};
-      if (mirror && jp > j) bin.dst[static_cast<int64_t>(jp) * bin.stride] = {Cr + Si, Ci - Sr};
-    }
-    z = cmul(z, g);
-  }
-}
-
-struct K2cArgs {
-  const c32* Yc;  // [Nr][N2][N1]
-  const c32* yM;  // [Nr]
-  c32* B;         // [Nr][N1/16][N2][16]
-  c32* yhalf;     // [Nr]
-  const c32* tw1;
-  int Nr;
-  int64_t M, Nf;
-  float oscale;   // output scale / M
-};
-
-// the rate loop of k2_czt with the spectra read from Yc instead of evaluated
-template <int N1, int N2, int ROWS>
-__global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c2r(K2cArgs a) {
-  constexpr int T = N1 < 512 ? N1 : 512;
-  static_assert(T == N1, "one Hermitian pair per thread");
-  constexpr int P = N1 + N1 / 8;
-  constexpr int HALF = N1 / 2;
-  __shared__ c32 buf[ROWS * P];
-  __shared__ c32 buf2[ROWS * P];
-  __shared__ c32 twl[N1];
-  for (int i = threadIdx.x; i < N1; i += T) twl[i] = a.tw1[i];
-  __syncthreads();
-  const int tid = threadIdx.x;
-  const int p = blockIdx.x;
-  const bool special = (p == 0);
-  const int colA = special ? 0 : p;
-  const int colB = special ? N2 / 2 : N2 - p;
-  const HPair hp = hpair<N1, N2>(p, tid);
-  const int rowA = hp.rowA, rowB = hp.rowB, k1A = hp.k1A, k1B = hp.k1B, cA = hp.cA, cB = hp.cB;
-  const bool nyq = hp.nyq;
-  const int64_t kA = cA + static_cast<int64_t>(N2) * k1A;
-  const c32 WA = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));
-  const float h = 0.5f * a.oscale;
-  if (special) {
-    for (int j = tid; j < a.Nr; j += T) {
-      const c32 y = a.Yc[static_cast<int64_t>(j) * a.M + HALF];  // bin M/2: k2 = 0, k1 = N1/2
-      a.yhalf[j] = {y.x * a.oscale, -y.y * a.oscale};
-    }
-  }
-  const c32* srcA = a.Yc + static_cast<int64_t>(cA) * N1 + k1A;
-  const c32* srcB = a.Yc + static_cast<int64_t>(cB) * N1 + (nyq ? 0 : k1B);
-  constexpr int JG = ROWS / 2;
-  constexpr int NW = T / 64;
-  constexpr int RPW = ROWS / NW;
-  static_assert(RPW * NW == ROWS && RPW * fft_max_nr<N1>() <= 64, "one butterfly per lane and pass");
-  const int wv = tid >> 6, lane = tid & 63;
-  c32* cur = buf;
-  c32* nxt = buf2;
-  // the next group's spectra are loaded right after this group's barrier, so
-  // their latency overlaps the inverse DFT
-  c32 ya[JG], yb[JG];
-  auto load = [&](int j0n) {
-    const int njn = min(JG, a.Nr - j0n);
-#pragma unroll
-    for (int jj = 0; jj < JG; ++jj) {
-      if (jj < njn) {
-        const int64_t off = static_cast<int64_t>(j0n + jj) * a.M;
-        ya[jj] = srcA[off];
-        yb[jj] = nyq ? a.yM[j0n + jj] : srcB[off];
-      }
-    }
-  };
-  load(0);
-  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
-    const int nj = min(JG, a.Nr - j0);
-#pragma unroll
-    for (int jj = 0; jj < JG; ++jj) {
-      if (jj < nj) {
-        const c32 YA = {ya[jj].x * h, ya[jj].y * h}, YB = {yb[jj].x * h, yb[jj].y * h};
-        if (nyq) {
-          cur[(2 * jj + rowA) * P] = {YA.x + YB.x, YA.x - YB.x};
-          cur[(2 * jj + rowA) * P + lsw(HALF)] = {0.0f, 0.0f};
-        } else {
-          const c32 E = {YA.x + YB.x, YA.y - YB.y};
-          const c32 D = {YA.x - YB.x, YA.y + YB.y};
-          const c32 O = cmulc(D, WA);
-          cur[(2 * jj + rowA) * P + lsw(k1A)] = {E.x - O.y, E.y + O.x};
-          cur[(2 * jj + rowB) * P + lsw(k1B)] = {E.x + O.y, -E.y + O.x};
-        }
-      }
-    }
-    __syncthreads();
-    if (j0 + JG < a.Nr) load(j0 + JG);
-    const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));
-    wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
-      constexpr int RL = sizeof(v) / sizeof(c32);
-      const int row = wv * RPW + f;
-      c32* dst = a.B + static_cast<int64_t>(j0 + (row >> 1)) * (N1 * N2) + ((row & 1) ? colB : colA) * 16;
-#pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const int n1 = jx + r * (N1 / RL);
-        dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
-      }
-    });
-    c32* t = cur;
-    cur = nxt;
-    nxt = t;
-  }
-}
-
-
-// In-place inverse/forward FFT of one row of N entries owned by one
-// wavefront, N/(64 R) radix-R butterflies per lane and pass (rows of 2^10 and
-// 2^11 entries, too long for wave_fft's one butterfly per lane).  A pass
-// issues all of its reads before any of its writes, as in wave_pass; the last
-// pass calls emit(j, v) for each butterfly j of the lane (elements
-// j + r N/R) instead of writing back.  Twiddles: the table tw of W_N^m.
-template <int N, bool INV, int NS = 1, typename Emit>
-__device__ __forceinline__ void wave_fft_row(c32* x, const c32* __restrict__ tw, int lane, Emit&& emit) {
-  constexpr int R = fft_radix<N, NS>();
-  constexpr int NR = N / R;
-  constexpr int BPL = NR / 64;
-  constexpr int LOGN = ilog2c(N), LOGR = ilog2c(R), LOGNS = ilog2c(NS);
-  static_assert(BPL * 64 == NR && NR % 8 == 0, "whole butterflies per lane");
-  c32 v[BPL][R];
-#pragma unroll
-  for (int b = 0; b < BPL; ++b) {
-    const c32* xr = x + lsw(lane + 64 * b);
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = xr[r * (NR + NR / 8)];
-  }
-#pragma unroll
-  for (int b = 0; b < BPL; ++b) {
-    if constexpr (NS > 1) {
-      const int k = (lane + 64 * b) & (NS - 1);
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const c32 w = tw[(r * k) << (LOGN - LOGNS - LOGR)];
-        v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
-      }
-    }
-    dftd<INV, float, R>(v[b]);
-  }
-  if constexpr (NS * R < N) {
-#pragma unroll
-    for (int b = 0; b < BPL; ++b) {
-      const int j = lane + 64 * b;
-      const int k = j & (NS - 1);
-      c32* yy = x + lsw(((j >> LOGNS) << (LOGNS + LOGR)) + k);
-      if constexpr (NS == 1) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[b][r];
-      } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) yy[r * (NS + (NS >> 3))] = v[b][r];
-      }
-    }
-    wave_sync();
-    wave_fft_row<N, INV, NS * R>(x, tw, lane, emit);
-  } else {
-#pragma unroll
-    for (int b = 0; b < BPL; ++b) emit(lane + 64 * b, v[b]);
-  }
-}
-
-// Inverse DFT of one 2048-entry row owned by one wavefront, in place and
-// with 32 row entries per lane but at most 16 live at a time (a Stockham row
-// FFT keeps all 32 live, which does not fit beside k2_c2r's state).  Stage 1
-// is a radix-16 Stockham pass from natural order (butterfly j reads entries
-// j + 128 r) whose outputs go to the digit-reversed positions of an in-place
-// decimation-in-time FFT (group g = j/16 + 8 (j mod 16), entries 16 g + r');
-// stages 2 (radix 8, span 16) and 3 (radix 16, span 128) are in-place DIT
-// butterflies on entries g L + k + r L', one butterfly at a time, and stage 3
-// calls emit(k, v) with v[r] = output k + 128 r.  Rows are stored with the
-// swizzle s2048 (bits 7-10 folded into bits 0-3), which keeps every stage's
-// accesses within two-way bank conflicts; tw is the LDS table of W_2048^m.
-__device__ __forceinline__ int s2048(int i) { return i ^ ((i >> 7) & 15); }
-
-template <typename Emit>
-__device__ __forceinline__ void wave_ifft_2048(c32* x, const c32* tw, int lane, Emit&& emit) {
-  // opaque to the optimizer: keeps the ~100 lane-dependent LDS addresses from
-  // being hoisted out of the caller's rate loop (where they would spill)
-  asm volatile("" : "+v"(lane));
-  {
-    c32 v[2][16];
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) v[b][r] = x[s2048(lane + 64 * b + 128 * r)];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      dftd<true, float, 16>(v[b]);
-      const int j = lane + 64 * b;
-      const int g = (j >> 4) + 8 * (j & 15);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) x[s2048(16 * g + r)] = v[b][r];
-    }
-  }
-  wave_sync();
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int bb = lane + 64 * b;
-    const int base = (bb >> 4) * 128 + (bb & 15);
-    const int k = bb & 15;
-    c32 v[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = x[s2048(base + 16 * r)];
-#pragma unroll
-    for (int r = 1; r < 8; ++r) v[r] = cmulc(v[r], tw[16 * r * k]);  // W_128^{-r k}
-    dftd<true, float, 8>(v);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) x[s2048(base + 16 * r)] = v[r];
-  }
-  wave_sync();
-#pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const int k = lane + 64 * b;
-    c32 v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = x[s2048(k + 128 * r)];
-#pragma unroll
-    for (int r = 1; r < 16; ++r) v[r] = cmulc(v[r], tw[r * k]);  // W_2048^{-r k}
-    dftd<true, float, 16>(v);
-    emit(k, v);
-  }
-}
-
-// k2_c2r for N1 = 1024, 2048 (Nf = 2^21, 2^22): 512 threads, PP = N1/512
-// pairs each, groups of 4 rates (8 rows, one per wavefront) in one LDS
-// buffer; each wavefront runs the inverse DFT_N1 of its row in place
-// (wave_fft_row) and stores it, so a group costs two barriers.  The next
-// group's spectra are loaded into registers before the DFTs (at N1 = 2048
-// this spills a few registers outside the DFTs, and is still 7% faster than
-// loading after the barrier).
-template <int N1, int N2>
-__global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
-  constexpr int T = 512;
-  constexpr int PP = N1 / T;
-  constexpr bool BIG = N1 == 2048;  // wave_ifft_2048 layout (no row padding)
-  constexpr int P = BIG ? N1 : N1 + N1 / 8;
-  constexpr int HALF = N1 / 2;
-  constexpr int ROWS = 8;
-  constexpr int JG = ROWS / 2;
-  __shared__ c32 buf[ROWS * P];
-  __shared__ c32 twl[BIG ? N1 : 1];
-  if constexpr (BIG) {
-    for (int i = threadIdx.x; i < N1; i += T) twl[i] = a.tw1[i];
-  }
-  auto pos = [](int i) { return BIG ? s2048(i) : lsw(i); };
-  const int tid = threadIdx.x;
-  const int p = blockIdx.x;
-  const bool special = (p == 0);
-  const int colA = special ? 0 : p;
-  const int colB = special ? N2 / 2 : N2 - p;
-  int rowA[PP], rowB[PP], k1A[PP], k1B[PP];
-  bool nyq[PP];
-  c32 WA[PP];
-  int offA[PP], offB[PP];  // within a rate's M spectra
-#pragma unroll
-  for (int i = 0; i < PP; ++i) {
-    const HPair h = hpair<N1, N2>(p, tid + i * T);
-    rowA[i] = h.rowA;
-    rowB[i] = h.rowB;
-    k1A[i] = h.k1A;
-    k1B[i] = h.k1B;
-    nyq[i] = h.nyq;
-    const int cA = h.cA, cB = h.cB;
-    const int64_t kA = cA + static_cast<int64_t>(N2) * k1A[i];
-    WA[i] = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));
-    offA[i] = cA * N1 + k1A[i];
-    offB[i] = cB * N1 + (nyq[i] ? 0 : k1B[i]);
-  }
-  const float h = 0.5f * a.oscale;
-  if (special) {
-    for (int j = tid; j < a.Nr; j += T) {
-      const c32 y = a.Yc[static_cast<int64_t>(j) * a.M + HALF];
-      a.yhalf[j] = {y.x * a.oscale, -y.y * a.oscale};
-    }
-  }
-  c32 ya[JG][PP], yb[JG][PP];
-  auto load = [&](int j0n) {
-    const int njn = min(JG, a.Nr - j0n);
-#pragma unroll
-    for (int jj = 0; jj < JG; ++jj) {
-      if (jj < njn) {
-        const c32* y = a.Yc + static_cast<int64_t>(j0n + jj) * a.M;  // uniform
-#pragma unroll
-        for (int i = 0; i < PP; ++i) {
-          ya[jj][i] = y[offA[i]];
-          yb[jj][i] = nyq[i] ? a.yM[j0n + jj] : y[offB[i]];
-        }
-      }
-    }
-  };
-  load(0);
-  const int wv = tid >> 6, lane = tid & 63;
-  for (int j0 = 0; j0 < a.Nr; j0 += JG) {
-    const int nj = min(JG, a.Nr - j0);
-#pragma unroll
-    for (int jj = 0; jj < JG; ++jj) {
-      if (jj < nj) {
-#pragma unroll
-        for (int i = 0; i < PP; ++i) {
-          const c32 YA = {ya[jj][i].x * h, ya[jj][i].y * h}, YB = {yb[jj][i].x * h, yb[jj][i].y * h};
-          if (nyq[i]) {
-            buf[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
-            buf[(2 * jj + rowA[i]) * P + pos(HALF)] = {0.0f, 0.0f};
-          } else {
-            const c32 E = {YA.x + YB.x, YA.y - YB.y};
-            const c32 D = {YA.x - YB.x, YA.y + YB.y};
-            const c32 O = cmulc(D, WA[i]);
-            buf[(2 * jj + rowA[i]) * P + pos(k1A[i])] = {E.x - O.y, E.y + O.x};
-            buf[(2 * jj + rowB[i]) * P + pos(k1B[i])] = {E.x + O.y, -E.y + O.x};
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (j0 + JG < a.Nr) load(j0 + JG);
-    if (wv < 2 * nj) {  // uniform per wavefront: wavefront wv owns row wv
-      c32* dst = a.B + static_cast<int64_t>(j0 + (wv >> 1)) * (N1 * N2) + ((wv & 1) ? colB : colA) * 16;
-      auto emit = [&](int jx, auto& v) {
-        constexpr int RL = sizeof(v) / sizeof(c32);
-#pragma unroll
-        for (int r = 0; r < RL; ++r) {
-          const int n1 = jx + r * (N1 / RL);
-          dst[(n1 >> 4) * (N2 * 16) + (n1 & 15)] = v[r];
-        }
-      };
-      if constexpr (BIG) {
-        wave_ifft_2048(buf + wv * P, twl, lane, emit);
-      } else {
-        int ln = lane;
-        asm volatile("" : "+v"(ln));  // keeps the row FFT's LDS addresses inside the rate loop
-        wave_fft_row<N1, true>(buf + wv * P, a.tw1, ln, emit);
-      }
-    }
-    __syncthreads();  // the rows are rewritten by the next group
-  }
-}
-
-// ---------------------------------------------------------------- fused-output input
-
-struct K2iArgs {
-  c32* G;          // in: [Nt][N2][N1] column FFTs of k1_input; out: X_t(k2 + N2 k1) at [t][k2][k1]
-  c32* XM;         // [Nt] X_t(M)
-  const c32* tw1;  // W_N1^m
-  int Nt;
-  int64_t M, Nf;
-};
-
-// input! for the fused output: the second half of the four-step r2c, in
-// place over k1_input's G.  Block per Hermitian column pair as in k2_czt
-// (regular block p: columns p and N2-p; block 0: columns 0 and N2/2); for each
-// group of TG = ROWS/2 time steps the 2 TG rows get the twiddle W_M^{n1 k2}
-// and a forward DFT_N1, then each thread untangles its pairs (k, M-k) into
-// the r2c spectrum and writes both bins back over the rows it read.
-template <int N1, int N2, int ROWS>
-__global__ __launch_bounds__(N1 < 1024 ? N1 : 1024) void k2_input(K2iArgs a) {
-  constexpr int T = N1 < 1024 ? N1 : 1024;
-  constexpr int PP = N1 / T;
-  constexpr int P = N1 + N1 / 8;
-  constexpr int HALF = N1 / 2;
-  constexpr int TG = ROWS / 2;
-  __shared__ c32 buf[ROWS * P];
-  __shared__ c32 buf2[ROWS * P];
-  const int tid = threadIdx.x;
-  const int p = blockIdx.x;
-  const bool special = (p == 0);
-  const int colA = special ? 0 : p;
-  const int colB = special ? N2 / 2 : N2 - p;
-
-  int rowA[PP], rowB[PP], k1A[PP], k1B[PP];
-  bool nyq[PP];
-  c32 WA[PP], twS0[PP], twS1[PP];
-  const float invM = 1.0f / static_cast<float>(a.M);
-#pragma unroll
-  for (int i = 0; i < PP; ++i) {
-    const int q = tid + i * T;
-    const HPair h = hpair<N1, N2>(p, q);
-    rowA[i] = h.rowA;
-    rowB[i] = h.rowB;
-    k1A[i] = h.k1A;
-    k1B[i] = h.k1B;
-    nyq[i] = h.nyq;
-    const int cA = h.cA;
-    const int64_t kA = cA + static_cast<int64_t>(N2) * k1A[i];
-    WA[i] = cis_rev(-static_cast<float>(kA) / static_cast<float>(a.Nf));  // W_Nf^{kA}
-    const int64_t n1 = q;
-    twS0[i] = cis_rev(static_cast<float>((n1 * colA) & (a.M - 1)) * invM);
-    twS1[i] = cis_rev(static_cast<float>((n1 * colB) & (a.M - 1)) * invM);
-  }
-
-  for (int t0 = 0; t0 < a.Nt; t0 += TG) {
-    const int nt = min(TG, a.Nt - t0);
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-      if (r < 2 * nt) {
-        const c32* src = a.G + (static_cast<int64_t>(t0 + (r >> 1)) * N2 + ((r & 1) ? colB : colA)) * N1;
-#pragma unroll
-        for (int i = 0; i < PP; ++i)
-          buf[r * P + lsw(tid + i * T)] = cmulc(src[tid + i * T], (r & 1) ? twS1[i] : twS0[i]);
-      }
-    }
-    __syncthreads();
-    const c32* fz = lds_fft_pp<N1, T, false>(buf, buf2, P, 2 * nt, a.tw1);
-#pragma unroll
-    for (int uu = 0; uu < TG; ++uu) {
-      if (uu < nt) {
-        const int64_t t = t0 + uu;
-        c32* rA = a.G + (t * N2 + colA) * N1;
-        c32* rB = a.G + (t * N2 + colB) * N1;
-#pragma unroll
-        for (int i = 0; i < PP; ++i) {
-          const c32 Za = fz[(2 * uu + rowA[i]) * P + lsw(k1A[i])];
-          c32* dA = rowA[i] ? rB : rA;
-          c32* dB = rowB[i] ? rB : rA;
-          if (nyq[i]) {
-            dA[0] = {Za.x + Za.y, 0.0f};
-            a.XM[t] = {Za.x - Za.y, 0.0f};
-          } else {
-            const c32 Zb = fz[(2 * uu + rowB[i]) * P + lsw(k1B[i])];
-            const c32 E = {0.5f * (Za.x + Zb.x), 0.5f * (Za.y - Zb.y)};
-            const c32 O = {0.5f * (Za.y + Zb.y), -0.5f * (Za.x - Zb.x)};
-            const c32 WO = cmul(WA[i], O);
-            dA[k1A[i]] = cadd(E, WO);
-            dB[k1B[i]] = {E.x - WO.x, -(E.y - WO.y)};
-          }
-        }
-        if (special && tid == 0) {
-          // bin M/2 (column 0, k1 = N1/2) is its own partner: X = conj(Z)
-          const c32 Zh = fz[(2 * uu) * P + lsw(HALF)];
-          rA[HALF] = cconj(Zh);
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// F (natural Nk x Nt) from the fused-output input layout
-__global__ __launch_bounds__(256) void ft_to_natural(const c32* __restrict__ Ft, c32* __restrict__ F, int64_t Nk,
-                                                     int Nt, int logN1, int logN2) {
-  const int64_t M = Nk - 1;
-  const int64_t total = Nk * Nt;
-  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
-       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t t = g / Nk, k = g - t * Nk;
-    const int64_t k2 = k & ((int64_t(1) << logN2) - 1), k1 = k >> logN2;
-    F[g] = (k == M) ? Ft[M * Nt + t] : Ft[t * M + (k2 << logN1) + k1];
-  }
-}
-
 // ---------------------------------------------------------------- F unpack
 
 // Gt[t][k2][n1] = G[t][k2][n1] W_M^{n1 k2} (the four-step twiddle k2 applies
@@ -1540,26 +629,8 @@ __global__ __launch_bounds__(256) void unpack_F(const c32* __restrict__ Z, c32* 
 
 namespace zdt {
 
-struct FastPlan {
-  int64_t Nf = 0, Nt = 0, Nr = 0, M = 0;
-  int N1 = 0, logN1 = 0, N2 = 0, logN2 = 0, NT = 0;
-  float dr = 0;
-  c32* G = nullptr;     // [Nt][N2][N1]
-  c32* B = nullptr;     // [Nr][N1/16][N2][16]
-  c32* tw1 = nullptr;   // W_N1^m
-  c32* tw2 = nullptr;   // W_N2^m
-  c32* tabq = nullptr;  // e^{i pi dr j}
-  c64* tab64 = nullptr;  // e^{i pi dr j} in FP64
-  c32* yhalf = nullptr;  // [Nr] bin M/2 term of the c2r (k2 -> k3)
-  c32* Yc = nullptr;     // fused output only: [Nr][N2][N1] spectra
-  c32* yM = nullptr;     // fused output only: [Nr] Nyquist bin
-  double* part = nullptr;  // k3's per-wavefront FDR statistics (fast_stats_doubles)
-  bool paired = false;     // the last k2 wrote B in mirrored rate pairs
-  bool have_B = false;
-  int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
-};
 
-static int ilog2i(int64_t n) {
+int ilog2i(int64_t n) {
   int l = 0;
   while ((int64_t(1) << l) < n) ++l;
   return l;
@@ -1579,14 +650,14 @@ int64_t fast_bytes(int64_t Nf, int64_t Nt, int64_t Nr) {
   return 8 * M * (Nt + Nr) + 8 * (N1 + N2 + 2 * Nr) + 8 * fast_stats_doubles(Nf, Nr);
 }
 
-static int upload_table(c32** dst, const std::vector<c32>& h) {
+int upload_table(c32** dst, const std::vector<c32>& h) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), h.size() * sizeof(c32));
   if (e != hipSuccess) return fail(ZDT_ENOMEM, "hipMalloc(table) failed");
   ZDT_TRY_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(c32), hipMemcpyHostToDevice));
   return ZDT_OK;
 }
 
-static std::vector<c32> twiddles(int N) {
+std::vector<c32> twiddles(int N) {
   std::vector<c32> t(N);
   for (int m = 0; m < N; ++m) {
     const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / N;
@@ -1595,7 +666,7 @@ static std::vector<c32> twiddles(int N) {
   return t;
 }
 
-static void alloc_part(FastPlan* p, int& rc) {
+void alloc_part(FastPlan* p, int& rc) {
   if (rc != ZDT_OK) return;
   const size_t n = static_cast<size_t>(fast_stats_doubles(p->Nf, p->Nr));
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->part), n * sizeof(double));
@@ -1694,7 +765,7 @@ int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, FastPlan**
   return ZDT_OK;
 }
 
-static int launch_k1(const FastPlan* p, const float* spec, c32* G, int64_t Nt, hipStream_t st) {
+int launch_k1(const FastPlan* p, const float* spec, c32* G, int64_t Nt, hipStream_t st) {
   K1Args a;
   a.spec = spec;
   a.ldspec = p->Nf;
@@ -1883,229 +954,6 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* 
     k3_stats_merge<<<static_cast<unsigned>(p->Nr), 256, 0, st>>>(p->part, np, p->Nf, colmean, colvar);
     ZDT_CHECK_LAUNCH("k3_stats_merge");
   }
-  return ZDT_OK;
-}
-
-// ---------------------------------------------------------------- fused output (host)
-
-bool fast_out_supported(int64_t Nf, int64_t Nt, int64_t Nr) {
-  if (Nf < (1 << 16) || Nf > (1 << 22) || (Nf & (Nf - 1)) != 0) return false;
-  return Nt >= 1 && Nt <= 64 && Nr >= 1 && Nr <= 65535;
-}
-
-int64_t fast_out_bytes(int64_t Nf, int64_t Nr) {
-  const int64_t M = Nf / 2;
-  const int64_t N2 = fast_n2(Nf);
-  return 16 * M * Nr + 8 * (M / N2 + N2 + 2 * Nr) + 8 * fast_stats_doubles(Nf, Nr);
-}
-
-int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out) {
-  *out = nullptr;
-  if (!fast_out_supported(Nf, 1, Nr)) return fail(ZDT_ENOTSUP, "fused output: unsupported shape");
-  FastPlan* p = new FastPlan();
-  p->Nf = Nf;
-  p->Nt = 0;
-  p->Nr = Nr;
-  p->M = Nf / 2;
-  p->N2 = fast_n2(Nf);
-  p->logN2 = ilog2i(p->N2);
-  p->N1 = static_cast<int>(p->M / p->N2);
-  p->logN1 = ilog2i(p->N1);
-  {
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    p->resident_blocks = 2 * static_cast<int64_t>(ncu);
-  }
-  int rc = ZDT_OK;
-  auto alloc = [&](c32** q, size_t n) {
-    if (rc != ZDT_OK) return;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(q), n * sizeof(c32));
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      rc = fail(ZDT_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(c32), hipGetErrorString(e));
-    }
-  };
-  alloc(&p->Yc, static_cast<size_t>(Nr * p->M));
-  alloc(&p->B, static_cast<size_t>(Nr * p->M));
-  alloc_part(p, rc);
-  alloc(&p->yM, static_cast<size_t>(Nr));
-  alloc(&p->yhalf, static_cast<size_t>(Nr));
-  if (rc == ZDT_OK) rc = upload_table(&p->tw1, twiddles(p->N1));
-  if (rc == ZDT_OK) rc = upload_table(&p->tw2, twiddles(p->N2));
-  if (rc != ZDT_OK) {
-    fast_destroy(p);
-    return rc;
-  }
-  *out = p;
-  return ZDT_OK;
-}
-
-template <int N1, int N2>
-static int launch_k2c(const K2cArgs& a, hipStream_t st) {
-  if constexpr (N1 <= 512) {
-    k2_c2r<N1, N2, k2_rows(N1)><<<N2 / 2, N1, 0, st>>>(a);
-  } else {
-    k2_c2r_wave<N1, N2><<<N2 / 2, 512, 0, st>>>(a);
-  }
-  ZDT_CHECK_LAUNCH("k2_c2r");
-  return ZDT_OK;
-}
-
-template <int N1, int N2>
-static void launch_k2i(const K2iArgs& a, hipStream_t st) {
-  // two ping-pong buffers of ~36 KB each: 8 rows up to N1 = 512, then 4, 2
-  constexpr int ROWS = N1 <= 512 ? 8 : (N1 == 1024 ? 4 : 2);
-  k2_input<N1, N2, ROWS><<<N2 / 2, (N1 < 1024 ? N1 : 1024), 0, st>>>(a);
-}
-
-int fast_out_input(FastPlan* p, const float* spec, int64_t Nt, c32* Ft, hipStream_t st) {
-  if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%lld", (long long)Nt);
-  ZDT_TRY(launch_k1(p, spec, Ft, Nt, st));
-  K2iArgs a;
-  a.G = Ft;
-  a.XM = Ft + p->M * Nt;
-  a.tw1 = p->tw1;
-  a.Nt = static_cast<int>(Nt);
-  a.M = p->M;
-  a.Nf = p->Nf;
-  ProfScope ps(st, "k2_input", 16.0 * p->M * Nt);
-  switch (p->N1 * 2048 + p->N2) {
-    case 64 * 2048 + 512:
-      launch_k2i<64, 512>(a, st);
-      break;
-    case 64 * 2048 + 1024:
-      launch_k2i<64, 1024>(a, st);
-      break;
-    case 128 * 2048 + 1024:
-      launch_k2i<128, 1024>(a, st);
-      break;
-    case 256 * 2048 + 1024:
-      launch_k2i<256, 1024>(a, st);
-      break;
-    case 512 * 2048 + 1024:
-      launch_k2i<512, 1024>(a, st);
-      break;
-    case 1024 * 2048 + 1024:
-      launch_k2i<1024, 1024>(a, st);
-      break;
-    case 2048 * 2048 + 1024:
-      launch_k2i<2048, 1024>(a, st);
-      break;
-    default:
-      return fail(ZDT_ENOTSUP, "fused output input: N1=%d N2=%d", p->N1, p->N2);
-  }
-  ZDT_CHECK_LAUNCH("k2_input");
-  return ZDT_OK;
-}
-
-int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_t st) {
-  const int64_t n = (p->M + 1) * Nt;
-  int64_t g = (n + 255) / 256;
-  if (g > (1 << 20)) g = 1 << 20;
-  ft_to_natural<<<static_cast<unsigned>(g), 256, 0, st>>>(Ft, F, p->M + 1, static_cast<int>(Nt), p->logN1, p->logN2);
-  ZDT_CHECK_LAUNCH("ft_to_natural");
-  return ZDT_OK;
-}
-
-// P when the rates r0 + j dr (j < Nr) pair up as r_j = -r_{P-j} for more than
-// a third of them (P = -2 r0/dr, an integer; the Reinsch evaluation of a
-// pair costs 3/4 of two Horner evaluations, of a lone rate 3/2 of one), else -1
-static int mirror_pairs(float r0, float dr, int64_t Nr) {
-  if (dr == 0.0f) return -1;
-  const double P = -2.0 * static_cast<double>(r0) / static_cast<double>(dr);
-  if (!std::isfinite(P) || P != std::rint(P) || P < 1.0 || P > 2.0 * static_cast<double>(Nr)) return -1;
-  const int64_t Pi = static_cast<int64_t>(P);
-  int64_t npairs = 0;
-  for (int64_t j = 0; j < Nr; ++j)
-    if (Pi - j > j && Pi - j < Nr) ++npairs;
-  return 3 * npairs > Nr ? static_cast<int>(Pi) : -1;
-}
-
-int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
-                 const float* window, float* dest, double scale, hipStream_t st, double* colmean, double* colvar) {
-  if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
-  CztColArgs c;
-  c.F = F;
-  c.ldF = ldF;
-  c.ftrans = ftrans ? 1 : 0;
-  c.Yc = p->Yc;
-  c.yM = p->yM;
-  c.M = p->M;
-  c.Nf = p->Nf;
-  c.logN1 = p->logN1;
-  c.logN2 = p->logN2;
-  c.Nt = Nt;
-  c.Nr = static_cast<int>(p->Nr);
-  c.hd2 = static_cast<double>(dr) / static_cast<double>(p->Nf);
-  c.h0 = static_cast<double>(r0) / static_cast<double>(p->Nf);
-  c.wkind = wkind;
-  c.wcustom = window;
-  c.scale = 1.0f;
-  c.pair = mirror_pairs(r0, dr, p->Nr);
-  {
-    const unsigned grid = static_cast<unsigned>(p->M / 256 + 1);
-    if (c.pair >= 0) {
-      ProfScope ps(st, "czt_reinsch_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
-      if (Nt <= 16) {
-        czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);
-      } else if (Nt <= 32) {
-        czt_reinsch_col<32><<<grid, 256, 0, st>>>(c);
-      } else {
-        czt_reinsch_col<64><<<grid, 256, 0, st>>>(c);
-      }
-      ZDT_CHECK_LAUNCH("czt_reinsch_col");
-    } else {
-      ProfScope ps(st, "czt_horner_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
-      if (Nt <= 16) {
-        czt_horner_col<16><<<grid, 256, 0, st>>>(c);
-      } else if (Nt <= 32) {
-        czt_horner_col<32><<<grid, 256, 0, st>>>(c);
-      } else {
-        czt_horner_col<64><<<grid, 256, 0, st>>>(c);
-      }
-      ZDT_CHECK_LAUNCH("czt_horner_col");
-    }
-  }
-  K2cArgs a;
-  a.Yc = p->Yc;
-  a.yM = p->yM;
-  a.B = p->B;
-  a.yhalf = p->yhalf;
-  a.tw1 = p->tw1;
-  a.Nr = static_cast<int>(p->Nr);
-  a.M = p->M;
-  a.Nf = p->Nf;
-  a.oscale = static_cast<float>(scale / static_cast<double>(p->M));
-  {
-    ProfScope ps(st, "k2_c2r", 16.0 * p->M * p->Nr);
-    switch (p->N1 * 2048 + p->N2) {
-      case 64 * 2048 + 512:
-        ZDT_TRY((launch_k2c<64, 512>(a, st)));
-        break;
-      case 64 * 2048 + 1024:
-        ZDT_TRY((launch_k2c<64, 1024>(a, st)));
-        break;
-      case 128 * 2048 + 1024:
-        ZDT_TRY((launch_k2c<128, 1024>(a, st)));
-        break;
-      case 256 * 2048 + 1024:
-        ZDT_TRY((launch_k2c<256, 1024>(a, st)));
-        break;
-      case 512 * 2048 + 1024:
-        ZDT_TRY((launch_k2c<512, 1024>(a, st)));
-        break;
-      case 1024 * 2048 + 1024:
-        ZDT_TRY((launch_k2c<1024, 1024>(a, st)));
-        break;
-      case 2048 * 2048 + 1024:
-        ZDT_TRY((launch_k2c<2048, 1024>(a, st)));
-        break;
-      default:
-        return fail(ZDT_ENOTSUP, "fused output: N1=%d N2=%d", p->N1, p->N2);
-    }
-  }
-  p->have_B = true;
-  if (dest != nullptr) ZDT_TRY(fast_output(p, dest, scale, st, colmean, colvar));
   return ZDT_OK;
 }
 

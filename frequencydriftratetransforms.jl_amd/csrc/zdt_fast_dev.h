@@ -1,0 +1,320 @@
+// Device building blocks of the fused kernels (zdt_fast.hip, zdt_fout.hip):
+// LDS and wavefront FFT passes, the register-resident column FFT of k1/k3,
+// window weights, FP64 phases, the Hermitian column pairs of k2 and the
+// chirp-Z evaluations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_butterflies.h"
+#include "zdt_internal.h"
+
+namespace zdt {
+
+// ---------------------------------------------------------------- LDS FFT
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// radix of the pass at stride NS in the LDS FFTs: 1024 = 16*8*8,
+// 512 = 8*8*8, 256 = 16*16, 128 = 16*8, 64 = 8*8
+template <int N, int NS>
+constexpr int fft_radix() {
+  return (N / NS == 16 || (NS == 1 && ilog2c(N) % 3 != 0)) ? 16 : (N / NS >= 8 ? 8 : N / NS);
+}
+
+// LDS row swizzle: one pad entry after every 8, so the strided Stockham
+// writes of the early passes spread over the banks.  Rows of length N occupy
+// N + N/8 entries.
+__device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
+
+// Ping-pong variant: radix-R butterflies stream from buffer `in` to buffer
+// `out` (one butterfly live per thread), one barrier per pass.  Rows are
+// swizzled (lsw); since NR and Ns are multiples of 8 or the write base is,
+// every butterfly needs the swizzle only once for its reads and once for its
+// writes.  Twiddles come from the LDS table tw (W_N^m, m < N); INV selects the
+// inverse butterflies and conjugate twiddles.
+template <int N, int R, int T, bool INV>
+__device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int nrows, int Ns,
+                                            const c32* __restrict__ tw) {
+  constexpr int NR = N / R;
+  constexpr int LOGNR = ilog2c(NR);
+  constexpr int LOGN = ilog2c(N);
+  constexpr int LOGR = ilog2c(R);
+  static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
+  const int logNs = 31 - __builtin_clz(Ns);
+  const int total = nrows * NR;
+  const int sh = LOGN - logNs - LOGR;
+  for (int b = threadIdx.x; b < total; b += T) {
+    const int f = b >> LOGNR;
+    const int j = b & (NR - 1);
+    int k = j & (Ns - 1);
+    // Opaque to the optimizer: keeps the R-1 twiddle addresses from being
+    // hoisted out of the caller's rate loop, where they would be spilled.
+    asm volatile("" : "+v"(k));
+    const c32* x = in + f * P + lsw(j);
+    c32 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = x[r * (NR + NR / 8)];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const c32 w = tw[(r * k) << sh];
+        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+      }
+    }
+    dftd<INV, float, R>(v);
+    const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
+    c32* y = out + f * P;
+    if (Ns == 1) {
+      // d0 = j*R is a multiple of 8 (R >= 8): lsw(d0 + r) = lsw(d0) + r + r/8
+      c32* yy = y + lsw(d0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
+    } else {
+      // Ns >= 8: the stride Ns keeps the swizzle offset linear
+      c32* yy = y + lsw(d0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
+    }
+  }
+  __syncthreads();
+}
+
+// radix schedule: 1024 = 16*8*8, 512 = 8*8*8, 256 = 16*16, 128 = 16*8,
+// 64 = 8*8; returns the buffer holding the result
+template <int N, int T, bool INV, int NS = 1>
+__device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, const c32* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int R = fft_radix<N, NS>();
+    lds_pass_pp<N, R, T, INV>(a, b, P, nrows, NS, tw);
+    return lds_fft_pp<N, T, INV, NS * R>(b, a, P, nrows, tw);
+  } else {
+    return a;
+  }
+}
+
+// Wavefront-level passes.  The rows [0, nrw) at `rows` (pitch P, swizzled)
+// belong to one wavefront, and lane b < nrw*N/R owns butterfly b of each
+// pass.  A wavefront issues all its reads of a pass before any of its
+// writes and its LDS accesses complete in order, so the passes run in place
+// with no workgroup barrier; wave_sync only keeps the compiler from moving
+// LDS accesses across pass boundaries.  Requires nrw*N/R <= 64 for every
+// radix R of the schedule.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int N, int R, bool INV>
+__device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, const c32* __restrict__ tw, int lane,
+                                          c32 (&v)[R], bool last) {
+  constexpr int NR = N / R;
+  constexpr int LOGNR = ilog2c(NR);
+  constexpr int LOGN = ilog2c(N);
+  constexpr int LOGR = ilog2c(R);
+  static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
+  if (lane < nrw * NR) {
+    const int logNs = 31 - __builtin_clz(Ns);
+    const int f = lane >> LOGNR;
+    const int j = lane & (NR - 1);
+    int k = j & (Ns - 1);
+    asm volatile("" : "+v"(k));  // keeps twiddle addresses out of the caller's loop
+    c32* x = rows + f * P;
+    const c32* xr = x + lsw(j);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = xr[r * (NR + NR / 8)];
+    if (Ns > 1) {
+      const int sh = LOGN - logNs - LOGR;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const c32 w = tw[(r * k) << sh];
+        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+      }
+    }
+    dftd<INV, float, R>(v);
+    if (!last) {
+      const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
+      c32* yy = x + lsw(d0);
+      if (Ns == 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
+      }
+    }
+  }
+  wave_sync();
+}
+
+template <int N, int NS = 1>
+constexpr int fft_max_nr() {
+  if constexpr (NS >= N) {
+    return 0;
+  } else {
+    constexpr int R = fft_radix<N, NS>();
+    constexpr int a = N / R, b = fft_max_nr<N, NS * R>();
+    return a > b ? a : b;
+  }
+}
+
+// In-place inverse/forward FFT of the wavefront's rows; the last pass leaves
+// its butterfly outputs in registers (lane (f, j): row f, elements
+// j + r*N/R) and calls emit(f, j, v) instead of writing them back.
+template <int N, bool INV, int NS = 1, typename Emit>
+__device__ __forceinline__ void wave_fft(c32* rows, int P, int nrw, const c32* __restrict__ tw, int lane, Emit&& emit) {
+  constexpr int R = fft_radix<N, NS>();
+  c32 v[R];
+  if constexpr (NS * R < N) {
+    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, false);
+    wave_fft<N, INV, NS * R>(rows, P, nrw, tw, lane, emit);
+  } else {
+    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, true);
+    constexpr int NR = N / R;
+    if (lane < nrw * NR) emit(lane / NR, lane % NR, v);
+  }
+}
+
+// ---------------------------------------------------------------- column FFT (N2 = 32*R1)
+//
+// k1 and k3 transform 16 adjacent columns (n1 or k1 fixed, 128 contiguous
+// bytes per row) along a strided axis of length N2 = 32*R1 (R1 = 16 or 32),
+// register resident: thread (c = tid&15, r = tid>>4) holds rows r + R1*m
+// (m < 32) of column c, does a DFT-32 over m, multiplies by W_N2^{r m'}, and
+// after one LDS exchange the thread (c, g) does the DFT-R1 over r for
+// m' = g + R1*h, giving output rows m' + 32 r'.  Inverse transforms conjugate
+// on entry and exit.  Blocks are persistent over tiles.
+
+template <int R1>
+struct CF {
+  static constexpr int N2 = 32 * R1;
+  static constexpr int THREADS = 16 * R1;
+  static constexpr int PITCH = R1 * 16 + 8;  // complex per m' slab ([r][c] + pad)
+  static constexpr int LDS = 32 * PITCH;
+  static constexpr int MPER = 32 / R1;       // m' values per thread in stage 2
+};
+
+template <int R1>
+__device__ __forceinline__ void cf_stage1(c32 (&x)[32], int r, const c32* twl, c32* ex, int c, bool inv) {
+  if (inv) {
+#pragma unroll
+    for (int m = 0; m < 32; ++m) x[m].y = -x[m].y;
+  }
+  dft32(x);
+#pragma unroll
+  for (int m = 1; m < 32; ++m) x[m] = cmul(x[m], twl[r * m]);  // W_N2^{r m'}
+#pragma unroll
+  for (int m = 0; m < 32; ++m) ex[m * CF<R1>::PITCH + r * 16 + c] = x[m];
+}
+
+// stage 2 for one m': y[r'] is output row m' + 32 r'
+template <int R1>
+__device__ __forceinline__ void cf_stage2(c32 (&y)[R1], const c32* ex, int c, int mp, bool inv) {
+#pragma unroll
+  for (int r = 0; r < R1; ++r) y[r] = ex[mp * CF<R1>::PITCH + r * 16 + c];
+  dft<float, R1>(y);
+  if (inv) {
+#pragma unroll
+    for (int r = 0; r < R1; ++r) y[r].y = -y[r].y;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void cf_load_twiddles(c32* twl, const c32* __restrict__ tw, int nthreads) {
+  for (int i = threadIdx.x; i < N; i += nthreads) twl[i] = tw[i];
+}
+
+__device__ __forceinline__ float fast_window(int wkind, const float* wcustom, int64_t k, int64_t Nf) {
+  if (wkind == ZDT_WINDOW_HAMMING) {
+    const float p = static_cast<float>(static_cast<double>(k) / static_cast<double>(Nf));
+    return 0.53836f + 0.46164f * cis_rev(p).x;
+  }
+  if (wkind == ZDT_WINDOW_CUSTOM) return wcustom[k];
+  return 1.0f;
+}
+
+__device__ __forceinline__ c64 cis64f(double rev) {
+  const double f = rev - __builtin_rint(rev);
+  double s, c;
+  sincospi(2.0 * f, &s, &c);
+  return {c, s};
+}
+
+// Horner: sum_u a[u] z^u
+template <int NT>
+__device__ __forceinline__ c32 horner(const c32* a, c32 z) {
+  c32 acc = a[NT - 1];
+#pragma unroll
+  for (int u = NT - 2; u >= 0; --u) acc = cfma(a[u], acc, z);
+  return acc;
+}
+
+// k2 occupancy target: two 512-thread blocks per CU (4 waves per SIMD) for
+// N1 = 512, whose LDS (2 x 8 rows x 576 + 512 entries, 78 KB) allows it.
+constexpr int k2_waves_per_eu(int n1) { return n1 >= 512 ? 4 : 2; }
+// rows per group: as many as keep one butterfly per lane in the
+// wavefront-level passes (8 rows for 512 = 8*8*8 over 8 wavefronts, ...)
+constexpr int k2_rows(int n1) { return n1 == 256 ? 16 : 8; }
+
+// The Hermitian pair q of column-pair block p (see k2_czt): bin
+// kA = cA + N2 k1A in row rowA of the block's two rows and its partner
+// M - kA = cB + N2 k1B in row rowB.  Block p > 0 holds columns p and N2-p;
+// block 0 holds the self-paired columns 0 (q < N1/2, pairs k1 and N1-k1, the
+// partner of q = 0 being the Nyquist bin M) and N2/2 (q >= N1/2).
+struct HPair {
+  int rowA, rowB, k1A, k1B, cA, cB;
+  bool nyq;
+};
+
+template <int N1, int N2>
+__device__ __forceinline__ HPair hpair(int p, int q) {
+  HPair h;
+  h.nyq = false;
+  if (p != 0) {
+    h.rowA = 0;
+    h.rowB = 1;
+    h.k1A = q;
+    h.k1B = N1 - 1 - q;
+    h.cA = p;
+    h.cB = N2 - p;
+  } else if (q < N1 / 2) {
+    h.rowA = h.rowB = 0;
+    h.k1A = q;
+    h.k1B = N1 - q;
+    h.nyq = (q == 0);
+    h.cA = h.cB = 0;
+  } else {
+    h.rowA = h.rowB = 1;
+    h.k1A = q - N1 / 2;
+    h.k1B = N1 - 1 - h.k1A;
+    h.cA = h.cB = N2 / 2;
+  }
+  return h;
+}
+
+// Sums of the coefficients c_t at z and conj(z), |z| = 1, by the
+// Reinsch-modified Goertzel recurrence over the real and imaginary parts
+// (see czt_reinsch_col); s, lambda and sin(theta) come from the FP64 z.
+template <int NT>
+__device__ __forceinline__ void reinsch2(const c32 (&c)[NT], c64 z, c32& Yz, c32& Yzc) {
+  const float s = z.x >= 0.0 ? 1.0f : -1.0f;
+  const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
+  const float sn = static_cast<float>(z.y);
+  float Dr = c[NT - 1].x, Ur = c[NT - 1].x, Di = c[NT - 1].y, Ui = c[NT - 1].y;
+#pragma unroll
+  for (int t = NT - 2; t >= 1; --t) {
+    Dr = fmaf(s, Dr, fmaf(lam, Ur, c[t].x));
+    Ur = fmaf(s, Ur, Dr);
+    Di = fmaf(s, Di, fmaf(lam, Ui, c[t].y));
+    Ui = fmaf(s, Ui, Di);
+  }
+  const float hl = 0.5f * lam;
+  const float Cr = fmaf(s, Dr, fmaf(hl, Ur, c[0].x));
+  const float Ci = fmaf(s, Di, fmaf(hl, Ui, c[0].y));
+  const float Sr = sn * Ur, Si = sn * Ui;
+  Yz = {Cr - Si, Sr + Ci};
+  Yzc = {Cr + Si, Ci - Sr};
+}
+
+}  // namespace zdt
