@@ -248,13 +248,15 @@ struct K2Args {
   const c32* tw1;    // W_N1^m
   const c32* tabq;   // e^{i pi dr j}, j < Nr
   const c64* tab64;  // the same in FP64 (paired rates)
-  c64 eir0;          // e^{i pi r0} (paired rates)
+  c64 eir0;          // e^{i pi (r0 - rc)} (paired rates)
+  double hrot;       // rc/Nf: the coefficients are pre-rotated by e^{2 pi i k u rc/Nf}
+  double rhrot;      // rc/2 (partner bin: e^{i pi u rc})
+  double hz0;        // (r0 - rc)/Nf (paired rates: z of rate 0)
   c32* yhalf;        // [Nr] c2r pre-pass value of the bin M/2, added by k3
   int Nt, Nr;
   int logN1;
   int64_t M, Nf;
   double hd2;        // dr/Nf: gamma_k = e^{2 pi i k hd2}
-  double h0;         // r0/Nf
   double r0half;     // r0/2:  e^{i pi u r0} = e^{2 pi i u r0half}
   int wkind;
   const float* wcustom;
@@ -272,11 +274,12 @@ struct K2Args {
 // keeps its registers out of the rate loop.  The window weight and output
 // scale are folded into the Vandermonde coefficients a_t.
 //
-// PAIRED: the rates are symmetric, r_j = -r_{Nr-1-j}.  The coefficients are
-// then not pre-rotated; rates j and Nr-1-j of a bin are its sums at
-// z = e^{2 pi i k r_j/Nf} and conj(z) and come from one reinsch2 (partner
-// bin: z = e^{i pi r_j} conj(z_k)), and a group of ROWS/2 rates holds the
-// pairs (j, Nr-1-j), (j+1, Nr-2-j), ...
+// PAIRED: the coefficients are pre-rotated by the centre rate
+// rc = r0 + (Nr-1) dr/2 instead of r0, so that the remaining rates
+// r_j - rc = (j - (Nr-1)/2) dr are mirrored: rates j and Nr-1-j of a bin are
+// its sums at z = e^{2 pi i k (r_j - rc)/Nf} and conj(z) and come from one
+// reinsch2 (partner bin: z = e^{i pi (r_j - rc)} conj(z_k)); a group of
+// ROWS/2 rates holds the pairs (j, Nr-1-j), (j+1, Nr-2-j), ...
 template <int N1, int N2, int NT, int ROWS, bool PAIRED = false>
 __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_czt(K2Args a) {
   constexpr int T = N1 < 512 ? N1 : 512;
@@ -377,7 +380,6 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       const int u = t0 + uu;
       if (u < NT && uu < nt) {
         const double ud = static_cast<double>(u);
-        const c32 rq = cis_nh(ud, a.r0half);  // e^{i pi u r0}
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
           const c32 Za = fz[(2 * uu + rowA[i]) * P + lsw(k1A[i])];
@@ -393,16 +395,16 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
             XA = cadd(E, WO);
             XB = {E.x - WO.x, -(E.y - WO.y)};
           }
-          if constexpr (PAIRED) {
-            aA[i][u] = {XA.x * wA[i], XA.y * wA[i]};
-            aB[i][u] = {XB.x * wB[i], XB.y * wB[i]};
-          } else {
-            // pre-rotation e^{2 pi i k u r0/Nf}; partner: e^{i pi u r0} conj(.)
-            const c32 rot = cis_nh(static_cast<double>(kA[i]) * ud, a.h0);
-            const c32 ra = cmul(XA, rot), rb = cmul(XB, cmulc(rq, rot));
-            aA[i][u] = {ra.x * wA[i], ra.y * wA[i]};
-            aB[i][u] = {rb.x * wB[i], rb.y * wB[i]};
+          // pre-rotation e^{2 pi i k u rc/Nf}; partner: e^{i pi u rc} conj(.)
+          c32 ra = XA, rb = XB;
+          if (a.hrot != 0.0) {  // uniform (rc = 0: rates symmetric about 0)
+            const c32 rq = cis_nh(ud, a.rhrot);  // e^{i pi u rc}
+            const c32 rot = cis_nh(static_cast<double>(kA[i]) * ud, a.hrot);
+            ra = cmul(XA, rot);
+            rb = cmul(XB, cmulc(rq, rot));
           }
+          aA[i][u] = {ra.x * wA[i], ra.y * wA[i]};
+          aB[i][u] = {rb.x * wB[i], rb.y * wB[i]};
         }
         if (special && tid == 0) {
           // bin M/2: X = conj(Z[M/2]); rotation e^{2 pi i (M/2) u r0/Nf} = e^{i pi u r0/2}
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     const int g0 = ((npairs - 1) / NPG) * NPG;
 #pragma unroll
     for (int i = 0; i < PP; ++i)  // e^{2 pi i k r_g0/Nf}
-      zp[i] = cis64f(static_cast<double>(kA[i]) * (a.h0 + static_cast<double>(g0) * a.hd2));
+      zp[i] = cis64f(static_cast<double>(kA[i]) * (a.hz0 + static_cast<double>(g0) * a.hd2));
     c32* cur = buf;
     c32* nxt = buf2;
     for (int p0 = g0; p0 >= 0; p0 -= NPG) {
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
         if (pp >= np) break;  // uniform
         const int jA = p0 + pp;
         const bool twin = Pm - jA != jA;
-        const c64 q = cmul(a.eir0, a.tab64[jA]);  // e^{i pi r_jA}
+        const c64 q = cmul(a.eir0, a.tab64[jA]);  // e^{i pi (r_jA - rc)}
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
           const c64 zA = zq[i];
@@ -824,15 +826,18 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
   a.tw1 = p->tw1;
   a.tabq = p->tabq;
   a.tab64 = p->tab64;
+  // the paired evaluation (compiled for NT = 16) around the centre rate
+  const bool paired = p->Nr > 1 && p->NT == 16;
+  p->paired = paired;
+  const double rc = paired ? static_cast<double>(r0) + 0.5 * static_cast<double>(p->Nr - 1) * p->dr
+                           : static_cast<double>(r0);
   {
-    const double h = 0.5 * static_cast<double>(r0);  // e^{i pi r0}
+    const double h = 0.5 * (static_cast<double>(r0) - rc);  // e^{i pi (r0 - rc)}
     a.eir0 = {std::cos(2.0 * M_PI * (h - std::rint(h))), std::sin(2.0 * M_PI * (h - std::rint(h)))};
   }
-  // symmetric rates (r0 = -(Nr-1) dr/2 exactly): the paired evaluation
-  // (compiled for NT = 16)
-  const bool paired = p->Nr > 1 && -2.0 * static_cast<double>(r0) == static_cast<double>(p->Nr - 1) * p->dr &&
-                      p->NT == 16;
-  p->paired = paired;
+  a.hrot = rc / static_cast<double>(p->Nf);
+  a.rhrot = 0.5 * rc;
+  a.hz0 = (static_cast<double>(r0) - rc) / static_cast<double>(p->Nf);
   a.yhalf = p->yhalf;
   a.Nt = static_cast<int>(p->Nt);
   a.Nr = static_cast<int>(p->Nr);
@@ -840,7 +845,6 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
   a.M = p->M;
   a.Nf = p->Nf;
   a.hd2 = static_cast<double>(p->dr) / static_cast<double>(p->Nf);
-  a.h0 = static_cast<double>(r0) / static_cast<double>(p->Nf);
   a.r0half = 0.5 * static_cast<double>(r0);
   a.wkind = wkind;
   a.wcustom = window;

@@ -42,7 +42,8 @@ struct CztColArgs {
   int wkind;
   const float* wcustom;
   float scale;
-  int pair;      // czt_reinsch_col: rates pair as r_j = -r_{pair-j}
+  double hc;     // czt_reinsch_col: rc/Nf, the coefficients' pre-rotation (centre rate rc)
+  double hz;     // czt_reinsch_col: (r0 - rc)/Nf
 };
 
 // The bin of a thread: one thread per bin; a block covers 16 columns k2 x 16
@@ -119,19 +120,20 @@ __global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
   }
 }
 
-// Rates in mirrored pairs r_j = -r_{P-j} (P = pair): the bin's sums at
-// z = e^{i theta} and conj(z), theta = 2 pi k r_j/Nf, come from one
-// Reinsch-modified Goertzel recurrence (C. Reinsch, 1967; J. Oliver, "An
-// error analysis of the modified Clenshaw method", 1977) over the real and
-// the imaginary parts of the coefficients.  With s = sign(cos theta) and
-// lambda = 2 (cos theta - s),
+// The chirp-Z of a bin for the rates in mirrored pairs: with the centre rate
+// rc = r0 + (Nr-1) dr/2 folded into the coefficients (c_t e^{2 pi i k t rc/Nf}),
+// rates j and Nr-1-j are the sums at z = e^{i theta} and conj(z),
+// theta = 2 pi k (r_j - rc)/Nf, and come from one Reinsch-modified Goertzel
+// recurrence (C. Reinsch, 1967; J. Oliver, "An error analysis of the modified
+// Clenshaw method", 1977) over the real and the imaginary parts of the
+// coefficients.  With s = sign(cos theta) and lambda = 2 (cos theta - s),
 //   D_t = a_t + lambda U_{t+1} + s D_{t+1},  U_t = D_t + s U_{t+1}  (t = Nt-1 .. 1),
 //   sum a_t cos(t theta) = a_0 + (lambda/2) U_1 + s D_1,  sum a_t sin(t theta) = U_1 sin theta:
 // three FMAs per term and part, six per term for both rates against eight
 // for two complex Horner evaluations.  lambda comes from the FP64 cos theta,
 // so no accuracy is lost near theta = 0 or pi (where Goertzel's rounded
-// 2 cos theta would lose it).  Rates without a partner are evaluated the same
-// way and only the sum at z is stored.
+// 2 cos theta would lose it).  The middle rate of an odd Nr is its own
+// partner.
 template <int NT>
 __global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
   ColBin bin;
@@ -142,7 +144,8 @@ __global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     if (t < a.Nt) {
-      const c32 f = a.F[bin.fo + static_cast<int64_t>(t) * bin.fs];
+      c32 f = a.F[bin.fo + static_cast<int64_t>(t) * bin.fs];
+      if (a.hc != 0.0) f = cmul(f, cis_nh(kd * static_cast<double>(t), a.hc));  // uniform
       cr[t] = f.x * w;
       ci[t] = f.y * w;
     } else {
@@ -151,11 +154,12 @@ __global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
     }
   }
   const c64 g = cis64f(kd * a.hd2);
-  c64 z = cis64f(kd * a.h0);  // e^{2 pi i k r_0/Nf}
-  for (int j = 0; j < a.Nr; ++j) {
-    const int jp = a.pair - j;
-    const bool mirror = jp >= 0 && jp < a.Nr;
-    if (!mirror || jp >= j) {  // uniform: the partner's sum is stored with j's
+  c64 z = cis64f(kd * a.hz);  // e^{2 pi i k (r_0 - rc)/Nf}
+  const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
+  for (int j = 0; j < npair; ++j) {
+    const int jp = a.Nr - 1 - j;
+    const bool mirror = jp != j;
+    {
       const float s = z.x >= 0.0 ? 1.0f : -1.0f;
       const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
       const float sn = static_cast<float>(z.y);
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
       const float Ci = fmaf(s, Di, fmaf(hl, Ui, ci[0]));
       const float Sr = sn * Ur, Si = sn * Ui;
       bin.dst[static_cast<int64_t>(j) * bin.stride] = {Cr - Si, Sr + Ci};
-      if (mirror && jp > j) bin.dst[static_cast<int64_t>(jp) * bin.stride] = {Cr + Si, Ci - Sr};
+      if (mirror) bin.dst[static_cast<int64_t>(jp) * bin.stride] = {Cr + Si, Ci - Sr};
     }
     z = cmul(z, g);
   }
@@ -747,20 +751,6 @@ int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_
   return ZDT_OK;
 }
 
-// P when the rates r0 + j dr (j < Nr) pair up as r_j = -r_{P-j} for more than
-// a third of them (P = -2 r0/dr, an integer; the Reinsch evaluation of a
-// pair costs 3/4 of two Horner evaluations, of a lone rate 3/2 of one), else -1
-static int mirror_pairs(float r0, float dr, int64_t Nr) {
-  if (dr == 0.0f) return -1;
-  const double P = -2.0 * static_cast<double>(r0) / static_cast<double>(dr);
-  if (!std::isfinite(P) || P != std::rint(P) || P < 1.0 || P > 2.0 * static_cast<double>(Nr)) return -1;
-  const int64_t Pi = static_cast<int64_t>(P);
-  int64_t npairs = 0;
-  for (int64_t j = 0; j < Nr; ++j)
-    if (Pi - j > j && Pi - j < Nr) ++npairs;
-  return 3 * npairs > Nr ? static_cast<int>(Pi) : -1;
-}
-
 int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
                  const float* window, float* dest, double scale, hipStream_t st, double* colmean, double* colvar) {
   if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
@@ -781,10 +771,13 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, fl
   c.wkind = wkind;
   c.wcustom = window;
   c.scale = 1.0f;
-  c.pair = mirror_pairs(r0, dr, p->Nr);
+  // rates in mirrored pairs around the centre rate (Reinsch) from Nr = 2 on
+  const double rc = static_cast<double>(r0) + 0.5 * static_cast<double>(p->Nr - 1) * static_cast<double>(dr);
+  c.hc = rc / static_cast<double>(p->Nf);
+  c.hz = (static_cast<double>(r0) - rc) / static_cast<double>(p->Nf);
   {
     const unsigned grid = static_cast<unsigned>(p->M / 256 + 1);
-    if (c.pair >= 0) {
+    if (p->Nr >= 2) {
       ProfScope ps(st, "czt_reinsch_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
       if (Nt <= 16) {
         czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);

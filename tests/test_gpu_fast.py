@@ -54,12 +54,15 @@ def test_fast_sizes(F, Nf, Nt):
 @pytest.mark.parametrize("Nf,r0,dr,Nr,Nt", [(1 << 17, -3.75, 0.5, 16, 16), (1 << 17, -4.0, 0.25, 33, 16),
                                            (1 << 17, 2.0, -0.25, 17, 11), (1 << 17, -0.5, 1.0, 2, 16),
                                            (1 << 17, -1.0, 1.0, 3, 13), (1 << 17, -1.0, 0.5, 5, 16),
-                                           (1 << 19, -2.0, 0.125, 33, 16), (1 << 19, -1.25, 0.25, 11, 14)])
+                                           (1 << 19, -2.0, 0.125, 33, 16), (1 << 19, -1.25, 0.25, 11, 14),
+                                           (1 << 17, 0.0, -0.25, 21, 16), (1 << 17, 0.5, 0.25, 20, 12),
+                                           (1 << 18, -4.0, 1.0 / 60, 481, 16), (1 << 19, 1.0, 0.0625, 64, 16)])
 def test_fast_paired_rates(F, Nf, r0, dr, Nr, Nt):
-    """Symmetric rate sets (r_j = -r_{Nr-1-j}) take k2's paired evaluation:
-    even Nr, odd Nr (a middle rate), descending rates, one pair, one pair and
-    the middle, a last group holding only the middle rate, and at 2^19 the
-    16-row groups (4 pairs) with partial first groups."""
+    """k2 pairs rates j and Nr-1-j around the centre rate: even Nr, odd Nr (a
+    middle rate), descending rates, one pair, one pair and the middle, a last
+    group holding only the middle rate, 16-row groups (4 pairs) at 2^19, and
+    rate sets not symmetric about 0 (all negative, all positive, a
+    non-dyadic step)."""
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.23, 0.75, 30.0), (Nf * 0.66, -1.5, 25.0)], seed=Nr + Nt)
     say(f"fast paired r0={r0} dr={dr} Nr={Nr} Nt={Nt}")
     out = host(F.zdtfdr(F.ZDTWorkspace(s, r0, dr, Nr)))
@@ -177,10 +180,9 @@ def test_fused_output_window_nodest_unnormalized(F):
 @pytest.mark.parametrize("r0,dr,Nr", [(-2.0, 0.125, 45), (-1.0, 0.125, 40), (0.5, 0.25, 20), (3.0, -0.25, 25),
                                      (-0.375, 0.25, 4), (-0.3, 0.25, 20)])
 def test_fused_output_rate_sets(F, r0, dr, Nr):
-    """Rate sets r_j = r0 + j dr with pairs r_j = -r_{P-j}, P = -2 r0/dr:
-    pairs and lone rates (P = 32 of 45), too few pairs for the paired
-    evaluation (P = 16 of 40), no pairs, descending rates (P = 24), all
-    paired (P = 3 of 4), and a P that is not an integer."""
+    """The paired evaluation around the centre rate on rate sets that are
+    not symmetric about 0 (shifted, all positive, descending, a
+    non-dyadic step) and on a symmetric one of 4 rates."""
     Nf, Nt = 1 << 17, 24
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.41, 1.75, 25.0), (Nf * 0.77, -0.5, 30.0)], seed=Nr)
     say(f"fused output rates r0={r0} dr={dr} Nr={Nr}")
