@@ -2,7 +2,7 @@
 // chirp-Z coefficients of a bin do not fit k2's registers (Nt > 16) or the
 // rows do not fit its LDS (Nf > 2^20):
 //   k1_input + k2_input   input!: the four-step r2c into Ft[t][k2][k1]
-//   czt_horner_col / czt_reinsch_col   the chirp-Z per bin -> Yc[j][k2][k1]
+//   czt_reinsch_col       the chirp-Z per bin -> Yc[j][k2][k1]
 //   k2_c2r / k2_c2r_wave  c2r pre-pass and inverse DFT_N1 -> B (as k2)
 //   k3_output             as in the fused path (fast_output)
 #include <hip/hip_runtime.h>
@@ -19,12 +19,12 @@ namespace zdt {
 // ---------------------------------------------------------------- fused output for Nt > 16
 //
 // For power-of-two Nf with Nt up to 64 (more coefficients than k2 keeps in
-// registers) the chirp-Z runs per bin (czt_horner_col, the Vandermonde form
-// of czt_horner) and the output irfft reuses the second half of k2 and k3:
-//   czt_horner_col  Ys[k, j] -> Yc[j][k2][k1] (k = k2 + N2 k1 < M), Ys[M, j] -> yM[j]
-//   k2_c2r          per Hermitian column pair: c2r pre-pass, inverse DFT_N1
-//                   over k1 (one wavefront per row) -> B, bin M/2 -> yhalf
-//   k3_output       as in the fused path
+// registers) the chirp-Z runs per bin (czt_reinsch_col) and the output irfft
+// reuses the second half of k2 and k3:
+//   czt_reinsch_col  Ys[k, j] -> Yc[j][k2][k1] (k = k2 + N2 k1 < M), Ys[M, j] -> yM[j]
+//   k2_c2r           per Hermitian column pair: c2r pre-pass, inverse DFT_N1
+//                    over k1 (one wavefront per row) -> B, bin M/2 -> yhalf
+//   k3_output        as in the fused path
 // which moves the spectra through HBM twice instead of once per FFT pass.
 
 struct CztColArgs {
@@ -38,7 +38,6 @@ struct CztColArgs {
   int logN1, logN2;
   int Nt, Nr;
   double hd2;    // dr/Nf
-  double h0;     // r0/Nf
   int wkind;
   const float* wcustom;
   float scale;
@@ -77,47 +76,6 @@ __device__ __forceinline__ bool col_bin(const CztColArgs& a, ColBin& b) {
     b.fs = a.ftrans ? a.M : a.ldF;
   }
   return true;
-}
-
-template <int NT>
-__global__ __launch_bounds__(256) void czt_horner_col(CztColArgs a) {
-  constexpr int B = NT < 16 ? NT : 16;
-  constexpr int NB = NT / B;
-  ColBin bin;
-  if (!col_bin(a, bin)) return;
-  const int64_t k = bin.k, fo = bin.fo, fs = bin.fs, stride = bin.stride;
-  c32* dst = bin.dst;
-  const double kd = static_cast<double>(k);
-  const float w = fast_window(a.wkind, a.wcustom, k, a.Nf) * a.scale;
-  c32 c[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    if (t < a.Nt) {
-      const c32 f = a.F[fo + static_cast<int64_t>(t) * fs];
-      const c32 r = cmul(f, cis_nh(kd * static_cast<double>(t), a.h0));
-      c[t] = {r.x * w, r.y * w};
-    } else {
-      c[t] = {0.0f, 0.0f};
-    }
-  }
-  const c64 g = cis64f(kd * a.hd2);
-  const c64 gB = cis64f(static_cast<double>(B) * kd * a.hd2);
-  c64 z = {1.0, 0.0}, zB = {1.0, 0.0};
-  for (int j = 0; j < a.Nr; ++j) {
-    const c32 zf = {static_cast<float>(z.x), static_cast<float>(z.y)};
-    const c32 zBf = {static_cast<float>(zB.x), static_cast<float>(zB.y)};
-    c32 acc = {0.0f, 0.0f};
-#pragma unroll
-    for (int b = NB - 1; b >= 0; --b) {
-      c32 q = c[b * B + B - 1];
-#pragma unroll
-      for (int u = B - 2; u >= 0; --u) q = cfma(c[b * B + u], q, zf);
-      acc = (b == NB - 1) ? q : cfma(q, acc, zBf);
-    }
-    dst[static_cast<int64_t>(j) * stride] = acc;
-    z = cmul(z, g);
-    zB = cmul(zB, gB);
-  }
 }
 
 // The chirp-Z of a bin for the rates in mirrored pairs: with the centre rate
@@ -767,37 +725,24 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, fl
   c.Nt = Nt;
   c.Nr = static_cast<int>(p->Nr);
   c.hd2 = static_cast<double>(dr) / static_cast<double>(p->Nf);
-  c.h0 = static_cast<double>(r0) / static_cast<double>(p->Nf);
   c.wkind = wkind;
   c.wcustom = window;
   c.scale = 1.0f;
-  // rates in mirrored pairs around the centre rate (Reinsch) from Nr = 2 on
+  // rates in mirrored pairs around the centre rate (a single rate is its own partner)
   const double rc = static_cast<double>(r0) + 0.5 * static_cast<double>(p->Nr - 1) * static_cast<double>(dr);
   c.hc = rc / static_cast<double>(p->Nf);
   c.hz = (static_cast<double>(r0) - rc) / static_cast<double>(p->Nf);
   {
     const unsigned grid = static_cast<unsigned>(p->M / 256 + 1);
-    if (p->Nr >= 2) {
-      ProfScope ps(st, "czt_reinsch_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
-      if (Nt <= 16) {
-        czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);
-      } else if (Nt <= 32) {
-        czt_reinsch_col<32><<<grid, 256, 0, st>>>(c);
-      } else {
-        czt_reinsch_col<64><<<grid, 256, 0, st>>>(c);
-      }
-      ZDT_CHECK_LAUNCH("czt_reinsch_col");
+    ProfScope ps(st, "czt_reinsch_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
+    if (Nt <= 16) {
+      czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);
+    } else if (Nt <= 32) {
+      czt_reinsch_col<32><<<grid, 256, 0, st>>>(c);
     } else {
-      ProfScope ps(st, "czt_horner_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
-      if (Nt <= 16) {
-        czt_horner_col<16><<<grid, 256, 0, st>>>(c);
-      } else if (Nt <= 32) {
-        czt_horner_col<32><<<grid, 256, 0, st>>>(c);
-      } else {
-        czt_horner_col<64><<<grid, 256, 0, st>>>(c);
-      }
-      ZDT_CHECK_LAUNCH("czt_horner_col");
+      czt_reinsch_col<64><<<grid, 256, 0, st>>>(c);
     }
+    ZDT_CHECK_LAUNCH("czt_reinsch_col");
   }
   K2cArgs a;
   a.Yc = p->Yc;

@@ -178,11 +178,11 @@ def test_fused_output_window_nodest_unnormalized(F):
 
 
 @pytest.mark.parametrize("r0,dr,Nr", [(-2.0, 0.125, 45), (-1.0, 0.125, 40), (0.5, 0.25, 20), (3.0, -0.25, 25),
-                                     (-0.375, 0.25, 4), (-0.3, 0.25, 20)])
+                                     (-0.375, 0.25, 4), (-0.3, 0.25, 20), (1.5, 0.25, 1), (-4.0, 1.0 / 60, 61)])
 def test_fused_output_rate_sets(F, r0, dr, Nr):
     """The paired evaluation around the centre rate on rate sets that are
-    not symmetric about 0 (shifted, all positive, descending, a
-    non-dyadic step) and on a symmetric one of 4 rates."""
+    not symmetric about 0 (shifted, all positive, descending, non-dyadic
+    steps), on a symmetric one of 4 rates and on a single rate."""
     Nf, Nt = 1 << 17, 24
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.41, 1.75, 25.0), (Nf * 0.77, -0.5, 30.0)], seed=Nr)
     say(f"fused output rates r0={r0} dr={dr} Nr={Nr}")
