@@ -183,10 +183,10 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_count(const float* __restric
 }
 
 // Exclusive scan of the tile counts in three steps: each block of
-// SCAN_THREADS scans SCAN_CHUNK counts locally and writes its total
+// SCAN_THREADS scans SCAN_CHUNK (2048) counts locally and writes its total
 // (hits_scan_local), one block scans the totals (hits_scan), and the totals'
 // offsets are added back (hits_scan_add).
-constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER_THREAD = 8;
 constexpr int64_t SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
 
@@ -361,19 +361,26 @@ int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld,
 
 // fdrdenormalize on the device (src/fdrutils.jl:104-119 with :39-50):
 // thr = snr * s + m, s = sqrt(min_j colvar[j]) (NaN if any is NaN),
-// m = colmean[0]; the minimum is taken in index order
-__global__ void hits_threshold(const double* __restrict__ colmean, const double* __restrict__ colvar, int64_t Nr,
-                               double snr, double* __restrict__ thr) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// m = colmean[0]; one wavefront, the minimum taken lane-wise then across the
+// lanes (min is exact, so the order does not matter)
+__global__ __launch_bounds__(64) void hits_threshold(const double* __restrict__ colmean,
+                                                     const double* __restrict__ colvar, int64_t Nr, double snr,
+                                                     double* __restrict__ thr) {
   double vmin = INFINITY;
-  bool nan = false;
-  for (int64_t j = 0; j < Nr; ++j) {
+  int nan = 0;
+  for (int64_t j = threadIdx.x; j < Nr; j += 64) {
     const double v = colvar[j];
-    if (v != v) nan = true;
+    if (v != v) nan = 1;
     vmin = fmin(vmin, v);
   }
-  const double sd = nan ? NAN : sqrt(fmax(vmin, 0.0));
-  *thr = snr * sd + colmean[0];
+  for (int off = 32; off > 0; off >>= 1) {
+    vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
+    nan |= __shfl_xor(nan, off, 64);
+  }
+  if (threadIdx.x == 0) {
+    const double sd = nan ? NAN : sqrt(fmax(vmin, 0.0));
+    *thr = snr * sd + colmean[0];
+  }
 }
 
 __global__ void set_double(double* p, double v) {
