@@ -311,6 +311,12 @@ int run_impl(zdt_ws* ws, float r0, int wkind, const float* window, float* dest) 
   a.Nr = static_cast<int>(ws->Nr);
   a.hd = ws->hd;
   a.h0 = static_cast<double>(r0) / static_cast<double>(ws->Nf);
+  {
+    // rates in mirrored pairs around the centre rate (a single rate is its own partner)
+    const double rc = static_cast<double>(r0) + 0.5 * static_cast<double>(ws->Nr - 1) * static_cast<double>(ws->dr);
+    a.hc = rc / static_cast<double>(ws->Nf);
+    a.hz = (static_cast<double>(r0) - rc) / static_cast<double>(ws->Nf);
+  }
   a.wkind = wkind;
   a.wcustom = window;
   a.scale = 1.0f;

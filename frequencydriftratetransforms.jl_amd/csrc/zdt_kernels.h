@@ -15,12 +15,14 @@ struct CztArgs {
   int Nt, Nr;
   double hd;  // dr/(2 Nf): chirp(k,l) = exp(-2 pi i k l^2 hd)
   double h0;  // r0/Nf:     pre(k,t) = exp(2 pi i (k t^2 hd + k t h0))
+  double hc;  // czt_reinsch: rc/Nf, rc = r0 + (Nr-1) dr/2 the centre rate
+  double hz;  // czt_reinsch: (r0 - rc)/Nf
   int wkind;
   const float* wcustom;
   float scale;
 };
 
-// F -> Ys by czt_horner (Nt <= 64) or czt_direct (any Nt)
+// F -> Ys by czt_reinsch (Nt <= 64) or czt_direct (any Nt)
 int launch_czt(const CztArgs& a, hipStream_t st);
 int launch_czt_direct(const CztArgs& a, hipStream_t st);
 // fftfdr: Ys[k,j] = sum_t F[k,t] e^{2 pi i k t rates[j]/Nf} (rates_over_nf: device, Nr doubles)

@@ -8,7 +8,7 @@
 // The reference evaluates the convolution over t as a length-Nl circular
 // convolution through FFTs (:249-253); the identity chirp(k,l) = chirp(k,-l)
 // makes the high lags of V (:182-184) the negative lags, so the result equals
-// the linear convolution above for any Nl >= Nt+Nr-1.  czt_horner (Nt <= 64)
+// the linear convolution above for any Nl >= Nt+Nr-1.  czt_reinsch (Nt <= 64)
 // and czt_direct evaluate it directly (Nt complex MACs per output), which
 // costs less than the two length-Nl FFTs, needs no V, and is exact up to
 // float rounding of the products.  The step functions (preprocess!,
@@ -127,70 +127,85 @@ __global__ __launch_bounds__(256) void czt_direct(CztArgs a) {
 // Vandermonde form for Nt <= 64: by Bluestein's identity the sum above is
 //   Ys[k,j] = w(k) sum_t a_t z_j^t,  a_t = F[k,t] e^{2 pi i k t r0/Nf},
 //   z_j = gamma^j, gamma = e^{2 pi i k dr/Nf},
-// a polynomial in z_j.  A thread keeps its bin's a_t in registers and writes
-// every Ys[k,j] once (czt_direct re-reads and re-writes Ys once per chunk of
-// 16 time steps).  z_j and z_j^16 come from FP64 recurrences and are rounded
-// to float once per rate; the polynomial is evaluated as 16-term Horner
-// blocks combined by Horner in z^16, so no power of a rounded z above the
-// 15th enters the sum.
+// a polynomial in z_j.  A thread keeps its bin's coefficients in registers
+// and writes every Ys[k,j] once (czt_direct re-reads and re-writes Ys once per
+// chunk of 16 time steps).  The rates are taken in mirrored pairs: with the
+// centre rate rc = r0 + (Nr-1) dr/2 folded into the coefficients
+// (c_t = F[k,t] e^{2 pi i k t rc/Nf}), rates j and Nr-1-j are the sums at
+// z = e^{i theta} and conj(z), theta = 2 pi k (r0 + j dr - rc)/Nf, and one
+// Reinsch-modified Goertzel recurrence over the real and the imaginary parts
+// gives both (C. Reinsch, 1967; J. Oliver, "An error analysis of the modified
+// Clenshaw method", 1977).  With s = sign(cos theta), lambda = 2 (cos theta - s),
+//   D_t = c_t + lambda U_{t+1} + s D_{t+1},  U_t = D_t + s U_{t+1}  (t = Nt-1 .. 1),
+//   sum c_t cos(t theta) = c_0 + (lambda/2) U_1 + s D_1,  sum c_t sin(t theta) = U_1 sin theta,
+// six FMAs per term for the two rates against eight for two complex Horner
+// evaluations.  z comes from an FP64 recurrence started at an exactly
+// evaluated value, and lambda from the FP64 cos theta, so no accuracy is lost
+// near theta = 0 or pi.  The middle rate of an odd Nr is its own partner.
 template <int NT>
-__global__ __launch_bounds__(256) void czt_horner(CztArgs a) {
-  constexpr int B = NT < 16 ? NT : 16;  // block length
-  constexpr int NB = NT / B;
+__global__ __launch_bounds__(256) void czt_reinsch(CztArgs a) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= a.Nk) return;
   const double kd = static_cast<double>(k);
   const float w = window_value(a.wkind, a.wcustom, k, a.Nf) * a.scale;
-  c32 c[NT];
+  float cr[NT], ci[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     if (t < a.Nt) {
-      const c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
-      const c32 r = cmul(f, cis_nh(kd * static_cast<double>(t), a.h0));
-      c[t] = {r.x * w, r.y * w};
+      c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
+      if (a.hc != 0.0) f = cmul(f, cis_nh(kd * static_cast<double>(t), a.hc));  // uniform
+      cr[t] = f.x * w;
+      ci[t] = f.y * w;
     } else {
-      c[t] = {0.0f, 0.0f};
+      cr[t] = 0.0f;
+      ci[t] = 0.0f;
     }
   }
-  const c64 g = cis64(2.0 * kd * a.hd);           // gamma = e^{2 pi i k dr/Nf}
-  const c64 gB = cis64(2.0 * B * kd * a.hd);      // gamma^B
-  c64 z = {1.0, 0.0}, zB = {1.0, 0.0};
+  const c64 g = cis64(2.0 * kd * a.hd);  // gamma = e^{2 pi i k dr/Nf}
+  c64 z = cis64(kd * a.hz);              // e^{2 pi i k (r0 - rc)/Nf}
   c32* dst = a.Ys + k;
-  for (int j = 0; j < a.Nr; ++j) {
-    const c32 zf = {static_cast<float>(z.x), static_cast<float>(z.y)};
-    const c32 zBf = {static_cast<float>(zB.x), static_cast<float>(zB.y)};
-    c32 acc = {0.0f, 0.0f};
+  const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
+  for (int j = 0; j < npair; ++j) {
+    const int jp = a.Nr - 1 - j;
+    const float s = z.x >= 0.0 ? 1.0f : -1.0f;
+    const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
+    const float sn = static_cast<float>(z.y);
+    float Dr = cr[NT - 1], Ur = cr[NT - 1], Di = ci[NT - 1], Ui = ci[NT - 1];
 #pragma unroll
-    for (int b = NB - 1; b >= 0; --b) {
-      c32 p = c[b * B + B - 1];
-#pragma unroll
-      for (int u = B - 2; u >= 0; --u) p = cfma(c[b * B + u], p, zf);
-      acc = (b == NB - 1) ? p : cfma(p, acc, zBf);
+    for (int t = NT - 2; t >= 1; --t) {
+      Dr = fmaf(s, Dr, fmaf(lam, Ur, cr[t]));
+      Ur = fmaf(s, Ur, Dr);
+      Di = fmaf(s, Di, fmaf(lam, Ui, ci[t]));
+      Ui = fmaf(s, Ui, Di);
     }
-    dst[static_cast<int64_t>(j) * a.ldY] = acc;
+    const float hl = 0.5f * lam;
+    const float Cr = fmaf(s, Dr, fmaf(hl, Ur, cr[0]));
+    const float Ci = fmaf(s, Di, fmaf(hl, Ui, ci[0]));
+    const float Sr = sn * Ur, Si = sn * Ui;
+    dst[static_cast<int64_t>(j) * a.ldY] = {Cr - Si, Sr + Ci};
+    if (jp != j) dst[static_cast<int64_t>(jp) * a.ldY] = {Cr + Si, Ci - Sr};
     z = cmul(z, g);
-    zB = cmul(zB, gB);
   }
 }
 
 int launch_czt(const CztArgs& a, hipStream_t st) {
   if (a.Nt > 64) return launch_czt_direct(a, st);
   const unsigned grid = static_cast<unsigned>((a.Nk + 255) / 256);
-  ProfScope ps(st, "czt_horner", 8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr));
+  ProfScope ps(st, "czt_reinsch", 8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr));
   if (a.Nt <= 2) {
-    czt_horner<2><<<grid, 256, 0, st>>>(a);
+    czt_reinsch<2><<<grid, 256, 0, st>>>(a);
   } else if (a.Nt <= 4) {
-    czt_horner<4><<<grid, 256, 0, st>>>(a);
+    czt_reinsch<4><<<grid, 256, 0, st>>>(a);
   } else if (a.Nt <= 8) {
-    czt_horner<8><<<grid, 256, 0, st>>>(a);
+    czt_reinsch<8><<<grid, 256, 0, st>>>(a);
   } else if (a.Nt <= 16) {
-    czt_horner<16><<<grid, 256, 0, st>>>(a);
+    czt_reinsch<16><<<grid, 256, 0, st>>>(a);
   } else if (a.Nt <= 32) {
-    czt_horner<32><<<grid, 256, 0, st>>>(a);
+    czt_reinsch<32><<<grid, 256, 0, st>>>(a);
   } else {
-    czt_horner<64><<<grid, 256, 0, st>>>(a);
+    czt_reinsch<64><<<grid, 256, 0, st>>>(a);
   }
-  ZDT_CHECK_LAUNCH("czt_horner");
+  ZDT_CHECK_LAUNCH("czt_reinsch");
   return ZDT_OK;
 }
 
@@ -198,8 +213,10 @@ int launch_czt(const CztArgs& a, hipStream_t st) {
 // fftfdr (src/fftfdr.jl:86-96,126-148) for an arbitrary list of rates:
 //   Ys[k,j] = sum_t F[k,t] e^{2 pi i k t rates[j]/Nf}.
 // Per bin and rate z = e^{2 pi i k rates[j]/Nf} and z^16 are evaluated
-// directly (k*rate/Nf reduced in FP64), then the same blocked Horner as
-// czt_horner.  For Nt > 64 every term's phase is evaluated directly.
+// directly (k*rate/Nf reduced in FP64), and the polynomial is evaluated as
+// 16-term Horner blocks combined by Horner in z^16, so no power of a rounded
+// z above the 15th enters the sum.  For Nt > 64 every term's phase is
+// evaluated directly.
 template <int NT>
 __global__ __launch_bounds__(256) void czt_rates(const c32* __restrict__ F, int64_t ldF, c32* __restrict__ Ys,
                                                  int64_t ldY, int64_t Nk, int64_t Nf, int Nt,
