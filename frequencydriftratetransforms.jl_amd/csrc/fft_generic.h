@@ -12,6 +12,13 @@ struct FftPlan {
   bool dbl = false;
   std::vector<int> radices;  // passes, in order
   void* tw = nullptr;        // device table W_N^m, m < N (c32 or c64)
+  // four-step split N = N1 * N2 of the FP32 contiguous transforms (N1 = 0: none):
+  // N1-point column DFTs and N2-point row DFTs, each in LDS, TC columns / TR rows
+  // per block; twiddles W_N^m = thi[m >> 10] * tlo[m & 1023]
+  int N1 = 0, N2 = 0, TC = 0, TR = 0;
+  std::vector<int> rad1, rad2;
+  c32* tlo = nullptr;
+  c32* thi = nullptr;
 };
 
 // Factor N into the supported radices, largest power-of-two radix first and

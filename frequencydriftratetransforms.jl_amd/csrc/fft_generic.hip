@@ -18,10 +18,12 @@
 // the transform axis is contiguous, along the batch otherwise.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "fft_butterflies.h"
@@ -128,6 +130,253 @@ __global__ __launch_bounds__(256) void scale_copy(const cplx<T>* __restrict__ in
   }
 }
 
+// ---------------------------------------------------------------- four-step transforms
+//
+// For FP32 transforms along a contiguous axis with N = N1 * N2 (N1 <= 1024,
+// N2 <= 2048, N <= 2^20) the passes are grouped into two kernels that each
+// move the data through HBM once (the "four-step" FFT, D. H. Bailey, "FFTs in
+// external or hierarchical memory", 1990):
+//   fs_cols  X1[n2 + N2 k1] = W_N^{n2 k1} sum_{n1} x[n2 + N2 n1] W_N1^{n1 k1}
+//            (TC adjacent columns n2 per block, so loads and stores are
+//            TC-element runs)
+//   fs_rows  X[k1 + N1 k2] = sum_{n2} X1[n2 + N2 k1] W_N2^{n2 k2}
+//            (TR adjacent rows k1 per block; the stores are TR-element runs)
+// Inside a block the sub-transforms run as Stockham passes on one LDS buffer
+// of up to FS_E elements: each pass reads its butterflies' inputs into
+// registers, synchronizes, and writes the outputs in place.  Twiddles are
+// W_N^m = thi[m >> 10] * tlo[m & 1023] from two 1024-entry tables held in LDS
+// (the sub-transform twiddle W_L^e is W_N^{e N/L}).  The inverse is
+// conj(forward(conj(x))): fs_cols conjugates on load, fs_rows on store.
+
+constexpr int FS_T = 512;    // threads per block
+constexpr int FS_E = 8192;   // elements per block (TC * N1 or TR * N2)
+constexpr int FS_LGB = 10;   // table split
+constexpr int FS_TAB = 1 << FS_LGB;
+constexpr int FS_MAXP = 12;  // passes per sub-transform
+
+struct FsArgs {
+  const c32* in;
+  int64_t ibs;
+  c32* out;
+  int64_t obs;
+  const c32* tlo;
+  const c32* thi;
+  int N, N1, N2, T;  // T = TC (fs_cols) or TR (fs_rows)
+  int nhi;           // entries of thi
+  int np;            // passes of the sub-transform
+  int rad[FS_MAXP];
+  int inverse;
+  float scale;
+};
+
+__device__ __forceinline__ c32 fs_tw(const c32* tl, const c32* th, int m) {
+  return cmul(th[m >> FS_LGB], tl[m & (FS_TAB - 1)]);
+}
+
+// One Stockham pass of radix R over T transforms of length L held in buf,
+// element n of transform c at n * es + c * ts.  Butterfly u covers transform
+// c and index j < L/R: c fast (cfast, es = T) or j fast (es = 1).
+template <int R>
+__device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, int L, int T, bool cfast,
+                                        int es, int ts, int Ns, int mstep) {
+  constexpr int NB = (FS_E / R + FS_T - 1) / FS_T;
+  const int Lr = L / R;
+  const int nbf = Lr * T;
+  // opaque to the optimizer, so that no index below is hoisted out of the
+  // pass loop (which would keep every pass's indices live and spill)
+  int tid = static_cast<int>(threadIdx.x);
+  asm volatile("" : "+v"(tid));
+  c32 v[NB][R];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int u = tid + i * FS_T;
+    if (u < nbf) {
+      const int j = cfast ? u / T : u % Lr;
+      const int c = cfast ? u % T : u / Lr;
+      const int base = j * es + c * ts;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[base + r * Lr * es];
+      const int k = j % Ns;
+      if (k != 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], fs_tw(tl, th, r * k * mstep));
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int u = tid + i * FS_T;
+    if (u < nbf) {
+      const int j = cfast ? u / T : u % Lr;
+      const int c = cfast ? u % T : u / Lr;
+      const int k = j % Ns;
+      dft<float, R>(v[i]);
+      const int ob = ((j - k) * R + k) * es + c * ts;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[ob + r * Ns * es] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// all passes of the sub-transform of length L (natural order in, natural out)
+__device__ __forceinline__ void fs_dft(c32* buf, const c32* tl, const c32* th, const FsArgs& a, int L, bool cfast,
+                                       int es, int ts) {
+  int Ns = 1;
+  for (int p = 0; p < a.np; ++p) {
+    const int R = a.rad[p];
+    const int mstep = a.N / (Ns * R);  // W_{Ns R}^{r k} = W_N^{r k mstep}
+    switch (R) {
+      case 2: fs_pass<2>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 3: fs_pass<3>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 4: fs_pass<4>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 5: fs_pass<5>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 7: fs_pass<7>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 8: fs_pass<8>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 11: fs_pass<11>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 13: fs_pass<13>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      default: fs_pass<16>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+    }
+    Ns *= R;
+  }
+}
+
+__device__ __forceinline__ void fs_tables(c32* tl, c32* th, const FsArgs& a) {
+  for (int i = threadIdx.x; i < FS_TAB; i += FS_T) tl[i] = a.tlo[i];
+  for (int i = threadIdx.x; i < a.nhi; i += FS_T) th[i] = a.thi[i];
+}
+
+// columns: block = (batch b, columns n2 in [c0, c0 + TC)); TC is a power of two
+__global__ __launch_bounds__(FS_T, 4) void fs_cols(FsArgs a) {
+  __shared__ c32 buf[FS_E];
+  __shared__ c32 tl[FS_TAB], th[FS_TAB];
+  fs_tables(tl, th, a);
+  const int TC = a.T;
+  const int tiles = (a.N2 + TC - 1) / TC;
+  const int64_t b = blockIdx.x / tiles;
+  const int c0 = static_cast<int>(blockIdx.x % tiles) * TC;
+  const int E = a.N1 * TC;
+  const c32* src = a.in + b * a.ibs + c0;
+  for (int e = threadIdx.x; e < E; e += FS_T) {
+    const int c = e & (TC - 1), n = e / TC;
+    c32 v = {0.0f, 0.0f};
+    if (c0 + c < a.N2) v = src[c + static_cast<int64_t>(n) * a.N2];
+    if (a.inverse) v.y = -v.y;
+    buf[e] = v;  // element n of column c at n * TC + c
+  }
+  __syncthreads();
+  fs_dft(buf, tl, th, a, a.N1, true, TC, 1);
+  c32* dst = a.out + b * static_cast<int64_t>(a.N) + c0;
+  for (int e = threadIdx.x; e < E; e += FS_T) {
+    const int c = e & (TC - 1), k1 = e / TC;
+    if (c0 + c < a.N2) dst[c + static_cast<int64_t>(k1) * a.N2] = cmul(buf[e], fs_tw(tl, th, (c0 + c) * k1));
+  }
+}
+
+// rows: block = (batch b, rows k1 in [r0, r0 + TR)); TR is a power of two
+__global__ __launch_bounds__(FS_T, 4) void fs_rows(FsArgs a) {
+  __shared__ c32 buf[FS_E];
+  __shared__ c32 tl[FS_TAB], th[FS_TAB];
+  fs_tables(tl, th, a);
+  const int TR = a.T;
+  const int tiles = (a.N1 + TR - 1) / TR;
+  const int64_t b = blockIdx.x / tiles;
+  const int r0 = static_cast<int>(blockIdx.x % tiles) * TR;
+  const int nrow = min(TR, a.N1 - r0);
+  const c32* src = a.in + b * static_cast<int64_t>(a.N) + static_cast<int64_t>(r0) * a.N2;
+  const int E = a.N2 * TR;
+  for (int e = threadIdx.x; e < E; e += FS_T) buf[e] = e < nrow * a.N2 ? src[e] : c32{0.0f, 0.0f};
+  __syncthreads();
+  fs_dft(buf, tl, th, a, a.N2, false, 1, a.N2);  // element n of row c at n + c * N2
+  c32* dst = a.out + b * a.obs + r0;
+  for (int e = threadIdx.x; e < E; e += FS_T) {
+    const int c = e & (TR - 1), k2 = e / TR;
+    if (c < nrow) {
+      c32 v = buf[k2 + c * a.N2];
+      v = {v.x * a.scale, (a.inverse ? -v.y : v.y) * a.scale};
+      dst[c + static_cast<int64_t>(k2) * a.N1] = v;
+    }
+  }
+}
+
+static int fs_c2c(const FftPlan* p, const c32* in, int64_t ibs, c32* out, int64_t obs, int64_t batch, bool inverse,
+                  double scale, c32* s0, hipStream_t st) {
+  FsArgs a;
+  a.tlo = p->tlo;
+  a.thi = p->thi;
+  a.N = static_cast<int>(p->N);
+  a.N1 = p->N1;
+  a.N2 = p->N2;
+  a.nhi = static_cast<int>((p->N + FS_TAB - 1) / FS_TAB);
+  a.inverse = inverse ? 1 : 0;
+  const double bytes = 16.0 * static_cast<double>(p->N) * static_cast<double>(batch);
+  {
+    a.in = in;
+    a.ibs = ibs;
+    a.out = s0;
+    a.obs = p->N;
+    a.T = p->TC;
+    a.np = static_cast<int>(p->rad1.size());
+    for (int i = 0; i < a.np; ++i) a.rad[i] = p->rad1[i];
+    a.scale = 1.0f;
+    const int64_t grid = (p->N2 + p->TC - 1) / p->TC * batch;
+    ProfScope ps(st, "fft_fs_cols", bytes);
+    fs_cols<<<static_cast<unsigned>(grid), FS_T, 0, st>>>(a);
+    ZDT_CHECK_LAUNCH("fs_cols");
+  }
+  {
+    a.in = s0;
+    a.ibs = p->N;
+    a.out = out;
+    a.obs = obs;
+    a.T = p->TR;
+    a.np = static_cast<int>(p->rad2.size());
+    for (int i = 0; i < a.np; ++i) a.rad[i] = p->rad2[i];
+    a.scale = static_cast<float>(scale);
+    const int64_t grid = (p->N1 + p->TR - 1) / p->TR * batch;
+    ProfScope ps(st, "fft_fs_rows", bytes);
+    fs_rows<<<static_cast<unsigned>(grid), FS_T, 0, st>>>(a);
+    ZDT_CHECK_LAUNCH("fs_rows");
+  }
+  return ZDT_OK;
+}
+
+static int pow2_floor(int64_t x) {
+  int r = 1;
+  while (static_cast<int64_t>(r) * 2 <= x) r *= 2;
+  return r;
+}
+
+// Chooses the four-step split of an FP32 plan (sets p->N1 = 0 if none applies):
+// the one whose column and row blocks both move the longest contiguous runs.
+static void fs_choose(FftPlan* p, int maxr) {
+  p->N1 = 0;
+  const int64_t N = p->N;
+  if (p->dbl || N < 4096 || N > (int64_t{1} << 20)) return;
+  int best = -1;
+  for (int64_t n1 = 2; n1 <= 1024; ++n1) {
+    if (N % n1 != 0) continue;
+    const int64_t n2 = N / n1;
+    if (n2 < 2 || n2 > 2048) continue;
+    std::vector<int> r1, r2;
+    if (!fft_factorize(n1, &r1, maxr) || !fft_factorize(n2, &r2, maxr)) continue;
+    if (static_cast<int>(r1.size()) > FS_MAXP || static_cast<int>(r2.size()) > FS_MAXP) continue;
+    const int tc = std::min(16, pow2_floor(FS_E / n1));
+    const int tr = std::min(16, pow2_floor(FS_E / n2));
+    const int score = std::min(tc, tr) * 100 + tc;
+    if (score > best) {
+      best = score;
+      p->N1 = static_cast<int>(n1);
+      p->N2 = static_cast<int>(n2);
+      p->TC = tc;
+      p->TR = tr;
+      p->rad1 = r1;
+      p->rad2 = r2;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- plans
 
 bool fft_factorize(int64_t N, std::vector<int>* radices, int maxr) {
@@ -206,6 +455,38 @@ int fft_plan_get(int64_t N, bool dbl, const FftPlan** out, int maxr) {
     delete p;
     return hip_fail(e, "hipMemcpy(twiddles)");
   }
+  fs_choose(p, maxr);
+  if (p->N1 > 0) {
+    // W_N^m for m < 1024 and W_N^{1024 m} for m < ceil(N/1024)
+    const int64_t nhi = (N + FS_TAB - 1) / FS_TAB;
+    std::vector<c32> lo(FS_TAB), hi(static_cast<size_t>(nhi));
+    const auto wn = [N](int64_t m) -> c32 {
+      m %= N;
+      const long double a = 2.0L * 3.14159265358979323846264338327950288L * static_cast<long double>(m) /
+                            static_cast<long double>(N);
+      double c = static_cast<double>(cosl(a)), sn = -static_cast<double>(sinl(a));
+      if (4 * m == N || 4 * m == 3 * N) c = 0.0;
+      if (2 * m == N) sn = 0.0;
+      return {static_cast<float>(c), static_cast<float>(sn)};
+    };
+    for (int64_t m = 0; m < FS_TAB; ++m) lo[static_cast<size_t>(m)] = wn(m);
+    for (int64_t m = 0; m < nhi; ++m) hi[static_cast<size_t>(m)] = wn(m * FS_TAB);
+    e = hipMalloc(&p->tlo, (FS_TAB + nhi) * sizeof(c32));
+    if (e != hipSuccess) {
+      (void)hipFree(p->tw);
+      delete p;
+      return fail(ZDT_ENOMEM, "hipMalloc of FFT twiddles failed");
+    }
+    p->thi = p->tlo + FS_TAB;
+    e = hipMemcpy(p->tlo, lo.data(), FS_TAB * sizeof(c32), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->thi, hi.data(), nhi * sizeof(c32), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(p->tlo);
+      (void)hipFree(p->tw);
+      delete p;
+      return hip_fail(e, "hipMemcpy(twiddles)");
+    }
+  }
   g_plans[key] = p;
   *out = p;
   return ZDT_OK;
@@ -218,6 +499,11 @@ static int c2c_impl(const FftPlan* p, const void* in, int64_t ies, int64_t ibs, 
   const int np = static_cast<int>(p->radices.size());
   const int64_t N = p->N;
   const cplx<T>* tw = static_cast<const cplx<T>*>(p->tw);
+  if constexpr (std::is_same<T, float>::value) {
+    if (p->N1 > 0 && ies == 1 && oes == 1)
+      return fs_c2c(p, static_cast<const c32*>(in), ibs, static_cast<c32*>(out), obs, batch, inverse, scale,
+                    static_cast<c32*>(s0), st);
+  }
   if (N == 1) {
     // the length-1 DFT is the identity (times scale)
     scale_copy<T><<<grid_for(batch, 256), 256, 0, st>>>(static_cast<const cplx<T>*>(in), ibs,
