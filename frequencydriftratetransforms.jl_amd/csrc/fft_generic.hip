@@ -173,15 +173,32 @@ __device__ __forceinline__ c32 fs_tw(const c32* tl, const c32* th, int m) {
   return cmul(th[m >> FS_LGB], tl[m & (FS_TAB - 1)]);
 }
 
+// u / d for u * d < 2^32 by one high multiply with m = ceil(2^32 / d)
+// (m = 0 marks d = 1)
+struct FsDiv {
+  uint32_t d, m;
+};
+__device__ __forceinline__ FsDiv fs_div(int d) {
+  const uint32_t ud = static_cast<uint32_t>(d);
+  return {ud, ud == 1 ? 0u : static_cast<uint32_t>(0xFFFFFFFFull / ud + 1)};
+}
+__device__ __forceinline__ int fs_quo(int u, FsDiv q) {
+  return q.m == 0 ? u : static_cast<int>(__umulhi(static_cast<uint32_t>(u), q.m));
+}
+
 // One Stockham pass of radix R over T transforms of length L held in buf,
 // element n of transform c at n * es + c * ts.  Butterfly u covers transform
-// c and index j < L/R: c fast (cfast, es = T) or j fast (es = 1).
+// c and index j < L/R: c fast (cfast, es = T, T a power of two) or j fast
+// (es = 1).  Every index is below 2^13, so the divisions are exact high
+// multiplies.
 template <int R>
 __device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, int L, int T, bool cfast,
                                         int es, int ts, int Ns, int mstep) {
   constexpr int NB = (FS_E / R + FS_T - 1) / FS_T;
   const int Lr = L / R;
   const int nbf = Lr * T;
+  const int lgT = __ffs(T) - 1;
+  const FsDiv dLr = fs_div(Lr), dNs = fs_div(Ns);
   // opaque to the optimizer, so that no index below is hoisted out of the
   // pass loop (which would keep every pass's indices live and spill)
   int tid = static_cast<int>(threadIdx.x);
@@ -191,12 +208,12 @@ __device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, 
   for (int i = 0; i < NB; ++i) {
     const int u = tid + i * FS_T;
     if (u < nbf) {
-      const int j = cfast ? u / T : u % Lr;
-      const int c = cfast ? u % T : u / Lr;
+      const int c = cfast ? (u & (T - 1)) : fs_quo(u, dLr);
+      const int j = cfast ? (u >> lgT) : u - c * Lr;
       const int base = j * es + c * ts;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[i][r] = buf[base + r * Lr * es];
-      const int k = j % Ns;
+      const int k = j - fs_quo(j, dNs) * Ns;
       if (k != 0) {
 #pragma unroll
         for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], fs_tw(tl, th, r * k * mstep));
@@ -208,9 +225,9 @@ __device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, 
   for (int i = 0; i < NB; ++i) {
     const int u = tid + i * FS_T;
     if (u < nbf) {
-      const int j = cfast ? u / T : u % Lr;
-      const int c = cfast ? u % T : u / Lr;
-      const int k = j % Ns;
+      const int c = cfast ? (u & (T - 1)) : fs_quo(u, dLr);
+      const int j = cfast ? (u >> lgT) : u - c * Lr;
+      const int k = j - fs_quo(j, dNs) * Ns;
       dft<float, R>(v[i]);
       const int ob = ((j - k) * R + k) * es + c * ts;
 #pragma unroll
@@ -258,19 +275,33 @@ __global__ __launch_bounds__(FS_T, 4) void fs_cols(FsArgs a) {
   const int c0 = static_cast<int>(blockIdx.x % tiles) * TC;
   const int E = a.N1 * TC;
   const c32* src = a.in + b * a.ibs + c0;
-  for (int e = threadIdx.x; e < E; e += FS_T) {
-    const int c = e & (TC - 1), n = e / TC;
-    c32 v = {0.0f, 0.0f};
-    if (c0 + c < a.N2) v = src[c + static_cast<int64_t>(n) * a.N2];
-    if (a.inverse) v.y = -v.y;
-    buf[e] = v;  // element n of column c at n * TC + c
+  {
+    // all loads issued before the first LDS store
+    c32 v[FS_E / FS_T];
+#pragma unroll
+    for (int i = 0; i < FS_E / FS_T; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * FS_T;
+        const int c = e & (TC - 1), n = e >> (__ffs(TC) - 1);
+      v[i] = {0.0f, 0.0f};
+      if (e < E && c0 + c < a.N2) v[i] = src[c + static_cast<int64_t>(n) * a.N2];
+    }
+#pragma unroll
+    for (int i = 0; i < FS_E / FS_T; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * FS_T;
+      if (e < E) buf[e] = {v[i].x, a.inverse ? -v[i].y : v[i].y};  // element n of column c at n * TC + c
+    }
   }
   __syncthreads();
   fs_dft(buf, tl, th, a, a.N1, true, TC, 1);
   c32* dst = a.out + b * static_cast<int64_t>(a.N) + c0;
-  for (int e = threadIdx.x; e < E; e += FS_T) {
-    const int c = e & (TC - 1), k1 = e / TC;
-    if (c0 + c < a.N2) dst[c + static_cast<int64_t>(k1) * a.N2] = cmul(buf[e], fs_tw(tl, th, (c0 + c) * k1));
+  const int lgT = __ffs(TC) - 1;
+  int tid = static_cast<int>(threadIdx.x);
+  asm volatile("" : "+v"(tid));  // keeps the store indices from being computed before the passes
+#pragma unroll
+  for (int i = 0; i < FS_E / FS_T; ++i) {
+    const int e = tid + i * FS_T;
+    const int c = e & (TC - 1), k1 = e >> lgT;
+    if (e < E && c0 + c < a.N2) dst[c + static_cast<int64_t>(k1) * a.N2] = cmul(buf[e], fs_tw(tl, th, (c0 + c) * k1));
   }
 }
 
@@ -286,13 +317,30 @@ __global__ __launch_bounds__(FS_T, 4) void fs_rows(FsArgs a) {
   const int nrow = min(TR, a.N1 - r0);
   const c32* src = a.in + b * static_cast<int64_t>(a.N) + static_cast<int64_t>(r0) * a.N2;
   const int E = a.N2 * TR;
-  for (int e = threadIdx.x; e < E; e += FS_T) buf[e] = e < nrow * a.N2 ? src[e] : c32{0.0f, 0.0f};
+  {
+    c32 v[FS_E / FS_T];
+#pragma unroll
+    for (int i = 0; i < FS_E / FS_T; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * FS_T;
+      v[i] = e < nrow * a.N2 ? src[e] : c32{0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int i = 0; i < FS_E / FS_T; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * FS_T;
+      if (e < E) buf[e] = v[i];
+    }
+  }
   __syncthreads();
   fs_dft(buf, tl, th, a, a.N2, false, 1, a.N2);  // element n of row c at n + c * N2
   c32* dst = a.out + b * a.obs + r0;
-  for (int e = threadIdx.x; e < E; e += FS_T) {
-    const int c = e & (TR - 1), k2 = e / TR;
-    if (c < nrow) {
+  const int lgT = __ffs(TR) - 1;
+  int tid = static_cast<int>(threadIdx.x);
+  asm volatile("" : "+v"(tid));  // keeps the store indices from being computed before the passes
+#pragma unroll
+  for (int i = 0; i < FS_E / FS_T; ++i) {
+    const int e = tid + i * FS_T;
+    const int c = e & (TR - 1), k2 = e >> lgT;
+    if (e < E && c < nrow) {
       c32 v = buf[k2 + c * a.N2];
       v = {v.x * a.scale, (a.inverse ? -v.y : v.y) * a.scale};
       dst[c + static_cast<int64_t>(k2) * a.N1] = v;
