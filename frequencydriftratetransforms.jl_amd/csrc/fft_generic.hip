@@ -186,19 +186,17 @@ __device__ __forceinline__ int fs_quo(int u, FsDiv q) {
   return q.m == 0 ? u : static_cast<int>(__umulhi(static_cast<uint32_t>(u), q.m));
 }
 
-// One Stockham pass of radix R over T transforms of length L held in buf,
-// element n of transform c at n * es + c * ts.  Butterfly u covers transform
-// c and index j < L/R: c fast (cfast, es = T, T a power of two) or j fast
-// (es = 1).  Every index is below 2^13, so the divisions are exact high
-// multiplies.
+// One Stockham pass of radix R over T interleaved transforms of length L
+// held in buf (element n of transform c at n * T + c, T a power of two).
+// Butterfly u covers transform c = u mod T and index j = u / T < L/R.  Every
+// index is below 2^13, so j mod Ns is an exact high multiply.
 template <int R>
-__device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, int L, int T, bool cfast,
-                                        int es, int ts, int Ns, int mstep) {
+__device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, int L, int T, int Ns, int mstep) {
   constexpr int NB = (FS_E / R + FS_T - 1) / FS_T;
   const int Lr = L / R;
   const int nbf = Lr * T;
   const int lgT = __ffs(T) - 1;
-  const FsDiv dLr = fs_div(Lr), dNs = fs_div(Ns);
+  const FsDiv dNs = fs_div(Ns);
   // opaque to the optimizer, so that no index below is hoisted out of the
   // pass loop (which would keep every pass's indices live and spill)
   int tid = static_cast<int>(threadIdx.x);
@@ -208,11 +206,9 @@ __device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, 
   for (int i = 0; i < NB; ++i) {
     const int u = tid + i * FS_T;
     if (u < nbf) {
-      const int c = cfast ? (u & (T - 1)) : fs_quo(u, dLr);
-      const int j = cfast ? (u >> lgT) : u - c * Lr;
-      const int base = j * es + c * ts;
+      const int c = u & (T - 1), j = u >> lgT;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[base + r * Lr * es];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * Lr) * T + c];
       const int k = j - fs_quo(j, dNs) * Ns;
       if (k != 0) {
 #pragma unroll
@@ -225,35 +221,34 @@ __device__ __forceinline__ void fs_pass(c32* buf, const c32* tl, const c32* th, 
   for (int i = 0; i < NB; ++i) {
     const int u = tid + i * FS_T;
     if (u < nbf) {
-      const int c = cfast ? (u & (T - 1)) : fs_quo(u, dLr);
-      const int j = cfast ? (u >> lgT) : u - c * Lr;
+      const int c = u & (T - 1), j = u >> lgT;
       const int k = j - fs_quo(j, dNs) * Ns;
       dft<float, R>(v[i]);
-      const int ob = ((j - k) * R + k) * es + c * ts;
+      const int ob = ((j - k) * R + k) * T + c;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[ob + r * Ns * es] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[ob + r * Ns * T] = v[i][r];
     }
   }
   __syncthreads();
 }
 
-// all passes of the sub-transform of length L (natural order in, natural out)
-__device__ __forceinline__ void fs_dft(c32* buf, const c32* tl, const c32* th, const FsArgs& a, int L, bool cfast,
-                                       int es, int ts) {
+// all passes of the T interleaved sub-transforms of length L (natural order in
+// and out)
+__device__ __forceinline__ void fs_dft(c32* buf, const c32* tl, const c32* th, const FsArgs& a, int L) {
   int Ns = 1;
   for (int p = 0; p < a.np; ++p) {
     const int R = a.rad[p];
     const int mstep = a.N / (Ns * R);  // W_{Ns R}^{r k} = W_N^{r k mstep}
     switch (R) {
-      case 2: fs_pass<2>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      case 3: fs_pass<3>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      case 4: fs_pass<4>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      case 5: fs_pass<5>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      case 7: fs_pass<7>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      case 8: fs_pass<8>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      case 11: fs_pass<11>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      case 13: fs_pass<13>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
-      default: fs_pass<16>(buf, tl, th, L, a.T, cfast, es, ts, Ns, mstep); break;
+      case 2: fs_pass<2>(buf, tl, th, L, a.T, Ns, mstep); break;
+      case 3: fs_pass<3>(buf, tl, th, L, a.T, Ns, mstep); break;
+      case 4: fs_pass<4>(buf, tl, th, L, a.T, Ns, mstep); break;
+      case 5: fs_pass<5>(buf, tl, th, L, a.T, Ns, mstep); break;
+      case 7: fs_pass<7>(buf, tl, th, L, a.T, Ns, mstep); break;
+      case 8: fs_pass<8>(buf, tl, th, L, a.T, Ns, mstep); break;
+      case 11: fs_pass<11>(buf, tl, th, L, a.T, Ns, mstep); break;
+      case 13: fs_pass<13>(buf, tl, th, L, a.T, Ns, mstep); break;
+      default: fs_pass<16>(buf, tl, th, L, a.T, Ns, mstep); break;
     }
     Ns *= R;
   }
@@ -292,7 +287,7 @@ __global__ __launch_bounds__(FS_T, 4) void fs_cols(FsArgs a) {
     }
   }
   __syncthreads();
-  fs_dft(buf, tl, th, a, a.N1, true, TC, 1);
+  fs_dft(buf, tl, th, a, a.N1);
   c32* dst = a.out + b * static_cast<int64_t>(a.N) + c0;
   const int lgT = __ffs(TC) - 1;
   int tid = static_cast<int>(threadIdx.x);
@@ -317,12 +312,17 @@ __global__ __launch_bounds__(FS_T, 4) void fs_rows(FsArgs a) {
   const int nrow = min(TR, a.N1 - r0);
   const c32* src = a.in + b * static_cast<int64_t>(a.N) + static_cast<int64_t>(r0) * a.N2;
   const int E = a.N2 * TR;
+  const int lgT = __ffs(TR) - 1;
   {
+    // rows interleaved as in fs_cols (element n of row c at n * TR + c), which
+    // keeps the passes' LDS accesses as conflict-free as fs_cols's
     c32 v[FS_E / FS_T];
 #pragma unroll
     for (int i = 0; i < FS_E / FS_T; ++i) {
       const int e = static_cast<int>(threadIdx.x) + i * FS_T;
-      v[i] = e < nrow * a.N2 ? src[e] : c32{0.0f, 0.0f};
+      const int c = e & (TR - 1), n = e >> lgT;
+      v[i] = {0.0f, 0.0f};
+      if (e < E && c < nrow) v[i] = src[static_cast<int64_t>(c) * a.N2 + n];
     }
 #pragma unroll
     for (int i = 0; i < FS_E / FS_T; ++i) {
@@ -331,9 +331,8 @@ __global__ __launch_bounds__(FS_T, 4) void fs_rows(FsArgs a) {
     }
   }
   __syncthreads();
-  fs_dft(buf, tl, th, a, a.N2, false, 1, a.N2);  // element n of row c at n + c * N2
+  fs_dft(buf, tl, th, a, a.N2);
   c32* dst = a.out + b * a.obs + r0;
-  const int lgT = __ffs(TR) - 1;
   int tid = static_cast<int>(threadIdx.x);
   asm volatile("" : "+v"(tid));  // keeps the store indices from being computed before the passes
 #pragma unroll
@@ -341,7 +340,7 @@ __global__ __launch_bounds__(FS_T, 4) void fs_rows(FsArgs a) {
     const int e = tid + i * FS_T;
     const int c = e & (TR - 1), k2 = e >> lgT;
     if (e < E && c < nrow) {
-      c32 v = buf[k2 + c * a.N2];
+      c32 v = buf[e];
       v = {v.x * a.scale, (a.inverse ? -v.y : v.y) * a.scale};
       dst[c + static_cast<int64_t>(k2) * a.N1] = v;
     }
