@@ -167,6 +167,8 @@ struct FsArgs {
   int rad[FS_MAXP];
   int inverse;
   float scale;
+  float pscale;  // fs_cols<true>: scale of the c2r pre-pass
+  c32 w2;        // fs_cols<true>: W_{2N}^1
 };
 
 __device__ __forceinline__ c32 fs_tw(const c32* tl, const c32* th, int m) {
@@ -259,24 +261,31 @@ __device__ __forceinline__ void fs_tables(c32* tl, c32* th, const FsArgs& a) {
   for (int i = threadIdx.x; i < a.nhi; i += FS_T) th[i] = a.thi[i];
 }
 
-// columns: block = (batch b, columns n2 in [c0, c0 + TC)); TC is a power of two
+// columns: block = (batch b, columns n2 in [c0, c0 + TC)); TC is a power of two.
+// C2R: the input is the half spectrum X[0..N] of a real sequence of length 2N
+// (column stride ibs) and the kernel forms the packed sequence of the c2r
+// transform on load (as c2r_pre): Z[k] = (E + i O) pscale with
+// E = (X[k] + conj X[N-k])/2, O = W_{2N}^{-k} (X[k] - conj X[N-k])/2,
+// W_{2N}^k = W_N^{k/2} (W_{2N}^1 for odd k); Im X[0] and Im X[N] are ignored.
+template <bool C2R>
 __global__ __launch_bounds__(FS_T, 4) void fs_cols(FsArgs a) {
   __shared__ c32 buf[FS_E];
   __shared__ c32 tl[FS_TAB], th[FS_TAB];
   fs_tables(tl, th, a);
   const int TC = a.T;
+  const int lgT = __ffs(TC) - 1;
   const int tiles = (a.N2 + TC - 1) / TC;
   const int64_t b = blockIdx.x / tiles;
   const int c0 = static_cast<int>(blockIdx.x % tiles) * TC;
   const int E = a.N1 * TC;
-  const c32* src = a.in + b * a.ibs + c0;
-  {
+  if constexpr (!C2R) {
+    const c32* src = a.in + b * a.ibs + c0;
     // all loads issued before the first LDS store
     c32 v[FS_E / FS_T];
 #pragma unroll
     for (int i = 0; i < FS_E / FS_T; ++i) {
       const int e = static_cast<int>(threadIdx.x) + i * FS_T;
-        const int c = e & (TC - 1), n = e >> (__ffs(TC) - 1);
+      const int c = e & (TC - 1), n = e >> lgT;
       v[i] = {0.0f, 0.0f};
       if (e < E && c0 + c < a.N2) v[i] = src[c + static_cast<int64_t>(n) * a.N2];
     }
@@ -285,11 +294,43 @@ __global__ __launch_bounds__(FS_T, 4) void fs_cols(FsArgs a) {
       const int e = static_cast<int>(threadIdx.x) + i * FS_T;
       if (e < E) buf[e] = {v[i].x, a.inverse ? -v[i].y : v[i].y};  // element n of column c at n * TC + c
     }
+  } else {
+    const c32* X = a.in + b * a.ibs;
+    c32 p[FS_E / FS_T], q[FS_E / FS_T];
+#pragma unroll
+    for (int i = 0; i < FS_E / FS_T; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * FS_T;
+      const int c = e & (TC - 1), n = e >> lgT;
+      p[i] = q[i] = {0.0f, 0.0f};
+      if (e < E && c0 + c < a.N2) {
+        const int k = c0 + c + n * a.N2;
+        p[i] = X[k];
+        q[i] = X[a.N - k];  // X[N] for k = 0
+      }
+    }
+    __syncthreads();  // tables
+#pragma unroll
+    for (int i = 0; i < FS_E / FS_T; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * FS_T;
+      const int c = e & (TC - 1), n = e >> lgT;
+      const int k = c0 + c + n * a.N2;
+      c32 z;
+      if (k == 0) {
+        z = {0.5f * (p[i].x + q[i].x), 0.5f * (p[i].x - q[i].x)};
+      } else {
+        const c32 Ev = {0.5f * (p[i].x + q[i].x), 0.5f * (p[i].y - q[i].y)};
+        const c32 D = {0.5f * (p[i].x - q[i].x), 0.5f * (p[i].y + q[i].y)};
+        c32 w = fs_tw(tl, th, k >> 1);
+        if (k & 1) w = cmul(w, a.w2);
+        const c32 O = cmulc(D, w);
+        z = {Ev.x - O.y, Ev.y + O.x};
+      }
+      if (e < E) buf[e] = {z.x * a.pscale, -z.y * a.pscale};  // the inverse: conjugated
+    }
   }
   __syncthreads();
   fs_dft(buf, tl, th, a, a.N1);
   c32* dst = a.out + b * static_cast<int64_t>(a.N) + c0;
-  const int lgT = __ffs(TC) - 1;
   int tid = static_cast<int>(threadIdx.x);
   asm volatile("" : "+v"(tid));  // keeps the store indices from being computed before the passes
 #pragma unroll
@@ -347,8 +388,11 @@ __global__ __launch_bounds__(FS_T, 4) void fs_rows(FsArgs a) {
   }
 }
 
+// c2r (c2r = true): in is the half spectrum X[0..N] (column stride ibs) of real
+// sequences of length 2N, out their values as N complex pairs, scaled by
+// `scale` / N (the normalized inverse times scale).
 static int fs_c2c(const FftPlan* p, const c32* in, int64_t ibs, c32* out, int64_t obs, int64_t batch, bool inverse,
-                  double scale, c32* s0, hipStream_t st) {
+                  double scale, c32* s0, hipStream_t st, bool c2r = false) {
   FsArgs a;
   a.tlo = p->tlo;
   a.thi = p->thi;
@@ -367,9 +411,17 @@ static int fs_c2c(const FftPlan* p, const c32* in, int64_t ibs, c32* out, int64_
     a.np = static_cast<int>(p->rad1.size());
     for (int i = 0; i < a.np; ++i) a.rad[i] = p->rad1[i];
     a.scale = 1.0f;
+    a.pscale = static_cast<float>(scale / static_cast<double>(p->N));
+    const double ph = -3.14159265358979323846 / static_cast<double>(p->N);
+    a.w2 = {static_cast<float>(std::cos(ph)), static_cast<float>(std::sin(ph))};
     const int64_t grid = (p->N2 + p->TC - 1) / p->TC * batch;
-    ProfScope ps(st, "fft_fs_cols", bytes);
-    fs_cols<<<static_cast<unsigned>(grid), FS_T, 0, st>>>(a);
+    if (c2r) {
+      ProfScope ps(st, "fft_fs_cols_c2r", bytes + 8.0 * static_cast<double>(p->N) * static_cast<double>(batch));
+      fs_cols<true><<<static_cast<unsigned>(grid), FS_T, 0, st>>>(a);
+    } else {
+      ProfScope ps(st, "fft_fs_cols", bytes);
+      fs_cols<false><<<static_cast<unsigned>(grid), FS_T, 0, st>>>(a);
+    }
     ZDT_CHECK_LAUNCH("fs_cols");
   }
   {
@@ -380,7 +432,7 @@ static int fs_c2c(const FftPlan* p, const c32* in, int64_t ibs, c32* out, int64_
     a.T = p->TR;
     a.np = static_cast<int>(p->rad2.size());
     for (int i = 0; i < a.np; ++i) a.rad[i] = p->rad2[i];
-    a.scale = static_cast<float>(scale);
+    a.scale = c2r ? 1.0f : static_cast<float>(scale);
     const int64_t grid = (p->N1 + p->TR - 1) / p->TR * batch;
     ProfScope ps(st, "fft_fs_rows", bytes);
     fs_rows<<<static_cast<unsigned>(grid), FS_T, 0, st>>>(a);
@@ -768,6 +820,12 @@ int fft_c2r(int64_t N, const c32* in, int64_t ldi, float* out, int64_t ldo, int6
     const int64_t M = N / 2;
     if (ldo % 2 != 0) return fail(ZDT_EINVAL, "c2r: column stride must be even for even N");
     const FftPlan *pM = nullptr, *pN = nullptr;
+    if (M > 1) {
+      ZDT_TRY(fft_plan_get(M, false, &pM, maxr));
+      if (pM->N1 > 0)  // the pre-pass fused into the four-step columns kernel
+        return fs_c2c(pM, in, ldi, reinterpret_cast<c32*>(out), ldo / 2, batch, true, scale, static_cast<c32*>(s0),
+                      st, true);
+    }
     ZDT_TRY(fft_plan_get(N, false, &pN, maxr));
     c32* Z = static_cast<c32*>(s2);
     {
