@@ -484,8 +484,9 @@ int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, in
       return zdt::fail(ZDT_ENOTSUP, "ZDT_FLAG_N2_512: no fused N2 = 512 variant for Nf=%lld", (long long)Nf);
     n2 = 512;
   }
-  return zdt::create_ws_tuned(spec, Nf, Nt, r0, dr, Nr, factors, nfactors, flags & ~ZDT_FLAG_N2_512, stream, n2, 16,
-                              out);
+  const int maxr = (flags & ZDT_FLAG_RADIX8) ? 8 : 16;
+  return zdt::create_ws_tuned(spec, Nf, Nt, r0, dr, Nr, factors, nfactors, flags & ~(ZDT_FLAG_N2_512 | ZDT_FLAG_RADIX8),
+                              stream, n2, maxr, out);
 }
 
 int zdt_destroy(zdt_ws* ws) {

@@ -29,6 +29,7 @@ ZDT_FLAG_UNNORMALIZED = 0x1
 ZDT_FLAG_GENERIC = 0x2
 ZDT_FLAG_PROFILE = 0x4
 ZDT_FLAG_N2_512 = 0x8
+ZDT_FLAG_RADIX8 = 0x10
 
 ZDT_PLAN_AUTOTUNE = 0x100
 

@@ -60,11 +60,12 @@ class ZDTWorkspace:
     Keyword-only extras: ``unnormalized=True`` scales the output by Nf*Nl
     (test/runtests.jl:31 convention); ``generic=True`` forces the unfused
     kernels; ``n2=512`` selects the fused path's 512-point strided FFTs
-    where 1024 is the default.
+    where 1024 is the default; ``radix_cap=8`` limits the generic FFTs to
+    radices up to 8 (16 is the default).
     """
 
     def __init__(self, spectrogram, *args, factors=None, output_aligned: bool = False,
-                 unnormalized: bool = False, generic: bool = False, n2: int = 0):
+                 unnormalized: bool = False, generic: bool = False, n2: int = 0, radix_cap: int = 16):
         if len(args) in (1, 2) and not isinstance(args[0], (int, float, np.floating, np.integer)):
             r0, dr, Nr = rates_params(args[0])
             if len(args) == 2:
@@ -86,7 +87,9 @@ class ZDTWorkspace:
         self.device = spec.device
         self._stream = D.stream_ptr()
         flags = ((_lib.ZDT_FLAG_UNNORMALIZED if unnormalized else 0) | (_lib.ZDT_FLAG_GENERIC if generic else 0)
-                 | (_lib.ZDT_FLAG_N2_512 if n2 == 512 else 0))
+                 | (_lib.ZDT_FLAG_N2_512 if n2 == 512 else 0) | (_lib.ZDT_FLAG_RADIX8 if radix_cap == 8 else 0))
+        if radix_cap not in (8, 16):
+            raise ValueError("radix_cap must be 8 or 16")
         arr, n = _lib.int_array(self.factors)
         h = C.c_void_p()
         self._h = None

@@ -53,6 +53,8 @@ extern "C" {
 #define ZDT_FLAG_PROFILE 0x4      /* time every kernel launch (HIP events)    */
 #define ZDT_FLAG_N2_512 0x8       /* fused path: strided FFT length 512 where  *
                                    * 1024 is the default (2^17 <= Nf <= 2^19)  */
+#define ZDT_FLAG_RADIX8 0x10      /* generic FFTs: radices up to 8 (16 is the  *
+                                   * default; ragged plans time both)          */
 
 /* buffers exposed by zdt_buffer */
 #define ZDT_BUF_F 0  /* Nk x Nt ComplexF32: input FFT (natural layout)        */

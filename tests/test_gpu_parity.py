@@ -109,6 +109,16 @@ def test_zdt_vs_fp64(F, Nf, Nt, r0, dr, Nr, generic):
     assert normwise(out, ref) <= 2e-6
 
 
+@pytest.mark.parametrize("Nf,Nt,r0,dr,Nr", [(98304, 16, -4.0, 0.0625, 129), (39366, 8, -1.0, 0.25, 9),
+                                             (1 << 15, 12, -2.0, 0.25, 17)])
+def test_zdt_generic_radix8_vs_fp64(F, Nf, Nt, r0, dr, Nr):
+    """The generic FFTs with radices up to 8 (a ragged plan's other candidate)."""
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.4, 0.75, 20.0)], seed=Nf + 3)
+    say(f"zdt radix 8 Nf={Nf} Nt={Nt}")
+    ws = F.ZDTWorkspace(s, r0, dr, Nr, generic=True, radix_cap=8)
+    assert normwise(host(F.zdtfdr(ws)), Z.fdr64(s, r0, dr, Nr)) <= 2e-6
+
+
 @pytest.mark.parametrize("generic", [False, True])
 def test_integer_rates_vs_intfdr(F, generic):
     s = synth.cfg1(seed=21)
