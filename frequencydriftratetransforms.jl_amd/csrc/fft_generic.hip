@@ -276,7 +276,15 @@ __global__ __launch_bounds__(FS_T, 4) void fs_cols(FsArgs a) {
   const int lgT = __ffs(TC) - 1;
   const int tiles = (a.N2 + TC - 1) / TC;
   const int64_t b = blockIdx.x / tiles;
-  const int c0 = static_cast<int>(blockIdx.x % tiles) * TC;
+  int q = static_cast<int>(blockIdx.x % tiles);
+  if (C2R && (tiles & 15) == 0) {
+    // tile i and its mirror tiles-1-i (which reads the same columns of X)
+    // eight blocks apart: same XCD (blocks go round-robin over the eight),
+    // in flight together, so the second read of a column hits in L2
+    const int g = q >> 4, x = q & 15;
+    q = x < 8 ? g * 8 + x : tiles - 1 - (g * 8 + x - 8);
+  }
+  const int c0 = q * TC;
   const int E = a.N1 * TC;
   if constexpr (!C2R) {
     const c32* src = a.in + b * a.ibs + c0;
