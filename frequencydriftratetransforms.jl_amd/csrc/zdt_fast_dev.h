@@ -293,28 +293,62 @@ __device__ __forceinline__ HPair hpair(int p, int q) {
   return h;
 }
 
-// Sums of the coefficients c_t at z and conj(z), |z| = 1, by the
-// Reinsch-modified Goertzel recurrence over the real and imaginary parts
-// (see czt_reinsch_col); s, lambda and sin(theta) come from the FP64 z.
-template <int NT>
-__device__ __forceinline__ void reinsch2(const c32 (&c)[NT], c64 z, c32& Yz, c32& Yzc) {
-  const float s = z.x >= 0.0 ? 1.0f : -1.0f;
-  const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
-  const float sn = static_cast<float>(z.y);
-  float Dr = c[NT - 1].x, Ur = c[NT - 1].x, Di = c[NT - 1].y, Ui = c[NT - 1].y;
+// The Reinsch recurrence of one part (real or imaginary) of the
+// coefficients, with the sign s = +-1 folded into the state:
+// D~_t = s^t D_t and U~_t = s^t U_t turn
+//   D_t = a_t + lambda U_{t+1} + s D_{t+1},  U_t = D_t + s U_{t+1}
+// into
+//   D~_t = D~_{t+1} + s^t a_t + (s lambda) U~_{t+1},  U~_t = D~_t + U~_{t+1},
+// whose every update accumulates in place (v_add_f32 / v_fmac_f32, the VOP2
+// forms that issue fastest); the values are those of the original recurrence
+// up to the sign.  Returns D~_1 and U~_1.
+template <int NT, typename A>
+__device__ __forceinline__ void reinsch_part(const A& a, float s, float lamp, float& D, float& U) {
+  D = ((NT - 1) & 1) ? s * a(NT - 1) : a(NT - 1);
+  U = D;
 #pragma unroll
   for (int t = NT - 2; t >= 1; --t) {
-    Dr = fmaf(s, Dr, fmaf(lam, Ur, c[t].x));
-    Ur = fmaf(s, Ur, Dr);
-    Di = fmaf(s, Di, fmaf(lam, Ui, c[t].y));
-    Ui = fmaf(s, Ui, Di);
+    if (t & 1) {
+      D = fmaf(s, a(t), D);
+    } else {
+      D += a(t);
+    }
+    D = fmaf(lamp, U, D);
+    U += D;
   }
-  const float hl = 0.5f * lam;
-  const float Cr = fmaf(s, Dr, fmaf(hl, Ur, c[0].x));
-  const float Ci = fmaf(s, Di, fmaf(hl, Ui, c[0].y));
-  const float Sr = sn * Ur, Si = sn * Ui;
+}
+
+// Sums of the coefficients (cr_t, ci_t) at z and conj(z), |z| = 1, by the
+// Reinsch-modified Goertzel recurrence over the real and imaginary parts (see
+// czt_reinsch_col): with s = sign(cos theta) and lambda = 2 (cos theta - s)
+// from the FP64 z, sum a_t cos(t theta) = a_0 + (lambda/2) U_1 + s D_1 =
+// a_0 + (s lambda/2) U~_1 + D~_1 and sum a_t sin(t theta) = U_1 sin theta =
+// U~_1 s sin theta.
+template <int NT, typename AR, typename AI>
+__device__ __forceinline__ void reinsch_pair(const AR& ar, const AI& ai, c64 z, c32& Yz, c32& Yzc) {
+  const float s = z.x >= 0.0 ? 1.0f : -1.0f;
+  const float lamp = static_cast<float>(2.0 * (z.x - static_cast<double>(s))) * s;
+  const float ssn = static_cast<float>(z.y) * s;
+  float Dr, Ur, Di, Ui;
+  reinsch_part<NT>(ar, s, lamp, Dr, Ur);
+  reinsch_part<NT>(ai, s, lamp, Di, Ui);
+  const float hl = 0.5f * lamp;
+  const float Cr = fmaf(hl, Ur, ar(0)) + Dr;
+  const float Ci = fmaf(hl, Ui, ai(0)) + Di;
+  const float Sr = ssn * Ur, Si = ssn * Ui;
   Yz = {Cr - Si, Sr + Ci};
   Yzc = {Cr + Si, Ci - Sr};
+}
+
+template <int NT>
+__device__ __forceinline__ void reinsch2(const c32 (&c)[NT], c64 z, c32& Yz, c32& Yzc) {
+  reinsch_pair<NT>([&](int t) { return c[t].x; }, [&](int t) { return c[t].y; }, z, Yz, Yzc);
+}
+
+template <int NT>
+__device__ __forceinline__ void reinsch_split(const float (&cr)[NT], const float (&ci)[NT], c64 z, c32& Yz,
+                                              c32& Yzc) {
+  reinsch_pair<NT>([&](int t) { return cr[t]; }, [&](int t) { return ci[t]; }, z, Yz, Yzc);
 }
 
 }  // namespace zdt

@@ -116,26 +116,10 @@ __global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
   const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
   for (int j = 0; j < npair; ++j) {
     const int jp = a.Nr - 1 - j;
-    const bool mirror = jp != j;
-    {
-      const float s = z.x >= 0.0 ? 1.0f : -1.0f;
-      const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
-      const float sn = static_cast<float>(z.y);
-      float Dr = cr[NT - 1], Ur = cr[NT - 1], Di = ci[NT - 1], Ui = ci[NT - 1];
-#pragma unroll
-      for (int t = NT - 2; t >= 1; --t) {
-        Dr = fmaf(s, Dr, fmaf(lam, Ur, cr[t]));
-        Ur = fmaf(s, Ur, Dr);
-        Di = fmaf(s, Di, fmaf(lam, Ui, ci[t]));
-        Ui = fmaf(s, Ui, Di);
-      }
-      const float hl = 0.5f * lam;
-      const float Cr = fmaf(s, Dr, fmaf(hl, Ur, cr[0]));
-      const float Ci = fmaf(s, Di, fmaf(hl, Ui, ci[0]));
-      const float Sr = sn * Ur, Si = sn * Ui;
-      bin.dst[static_cast<int64_t>(j) * bin.stride] = {Cr - Si, Sr + Ci};
-      if (mirror) bin.dst[static_cast<int64_t>(jp) * bin.stride] = {Cr + Si, Ci - Sr};
-    }
+    c32 yz, yc;
+    reinsch_split<NT>(cr, ci, z, yz, yc);
+    bin.dst[static_cast<int64_t>(j) * bin.stride] = yz;
+    if (jp != j) bin.dst[static_cast<int64_t>(jp) * bin.stride] = yc;
     z = cmul(z, g);
   }
 }

@@ -138,7 +138,9 @@ __global__ __launch_bounds__(256) void czt_direct(CztArgs a) {
 // Clenshaw method", 1977).  With s = sign(cos theta), lambda = 2 (cos theta - s),
 //   D_t = c_t + lambda U_{t+1} + s D_{t+1},  U_t = D_t + s U_{t+1}  (t = Nt-1 .. 1),
 //   sum c_t cos(t theta) = c_0 + (lambda/2) U_1 + s D_1,  sum c_t sin(t theta) = U_1 sin theta,
-// six FMAs per term for the two rates against eight for two complex Horner
+// (evaluated with s folded into the state, so that every update is an
+// in-place add or FMA), six operations per term for the two rates against
+// eight for two complex Horner
 // evaluations.  z comes from an FP64 recurrence started at an exactly
 // evaluated value, and lambda from the FP64 cos theta, so no accuracy is lost
 // near theta = 0 or pi.  The middle rate of an odd Nr is its own partner.
@@ -167,21 +169,32 @@ __global__ __launch_bounds__(256) void czt_reinsch(CztArgs a) {
   const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
   for (int j = 0; j < npair; ++j) {
     const int jp = a.Nr - 1 - j;
+    // the recurrence with s folded into the state (D~_t = s^t D_t,
+    // U~_t = s^t U_t): every update accumulates in place
     const float s = z.x >= 0.0 ? 1.0f : -1.0f;
-    const float lam = static_cast<float>(2.0 * (z.x - static_cast<double>(s)));
-    const float sn = static_cast<float>(z.y);
-    float Dr = cr[NT - 1], Ur = cr[NT - 1], Di = ci[NT - 1], Ui = ci[NT - 1];
+    const float lamp = static_cast<float>(2.0 * (z.x - static_cast<double>(s))) * s;
+    const float ssn = static_cast<float>(z.y) * s;
+    constexpr bool odd = (NT - 1) & 1;
+    float Dr = odd ? s * cr[NT - 1] : cr[NT - 1], Di = odd ? s * ci[NT - 1] : ci[NT - 1];
+    float Ur = Dr, Ui = Di;
 #pragma unroll
     for (int t = NT - 2; t >= 1; --t) {
-      Dr = fmaf(s, Dr, fmaf(lam, Ur, cr[t]));
-      Ur = fmaf(s, Ur, Dr);
-      Di = fmaf(s, Di, fmaf(lam, Ui, ci[t]));
-      Ui = fmaf(s, Ui, Di);
+      if (t & 1) {
+        Dr = fmaf(s, cr[t], Dr);
+        Di = fmaf(s, ci[t], Di);
+      } else {
+        Dr += cr[t];
+        Di += ci[t];
+      }
+      Dr = fmaf(lamp, Ur, Dr);
+      Di = fmaf(lamp, Ui, Di);
+      Ur += Dr;
+      Ui += Di;
     }
-    const float hl = 0.5f * lam;
-    const float Cr = fmaf(s, Dr, fmaf(hl, Ur, cr[0]));
-    const float Ci = fmaf(s, Di, fmaf(hl, Ui, ci[0]));
-    const float Sr = sn * Ur, Si = sn * Ui;
+    const float hl = 0.5f * lamp;
+    const float Cr = fmaf(hl, Ur, cr[0]) + Dr;
+    const float Ci = fmaf(hl, Ui, ci[0]) + Di;
+    const float Sr = ssn * Ur, Si = ssn * Ui;
     dst[static_cast<int64_t>(j) * a.ldY] = {Cr - Si, Sr + Ci};
     if (jp != j) dst[static_cast<int64_t>(jp) * a.ldY] = {Cr + Si, Ci - Sr};
     z = cmul(z, g);
