@@ -119,6 +119,17 @@ def test_zdt_generic_radix8_vs_fp64(F, Nf, Nt, r0, dr, Nr):
     assert normwise(host(F.zdtfdr(ws)), Z.fdr64(s, r0, dr, Nr)) <= 2e-6
 
 
+@pytest.mark.parametrize("Nf,Nt,r0,dr,Nr", [(3 << 19, 4, -1.0, 0.5, 5), (3 << 20, 2, 0.0, 1.0, 2)])
+def test_zdt_generic_fft_limits(F, Nf, Nt, r0, dr, Nr):
+    """The largest four-step frequency FFTs (M = 768 x 1024: 8-column tiles,
+    a 768-entry high twiddle table) and the pass-per-radix FFTs above them
+    (M = 3 x 2^19 > 2^20)."""
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.25, 0.5, 30.0)], seed=Nf % 977)
+    say(f"zdt generic FFT limits Nf={Nf} Nt={Nt}")
+    ws = F.ZDTWorkspace(s, r0, dr, Nr, generic=True)
+    assert normwise(host(F.zdtfdr(ws)), Z.fdr64(s, r0, dr, Nr)) <= 2e-6
+
+
 @pytest.mark.parametrize("generic", [False, True])
 def test_integer_rates_vs_intfdr(F, generic):
     s = synth.cfg1(seed=21)
