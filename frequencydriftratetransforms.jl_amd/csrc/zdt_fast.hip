@@ -288,8 +288,8 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   constexpr int HALF = N1 / 2;
   __shared__ c32 buf[ROWS * P];
   __shared__ c32 buf2[ROWS * P];
-  __shared__ c32 twl[N1];  // W_N1^m
-  for (int i = threadIdx.x; i < N1; i += T) twl[i] = a.tw1[i];
+  __shared__ c32 twl[N1 + N1 / 8];  // W_N1^m at lsw(m)
+  for (int i = threadIdx.x; i < N1; i += T) twl[lsw(i)] = a.tw1[i];
   __shared__ c32 ahalf[NT];  // a_t of the bin M/2 (block 0 only)
 
   const int tid = threadIdx.x;
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       }
     }
     __syncthreads();
-    const c32* fz = lds_fft_pp<N1, T, false>(buf, buf2, P, 2 * nt, twl);
+    const c32* fz = lds_fft_pp<N1, T, false, 1, true>(buf, buf2, P, 2 * nt, twl);
 #pragma unroll
     for (int uu = 0; uu < TG; ++uu) {
       const int u = t0 + uu;
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       // slots 2pp, 2pp+1 hold rates p0+pp, Pm-p0-pp (the middle rate once, last)
       const int nslots = 2 * np - ((Pm & 1) == 0 && p0 + np == npairs ? 1 : 0);
       const int nrw = min(RPW, max(0, 2 * nslots - wv * RPW));  // uniform per wavefront
-      wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+      wave_fft<N1, true, 1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
         constexpr int RL = sizeof(v) / sizeof(c32);
         const int row = wv * RPW + f;
         const int slot = row >> 1;
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       }
       __syncthreads();
       const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));  // uniform per wavefront
-      wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+      wave_fft<N1, true, 1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
         constexpr int RL = sizeof(v) / sizeof(c32);
         const int row = wv * RPW + f;
         // B is blocked by 16 n1: B[j][n1/16][k2][n1%16]

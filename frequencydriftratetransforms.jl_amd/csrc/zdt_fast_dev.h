@@ -27,13 +27,18 @@ constexpr int fft_radix() {
 // N + N/8 entries.
 __device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
 
+// Twiddle tables in LDS are read at (r k) << sh, a stride of 2^sh entries
+// across the lanes of a pass; with SW the passes read a table stored
+// swizzled like the rows (entry m at lsw(m), N + N/8 entries), which spreads
+// those strided reads over the banks.
+
 // Ping-pong variant: radix-R butterflies stream from buffer `in` to buffer
 // `out` (one butterfly live per thread), one barrier per pass.  Rows are
 // swizzled (lsw); since NR and Ns are multiples of 8 or the write base is,
 // every butterfly needs the swizzle only once for its reads and once for its
 // writes.  Twiddles come from the LDS table tw (W_N^m, m < N); INV selects the
 // inverse butterflies and conjugate twiddles.
-template <int N, int R, int T, bool INV>
+template <int N, int R, int T, bool INV, bool SW = false>
 __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int nrows, int Ns,
                                             const c32* __restrict__ tw) {
   constexpr int NR = N / R;
@@ -58,7 +63,7 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const c32 w = tw[(r * k) << sh];
+        const c32 w = tw[SW ? lsw((r * k) << sh) : (r * k) << sh];
         v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
       }
     }
@@ -82,12 +87,12 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
 
 // radix schedule: 1024 = 16*8*8, 512 = 8*8*8, 256 = 16*16, 128 = 16*8,
 // 64 = 8*8; returns the buffer holding the result
-template <int N, int T, bool INV, int NS = 1>
+template <int N, int T, bool INV, int NS = 1, bool SW = false>
 __device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, const c32* __restrict__ tw) {
   if constexpr (NS < N) {
     constexpr int R = fft_radix<N, NS>();
-    lds_pass_pp<N, R, T, INV>(a, b, P, nrows, NS, tw);
-    return lds_fft_pp<N, T, INV, NS * R>(b, a, P, nrows, tw);
+    lds_pass_pp<N, R, T, INV, SW>(a, b, P, nrows, NS, tw);
+    return lds_fft_pp<N, T, INV, NS * R, SW>(b, a, P, nrows, tw);
   } else {
     return a;
   }
@@ -106,7 +111,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int N, int R, bool INV>
+template <int N, int R, bool INV, bool SW = false>
 __device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, const c32* __restrict__ tw, int lane,
                                           c32 (&v)[R], bool last) {
   constexpr int NR = N / R;
@@ -128,7 +133,7 @@ __device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, con
       const int sh = LOGN - logNs - LOGR;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const c32 w = tw[(r * k) << sh];
+        const c32 w = tw[SW ? lsw((r * k) << sh) : (r * k) << sh];
         v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
       }
     }
@@ -162,15 +167,15 @@ constexpr int fft_max_nr() {
 // In-place inverse/forward FFT of the wavefront's rows; the last pass leaves
 // its butterfly outputs in registers (lane (f, j): row f, elements
 // j + r*N/R) and calls emit(f, j, v) instead of writing them back.
-template <int N, bool INV, int NS = 1, typename Emit>
+template <int N, bool INV, int NS = 1, bool SW = false, typename Emit>
 __device__ __forceinline__ void wave_fft(c32* rows, int P, int nrw, const c32* __restrict__ tw, int lane, Emit&& emit) {
   constexpr int R = fft_radix<N, NS>();
   c32 v[R];
   if constexpr (NS * R < N) {
-    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, false);
-    wave_fft<N, INV, NS * R>(rows, P, nrw, tw, lane, emit);
+    wave_pass<N, R, INV, SW>(rows, P, nrw, NS, tw, lane, v, false);
+    wave_fft<N, INV, NS * R, SW>(rows, P, nrw, tw, lane, emit);
   } else {
-    wave_pass<N, R, INV>(rows, P, nrw, NS, tw, lane, v, true);
+    wave_pass<N, R, INV, SW>(rows, P, nrw, NS, tw, lane, v, true);
     constexpr int NR = N / R;
     if (lane < nrw * NR) emit(lane / NR, lane % NR, v);
   }

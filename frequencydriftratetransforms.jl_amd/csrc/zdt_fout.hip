@@ -144,8 +144,8 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   constexpr int HALF = N1 / 2;
   __shared__ c32 buf[ROWS * P];
   __shared__ c32 buf2[ROWS * P];
-  __shared__ c32 twl[N1];
-  for (int i = threadIdx.x; i < N1; i += T) twl[i] = a.tw1[i];
+  __shared__ c32 twl[N1 + N1 / 8];  // W_N1^m at lsw(m)
+  for (int i = threadIdx.x; i < N1; i += T) twl[lsw(i)] = a.tw1[i];
   __syncthreads();
   const int tid = threadIdx.x;
   const int p = blockIdx.x;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     __syncthreads();
     if (j0 + JG < a.Nr) load(j0 + JG);
     const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));
-    wave_fft<N1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+    wave_fft<N1, true, 1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
       constexpr int RL = sizeof(v) / sizeof(c32);
       const int row = wv * RPW + f;
       c32* dst = a.B + static_cast<int64_t>(j0 + (row >> 1)) * (N1 * N2) + ((row & 1) ? colB : colA) * 16;
