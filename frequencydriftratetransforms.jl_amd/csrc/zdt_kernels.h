@@ -22,9 +22,8 @@ struct CztArgs {
   float scale;
 };
 
-// F -> Ys by czt_reinsch (Nt <= 64) or czt_direct (any Nt)
+// F -> Ys by czt_reinsch (Nt <= 64) or czt_reinsch_chunks (Nt > 64)
 int launch_czt(const CztArgs& a, hipStream_t st);
-int launch_czt_direct(const CztArgs& a, hipStream_t st);
 // fftfdr: Ys[k,j] = sum_t F[k,t] e^{2 pi i k t rates[j]/Nf} (rates_over_nf: device, Nr doubles)
 int launch_czt_rates(const c32* F, int64_t ldF, c32* Ys, int64_t ldY, int64_t Nk, int64_t Nf, int Nt,
                      const double* rates_over_nf, int Nr, hipStream_t st);

@@ -9,9 +9,9 @@
 // convolution through FFTs (:249-253); the identity chirp(k,l) = chirp(k,-l)
 // makes the high lags of V (:182-184) the negative lags, so the result equals
 // the linear convolution above for any Nl >= Nt+Nr-1.  czt_reinsch (Nt <= 64)
-// and czt_direct evaluate it directly (Nt complex MACs per output), which
-// costs less than the two length-Nl FFTs, needs no V, and is exact up to
-// float rounding of the products.  The step functions (preprocess!,
+// and czt_reinsch_chunks (Nt > 64) evaluate it directly (about Nt operations
+// per output), which costs less than the two length-Nl FFTs, needs no V, and
+// is exact up to float rounding.  The step functions (preprocess!,
 // convolve!, postprocess!) keep the reference's FFT data flow on the
 // materialized Nk x Nl matrix Y for API parity.
 #include <hip/hip_runtime.h>
@@ -52,84 +52,11 @@ __device__ __forceinline__ float window_value(int wkind, const float* wcustom, i
 }
 
 // ---------------------------------------------------------------------------
-// generic fused CZT: F (Nk x Nt) -> Ys (Nk x Nr), one thread per bin k.
-//
-// Time steps are taken TC at a time (chunks processed last to first so that
-// the chunk starting at t=0 finishes each output); the chirp values at the
-// TC lags an output needs live in a register ring indexed at compile time.
-// chirp(k,l) for successive l comes from an FP64 recurrence
-//   chirp(l+1) = chirp(l) g(l),  g(l+1) = g(l) h,  h = exp(-4 pi i k dr/(2Nf))
-// started from exactly evaluated values, so the phases stay at FP64 accuracy
-// however large k*l^2*dr/Nf grows.
-// ---------------------------------------------------------------------------
-template <int TC>
-__global__ __launch_bounds__(256) void czt_direct(CztArgs a) {
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= a.Nk) return;
-  const double kd = static_cast<double>(k);
-  const float w = window_value(a.wkind, a.wcustom, k, a.Nf) * a.scale;
-  const c64 h = cis64(-2.0 * kd * a.hd);
-  const int nchunks = (a.Nt + TC - 1) / TC;
-  for (int c = nchunks - 1; c >= 0; --c) {
-    const int c0 = c * TC;
-    c32 y[TC];
-#pragma unroll
-    for (int u = 0; u < TC; ++u) {
-      const int t = c0 + u;
-      if (t < a.Nt) {
-        const double td = static_cast<double>(t);
-        const c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
-        y[u] = cmul(f, cis_sum2(kd * td * td, a.hd, kd * td, a.h0));
-      } else {
-        y[u] = {0.0f, 0.0f};
-      }
-    }
-    // ring slot of lag L is (L + c0) mod TC; fill lags -c0-TC+1 .. -c0-1
-    const double L0 = static_cast<double>(-c0 - TC + 1);
-    c64 ch = cis64(-(kd * L0 * L0) * a.hd);
-    c64 g = cis64(-(kd * (2.0 * L0 + 1.0)) * a.hd);
-    c32 ring[TC];
-#pragma unroll
-    for (int i = 0; i < TC - 1; ++i) {
-      ring[(i + 1) % TC] = {static_cast<float>(ch.x), static_cast<float>(ch.y)};
-      ch = cmul(ch, g);
-      g = cmul(g, h);
-    }
-    // now ch = chirp(k, -c0)
-    const bool first = (c == nchunks - 1);
-    const bool final = (c == 0);
-    for (int J = 0; J < a.Nr; J += TC) {
-#pragma unroll
-      for (int m = 0; m < TC; ++m) {
-        const int j = J + m;
-        if (j < a.Nr) {
-          ring[m] = {static_cast<float>(ch.x), static_cast<float>(ch.y)};  // lag j - c0
-          ch = cmul(ch, g);
-          g = cmul(g, h);
-          c32 acc = {0.0f, 0.0f};
-#pragma unroll
-          for (int u = 0; u < TC; ++u) acc = cfma(acc, y[u], ring[(m - u + TC) % TC]);
-          c32* dst = a.Ys + k + static_cast<int64_t>(j) * a.ldY;
-          if (!first) acc = cadd(acc, *dst);
-          if (final) {
-            // c0 == 0: ring[m] = chirp(k, j), post(k,j) = conj(chirp(k,j))
-            acc = cmulc(acc, ring[m]);
-            acc = {acc.x * w, acc.y * w};
-          }
-          *dst = acc;
-        }
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Vandermonde form for Nt <= 64: by Bluestein's identity the sum above is
 //   Ys[k,j] = w(k) sum_t a_t z_j^t,  a_t = F[k,t] e^{2 pi i k t r0/Nf},
 //   z_j = gamma^j, gamma = e^{2 pi i k dr/Nf},
 // a polynomial in z_j.  A thread keeps its bin's coefficients in registers
-// and writes every Ys[k,j] once (czt_direct re-reads and re-writes Ys once per
-// chunk of 16 time steps).  The rates are taken in mirrored pairs: with the
+// and writes every Ys[k,j] once.  The rates are taken in mirrored pairs: with the
 // centre rate rc = r0 + (Nr-1) dr/2 folded into the coefficients
 // (c_t = F[k,t] e^{2 pi i k t rc/Nf}), rates j and Nr-1-j are the sums at
 // z = e^{i theta} and conj(z), theta = 2 pi k (r0 + j dr - rc)/Nf, and one
@@ -201,9 +128,89 @@ __global__ __launch_bounds__(256) void czt_reinsch(CztArgs a) {
   }
 }
 
+// Nt > 64: the sum in chunks of 64 time steps, sum_t c_t z^t =
+// sum_q z^{64q} P_q(z) with P_q the polynomial of chunk q, evaluated for both
+// rates of a pair by the recurrence of czt_reinsch; z^{64q} (conj for the
+// partner) comes from an FP64 recurrence over the rates.  Chunk 0 writes Ys,
+// later chunks add to it (one read-modify-write of Ys per further chunk).
+__global__ __launch_bounds__(256) void czt_reinsch_chunks(CztArgs a) {
+  constexpr int NT = 64;
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= a.Nk) return;
+  const double kd = static_cast<double>(k);
+  const float w = window_value(a.wkind, a.wcustom, k, a.Nf) * a.scale;
+  const c64 g = cis64(2.0 * kd * a.hd);  // gamma = e^{2 pi i k dr/Nf}
+  c32* dst = a.Ys + k;
+  const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
+  for (int t0 = 0; t0 < a.Nt; t0 += NT) {
+    float cr[NT], ci[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const int t = t0 + u;
+      if (t < a.Nt) {
+        c32 f = a.F[k + static_cast<int64_t>(t) * a.ldF];
+        if (a.hc != 0.0) f = cmul(f, cis_nh(kd * static_cast<double>(t), a.hc));  // uniform
+        cr[u] = f.x * w;
+        ci[u] = f.y * w;
+      } else {
+        cr[u] = 0.0f;
+        ci[u] = 0.0f;
+      }
+    }
+    const double td = static_cast<double>(t0);
+    c64 z = cis64(kd * a.hz);               // e^{2 pi i k (r0 - rc)/Nf}
+    c64 zq = cis64(kd * a.hz * td);         // z^{t0} of rate 0
+    const c64 gq = cis64(2.0 * kd * a.hd * td);  // gamma^{t0}
+    for (int j = 0; j < npair; ++j) {
+      const int jp = a.Nr - 1 - j;
+      const float s = z.x >= 0.0 ? 1.0f : -1.0f;
+      const float lamp = static_cast<float>(2.0 * (z.x - static_cast<double>(s))) * s;
+      const float ssn = static_cast<float>(z.y) * s;
+      float Dr = s * cr[NT - 1], Di = s * ci[NT - 1];  // NT - 1 is odd
+      float Ur = Dr, Ui = Di;
+#pragma unroll
+      for (int t = NT - 2; t >= 1; --t) {
+        if (t & 1) {
+          Dr = fmaf(s, cr[t], Dr);
+          Di = fmaf(s, ci[t], Di);
+        } else {
+          Dr += cr[t];
+          Di += ci[t];
+        }
+        Dr = fmaf(lamp, Ur, Dr);
+        Di = fmaf(lamp, Ui, Di);
+        Ur += Dr;
+        Ui += Di;
+      }
+      const float hl = 0.5f * lamp;
+      const float Cr = fmaf(hl, Ur, cr[0]) + Dr;
+      const float Ci = fmaf(hl, Ui, ci[0]) + Di;
+      const float Sr = ssn * Ur, Si = ssn * Ui;
+      const c32 q = {static_cast<float>(zq.x), static_cast<float>(zq.y)};
+      c32 yz = cmul(c32{Cr - Si, Sr + Ci}, q);
+      c32 yc = cmulc(c32{Cr + Si, Ci - Sr}, q);
+      c32* dj = dst + static_cast<int64_t>(j) * a.ldY;
+      c32* dp = dst + static_cast<int64_t>(jp) * a.ldY;
+      if (t0 > 0) {
+        yz = cadd(yz, *dj);
+        if (jp != j) yc = cadd(yc, *dp);
+      }
+      *dj = yz;
+      if (jp != j) *dp = yc;
+      z = cmul(z, g);
+      zq = cmul(zq, gq);
+    }
+  }
+}
+
 int launch_czt(const CztArgs& a, hipStream_t st) {
-  if (a.Nt > 64) return launch_czt_direct(a, st);
   const unsigned grid = static_cast<unsigned>((a.Nk + 255) / 256);
+  if (a.Nt > 64) {
+    ProfScope ps(st, "czt_reinsch_chunks", 8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr * ((a.Nt + 63) / 64)));
+    czt_reinsch_chunks<<<grid, 256, 0, st>>>(a);
+    ZDT_CHECK_LAUNCH("czt_reinsch_chunks");
+    return ZDT_OK;
+  }
   ProfScope ps(st, "czt_reinsch", 8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr));
   if (a.Nt <= 2) {
     czt_reinsch<2><<<grid, 256, 0, st>>>(a);
@@ -311,23 +318,6 @@ int launch_shift_phase(const c32* F, int64_t ldF, c32* Y, int64_t ldY, int64_t N
   return ZDT_OK;
 }
 
-int launch_czt_direct(const CztArgs& a, hipStream_t st) {
-  const unsigned grid = static_cast<unsigned>((a.Nk + 255) / 256);
-  const int nchunks = (a.Nt + 15) / 16;
-  ProfScope ps(st, "czt_direct",
-               8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr * (2.0 * nchunks - 1.0)));
-  if (a.Nt <= 2) {
-    czt_direct<2><<<grid, 256, 0, st>>>(a);
-  } else if (a.Nt <= 4) {
-    czt_direct<4><<<grid, 256, 0, st>>>(a);
-  } else if (a.Nt <= 8) {
-    czt_direct<8><<<grid, 256, 0, st>>>(a);
-  } else {
-    czt_direct<16><<<grid, 256, 0, st>>>(a);
-  }
-  ZDT_CHECK_LAUNCH("czt_direct");
-  return ZDT_OK;
-}
 
 // ---------------------------------------------------------------------------
 // step functions (materialized Y, src/zdtfdr.jl:221-321)

@@ -90,6 +90,7 @@ CASES = [
     (6, 2, 0.0, 1.0, 1),
     (8192, 64, -2.0, 1.0 / 16, 65),
     (4096, 70, -1.0, 0.125, 17),
+    (6000, 150, 0.5, 0.25, 21),  # three 64-step chunks of the chirp-Z
     # four-step frequency FFTs: M = 96 x 512, 100 x 500 and 81 x 243 (partial tiles)
     (98304, 16, -4.0, 0.0625, 129),
     (100000, 16, -2.0, 0.125, 33),
