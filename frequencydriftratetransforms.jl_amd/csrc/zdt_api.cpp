@@ -155,7 +155,7 @@ struct zdt_ws {
 
   // fused fast path (power-of-two Nf)
   FastPlan* fast = nullptr;
-  // fused output for the generic chirp-Z (power-of-two Nf, 16 < Nt <= 64)
+  // fused output for the generic chirp-Z (power-of-two Nf, 16 < Nt <= FOUT_MAX_NT)
   FastPlan* fout = nullptr;
   c32* Ft = nullptr;  // its input spectrum, Nk x Nt entries in the transposed layout
   bool ys_fout = false;  // last run left its spectra in fout

@@ -27,7 +27,11 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* 
 // doubles of k3's statistics partials (held by every fused plan)
 int64_t fast_stats_doubles(int64_t Nf, int64_t Nr);
 
-// Fused output for power-of-two Nf in [2^16, 2^22] and Nt <= 64: the chirp-Z
+// Time steps the fused output takes (the chirp-Z runs in 64-step chunks
+// above 64; the input spectrum Ft holds Nk x Nt entries)
+constexpr int64_t FOUT_MAX_NT = 1024;
+
+// Fused output for power-of-two Nf in [2^16, 2^22] and Nt <= FOUT_MAX_NT: the chirp-Z
 // per bin from the input spectrum F, then the c2r through k2_c2r and k3
 // (fast_output).  F is either natural (Nk x Nt, column stride ldF) or, with
 // ftrans, the layout fast_out_input writes: (Nk-1) x Nt entries [t][k2][k1]

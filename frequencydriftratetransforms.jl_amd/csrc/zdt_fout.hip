@@ -1,6 +1,6 @@
-// Fused output for power-of-two Nf in [2^16, 2^22] and Nt <= 64, where the
-// chirp-Z coefficients of a bin do not fit k2's registers (Nt > 16) or the
-// rows do not fit its LDS (Nf > 2^20):
+// Fused output for power-of-two Nf in [2^16, 2^22] and Nt <= FOUT_MAX_NT,
+// where the chirp-Z coefficients of a bin do not fit k2's registers (Nt > 16)
+// or the rows do not fit its LDS (Nf > 2^20):
 //   k1_input + k2_input   input!: the four-step r2c into Ft[t][k2][k1]
 //   czt_reinsch_col       the chirp-Z per bin -> Yc[j][k2][k1]
 //   k2_c2r / k2_c2r_wave  c2r pre-pass and inverse DFT_N1 -> B (as k2)
@@ -121,6 +121,57 @@ __global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
     bin.dst[static_cast<int64_t>(j) * bin.stride] = yz;
     if (jp != j) bin.dst[static_cast<int64_t>(jp) * bin.stride] = yc;
     z = cmul(z, g);
+  }
+}
+
+// Nt > 64: czt_reinsch_col over chunks of 64 time steps, sum_t c_t z^t =
+// sum_q z^{64q} P_q(z), with z^{64q} (conj for the partner rate) from an FP64
+// recurrence over the rates; chunk 0 writes Yc, later chunks add to it.
+__global__ __launch_bounds__(256) void czt_reinsch_col_chunks(CztColArgs a) {
+  constexpr int NT = 64;
+  ColBin bin;
+  if (!col_bin(a, bin)) return;
+  const double kd = static_cast<double>(bin.k);
+  const float w = fast_window(a.wkind, a.wcustom, bin.k, a.Nf) * a.scale;
+  const c64 g = cis64f(kd * a.hd2);
+  const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
+  for (int t0 = 0; t0 < a.Nt; t0 += NT) {
+    float cr[NT], ci[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const int t = t0 + u;
+      if (t < a.Nt) {
+        c32 f = a.F[bin.fo + static_cast<int64_t>(t) * bin.fs];
+        if (a.hc != 0.0) f = cmul(f, cis_nh(kd * static_cast<double>(t), a.hc));  // uniform
+        cr[u] = f.x * w;
+        ci[u] = f.y * w;
+      } else {
+        cr[u] = 0.0f;
+        ci[u] = 0.0f;
+      }
+    }
+    const double td = static_cast<double>(t0);
+    c64 z = cis64f(kd * a.hz);            // e^{2 pi i k (r_0 - rc)/Nf}
+    c64 zq = cis64f(kd * a.hz * td);      // z^{t0} of rate 0
+    const c64 gq = cis64f(kd * a.hd2 * td);  // gamma^{t0}
+    for (int j = 0; j < npair; ++j) {
+      const int jp = a.Nr - 1 - j;
+      c32 yz, yc;
+      reinsch_split<NT>(cr, ci, z, yz, yc);
+      const c32 q = {static_cast<float>(zq.x), static_cast<float>(zq.y)};
+      yz = cmul(yz, q);
+      yc = cmulc(yc, q);
+      c32* dj = bin.dst + static_cast<int64_t>(j) * bin.stride;
+      c32* dp = bin.dst + static_cast<int64_t>(jp) * bin.stride;
+      if (t0 > 0) {
+        yz = cadd(yz, *dj);
+        if (jp != j) yc = cadd(yc, *dp);
+      }
+      *dj = yz;
+      if (jp != j) *dp = yc;
+      z = cmul(z, g);
+      zq = cmul(zq, gq);
+    }
   }
 }
 
@@ -576,7 +627,7 @@ namespace zdt {
 
 bool fast_out_supported(int64_t Nf, int64_t Nt, int64_t Nr) {
   if (Nf < (1 << 16) || Nf > (1 << 22) || (Nf & (Nf - 1)) != 0) return false;
-  return Nt >= 1 && Nt <= 64 && Nr >= 1 && Nr <= 65535;
+  return Nt >= 1 && Nt <= FOUT_MAX_NT && Nr >= 1 && Nr <= 65535;
 }
 
 int64_t fast_out_bytes(int64_t Nf, int64_t Nr) {
@@ -645,7 +696,7 @@ static void launch_k2i(const K2iArgs& a, hipStream_t st) {
 }
 
 int fast_out_input(FastPlan* p, const float* spec, int64_t Nt, c32* Ft, hipStream_t st) {
-  if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%lld", (long long)Nt);
+  if (Nt < 1 || Nt > FOUT_MAX_NT) return fail(ZDT_ENOTSUP, "fused output: Nt=%lld", (long long)Nt);
   ZDT_TRY(launch_k1(p, spec, Ft, Nt, st));
   K2iArgs a;
   a.G = Ft;
@@ -695,7 +746,7 @@ int fast_out_unpack_F(FastPlan* p, const c32* Ft, int64_t Nt, c32* F, hipStream_
 
 int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, float r0, float dr, int wkind,
                  const float* window, float* dest, double scale, hipStream_t st, double* colmean, double* colvar) {
-  if (Nt < 1 || Nt > 64) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
+  if (Nt < 1 || Nt > FOUT_MAX_NT) return fail(ZDT_ENOTSUP, "fused output: Nt=%d", Nt);
   CztColArgs c;
   c.F = F;
   c.ldF = ldF;
@@ -723,8 +774,10 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, fl
       czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);
     } else if (Nt <= 32) {
       czt_reinsch_col<32><<<grid, 256, 0, st>>>(c);
-    } else {
+    } else if (Nt <= 64) {
       czt_reinsch_col<64><<<grid, 256, 0, st>>>(c);
+    } else {
+      czt_reinsch_col_chunks<<<grid, 256, 0, st>>>(c);
     }
     ZDT_CHECK_LAUNCH("czt_reinsch_col");
   }

@@ -146,10 +146,10 @@ def test_fast_unnormalized_and_hits(F):
     np.testing.assert_array_equal(F.findprotohits(fdr, 10, snr=True), H.findprotohits(host(fdr), 10, snr=True))
 
 
-# ---- per-bin chirp-Z with the fused output (power-of-two Nf, 16 < Nt <= 64)
+# ---- per-bin chirp-Z with the fused output (power-of-two Nf, Nt > 16)
 
 @pytest.mark.parametrize("Nf,Nt", [(1 << 16, 17), (1 << 17, 32), (1 << 18, 23), (1 << 19, 60), (1 << 20, 33),
-                                   (1 << 16, 64)])
+                                   (1 << 16, 64), (1 << 16, 150), (1 << 17, 100)])
 def test_fused_output_vs_fp64(F, Nf, Nt):
     # rates -2:1/8:2 pair as r_j = -r_{32-j} (the paired-rate evaluation)
     Nr = 33
@@ -157,6 +157,19 @@ def test_fused_output_vs_fp64(F, Nf, Nt):
     say(f"fused output Nf={Nf} Nt={Nt}")
     out = host(F.zdtfdr(F.ZDTWorkspace(s, -2.0, 0.125, Nr)))
     assert normwise(out, Z.fdr64(s, -2.0, 0.125, Nr)) <= 2e-6
+
+
+def test_fused_output_chunked_asymmetric_rates(F):
+    """Nt = 129 (chirp-Z chunks of 64, 64 and 1 steps) with rates not
+    symmetric about 0 (centre rate 4), against the oracle and the generic
+    kernels."""
+    Nf, Nt, r0, dr, Nr = 1 << 16, 129, 1.5, 0.25, 21
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.3, 3.0, 25.0), (Nf * 0.6, 5.25, 20.0)], seed=129)
+    say("fused output Nt=129, rates 1.5:0.25:6.5")
+    out = host(F.zdtfdr(F.ZDTWorkspace(s, r0, dr, Nr)))
+    assert normwise(out, Z.fdr64(s, r0, dr, Nr)) <= 2e-6
+    g = host(F.zdtfdr(F.ZDTWorkspace(s, r0, dr, Nr, generic=True)))
+    assert normwise(out, g.astype(np.float64)) <= 2e-6
 
 
 def test_fused_output_window_nodest_unnormalized(F):
