@@ -299,7 +299,7 @@ def main(argv=None) -> int:
 
     def create(flags):
         h = C.c_void_p()
-        _lib.check(L.zdt_create(spec.data_ptr(), NF, NT, R0, DR, NR, arr, n, flags, stream, C.byref(h)))
+        _lib.check(L.zdt_create(spec.data_ptr(), NF, NT, R0, DR, NR, arr, n, flags, 0, stream, C.byref(h)))
         return h
 
     hp, h = create(base), create(base | _lib.ZDT_FLAG_PROFILE)

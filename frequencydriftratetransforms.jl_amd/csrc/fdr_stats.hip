@@ -14,6 +14,8 @@
 // is read about once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "zdt_kernels.h"
 
 namespace zdt {
@@ -26,55 +28,58 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// one block per (chunk of STAT_ROWS rows, column); vec: the column is
-// 16-byte aligned (float4 loads)
+// one block per (chunk of STAT_ROWS rows, column); grid.y is capped at the
+// launch limit, so a block walks the columns j = blockIdx.y + q*gridDim.y;
+// vec: the columns are 16-byte aligned (float4 loads)
 __global__ __launch_bounds__(STAT_THREADS) void colstats_partial(const float* __restrict__ fdr, int64_t Nf,
-                                                                 int64_t ld, double* __restrict__ part,
+                                                                 int64_t Nr, int64_t ld, double* __restrict__ part,
                                                                  int nchunk, int vec) {
   const int chunk = blockIdx.x;
-  const int64_t j = blockIdx.y;
-  const float* col = fdr + j * ld;
-  const double c = static_cast<double>(col[0]);
-  const int64_t r0 = static_cast<int64_t>(chunk) * STAT_ROWS;
-  int64_t r1 = r0 + STAT_ROWS;
-  if (r1 > Nf) r1 = Nf;
-  double s1 = 0.0, s2 = 0.0;
-  if (vec && r1 - r0 == STAT_ROWS) {
-    const float4* p = reinterpret_cast<const float4*>(col + r0);
-    constexpr int NV = STAT_ROWS / 4 / STAT_THREADS;
-#pragma unroll 8
-    for (int k = 0; k < NV; ++k) {
-      const float4 v = p[threadIdx.x + k * STAT_THREADS];
-      const double d0 = static_cast<double>(v.x) - c, d1 = static_cast<double>(v.y) - c;
-      const double d2 = static_cast<double>(v.z) - c, d3 = static_cast<double>(v.w) - c;
-      s1 += (d0 + d1) + (d2 + d3);
-      s2 += (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
-    }
-  } else {
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += STAT_THREADS) {
-      const double d = static_cast<double>(col[i]) - c;
-      s1 += d;
-      s2 += d * d;
-    }
-  }
-  s1 = wave_sum(s1);
-  s2 = wave_sum(s2);
   constexpr int NW = STAT_THREADS / 64;
   __shared__ double sh1[NW], sh2[NW];
   const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sh1[wv] = s1;
-    sh2[wv] = s2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t1 = 0.0, t2 = 0.0;
-    for (int w = 0; w < NW; ++w) {
-      t1 += sh1[w];
-      t2 += sh2[w];
+  for (int64_t j = blockIdx.y; j < Nr; j += gridDim.y) {  // uniform per block
+    const float* col = fdr + j * ld;
+    const double c = static_cast<double>(col[0]);
+    const int64_t r0 = static_cast<int64_t>(chunk) * STAT_ROWS;
+    int64_t r1 = r0 + STAT_ROWS;
+    if (r1 > Nf) r1 = Nf;
+    double s1 = 0.0, s2 = 0.0;
+    if (vec && r1 - r0 == STAT_ROWS) {
+      const float4* p = reinterpret_cast<const float4*>(col + r0);
+      constexpr int NV = STAT_ROWS / 4 / STAT_THREADS;
+#pragma unroll 8
+      for (int k = 0; k < NV; ++k) {
+        const float4 v = p[threadIdx.x + k * STAT_THREADS];
+        const double d0 = static_cast<double>(v.x) - c, d1 = static_cast<double>(v.y) - c;
+        const double d2 = static_cast<double>(v.z) - c, d3 = static_cast<double>(v.w) - c;
+        s1 += (d0 + d1) + (d2 + d3);
+        s2 += (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
+      }
+    } else {
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += STAT_THREADS) {
+        const double d = static_cast<double>(col[i]) - c;
+        s1 += d;
+        s2 += d * d;
+      }
     }
-    part[(j * nchunk + chunk) * 2 + 0] = t1;
-    part[(j * nchunk + chunk) * 2 + 1] = t2;
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if ((threadIdx.x & 63) == 0) {
+      sh1[wv] = s1;
+      sh2[wv] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t1 = 0.0, t2 = 0.0;
+      for (int w = 0; w < NW; ++w) {
+        t1 += sh1[w];
+        t2 += sh2[w];
+      }
+      part[(j * nchunk + chunk) * 2 + 0] = t1;
+      part[(j * nchunk + chunk) * 2 + 1] = t2;
+    }
+    __syncthreads();  // sh1/sh2 are rewritten for the next column
   }
 }
 
@@ -101,9 +106,9 @@ int colstats_nchunk(int64_t Nf) { return static_cast<int>((Nf + STAT_ROWS - 1) /
 int launch_colstats2(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double* part, double* colmean,
                      double* colvar, hipStream_t st) {
   const int nchunk = colstats_nchunk(Nf);
-  dim3 grid(nchunk, static_cast<unsigned>(Nr));
+  dim3 grid(nchunk, static_cast<unsigned>(std::min<int64_t>(Nr, 65535)));
   const int vec = (ld % 4 == 0 && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
-  colstats_partial<<<grid, STAT_THREADS, 0, st>>>(fdr, Nf, ld, part, nchunk, vec);
+  colstats_partial<<<grid, STAT_THREADS, 0, st>>>(fdr, Nf, Nr, ld, part, nchunk, vec);
   ZDT_CHECK_LAUNCH("colstats_partial");
   colstats_final<<<static_cast<unsigned>((Nr + 255) / 256), 256, 0, st>>>(fdr, Nf, Nr, ld, part, nchunk,
                                                                           colmean, colvar);
@@ -272,11 +277,13 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restric
                                                           int64_t ld, double thr_h, const double* thr_d,
                                                           const int64_t* __restrict__ offsets, int64_t ntiles,
                                                           const int64_t* __restrict__ total, int64_t base_index,
+                                                          const int64_t* __restrict__ pos_base,
                                                           int64_t* __restrict__ hits, int64_t cap, int vec) {
   const double thr = thr_d ? *thr_d : thr_h;
   const int64_t tile = blockIdx.x;
-  const int64_t start = offsets[tile];
-  const int64_t end = (tile + 1 < ntiles) ? offsets[tile + 1] : *total;
+  const int64_t pb = pos_base ? *pos_base : 0;  // hits already written by earlier matrices
+  const int64_t start = offsets[tile] + pb;
+  const int64_t end = ((tile + 1 < ntiles) ? offsets[tile + 1] : *total) + pb;
   if (end == start || start >= cap) return;  // uniform: no hits here (or out of room)
   // thread handles HIT_PER_THREAD consecutive elements, so a block scan of
   // the per-thread counts gives each hit its ordered slot
@@ -346,7 +353,7 @@ int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, int64_t* scratc
 int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr,
                            const double* thr_dev, const int64_t* offsets, int64_t ntiles_all, int64_t tile0,
                            const int64_t* total_dev, int64_t base_index, int64_t* hits, int64_t cap,
-                           hipStream_t st) {
+                           hipStream_t st, const int64_t* pos_base) {
   const int64_t nt = hits_tiles(Nf, Nr);
   if (nt == 0) return ZDT_OK;
   // offsets of this matrix's tiles start at tile0 of the global scan; the end
@@ -354,8 +361,18 @@ int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld,
   const int vec = (ld == Nf && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
   hits_write<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev,
                                                                 offsets + tile0, ntiles_all - tile0, total_dev,
-                                                                base_index, hits, cap, vec);
+                                                                base_index, pos_base, hits, cap, vec);
   ZDT_CHECK_LAUNCH("hits_write");
+  return ZDT_OK;
+}
+
+__global__ void add_i64(int64_t* __restrict__ acc, const int64_t* __restrict__ v, int init) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *acc = (init ? 0 : *acc) + *v;
+}
+
+int launch_accumulate_i64(int64_t* acc, const int64_t* v, bool init, hipStream_t st) {
+  add_i64<<<1, 64, 0, st>>>(acc, v, init ? 1 : 0);
+  ZDT_CHECK_LAUNCH("add_i64");
   return ZDT_OK;
 }
 

@@ -168,6 +168,7 @@ struct zdt_ws {
   int64_t bytes = 0;
   Profiler* prof = nullptr;
   int maxr = 16;  // radix cap of the generic FFTs (autotuned by plans)
+  int precision = ZDT_PRECISION_FP32;
 };
 
 namespace {
@@ -335,7 +336,7 @@ extern "C" {
 
 const char* zdt_last_error(void) { return g_err.c_str(); }
 
-int zdt_version(void) { return 1; }
+int zdt_version(void) { return 2; }
 
 int64_t zdt_calc_nl(int64_t Nt, int64_t Nr, const int* factors, int nfactors) {
   std::vector<int> f;
@@ -375,9 +376,12 @@ namespace zdt {
 // zdt_create with the tunables the plan autotuner chooses: n2 (the fused
 // path's strided FFT length, 0 = default) and maxr (generic FFT radix cap).
 int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, const int* factors,
-                    int nfactors, int flags, void* stream, int n2, int maxr, zdt_ws** out) {
+                    int nfactors, int flags, int precision, void* stream, int n2, int maxr, zdt_ws** out) {
   if (out == nullptr) return fail(ZDT_EINVAL, "out is NULL");
   *out = nullptr;
+  if (precision != ZDT_PRECISION_FP32 && precision != ZDT_PRECISION_SPLIT_BF16 &&
+      precision != ZDT_PRECISION_SPLIT_FP16)
+    return fail(ZDT_EINVAL, "unknown precision %d", precision);
   if (Nf < 1 || Nt < 1 || Nr < 1)
     return fail(ZDT_EINVAL, "invalid shape Nf=%lld Nt=%lld Nr=%lld", (long long)Nf, (long long)Nt, (long long)Nr);
   if (Nt > (1 << 20) || Nr > (1 << 24)) return fail(ZDT_ENOTSUP, "Nt or Nr too large");
@@ -403,8 +407,13 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
   if (hipGetDevice(&ws->device) != hipSuccess) rc = fail(ZDT_EHIP, "hipGetDevice failed");
   if (flags & ZDT_FLAG_PROFILE) ws->prof = new Profiler();
   ws->maxr = maxr;
-  if (rc == ZDT_OK && !(flags & ZDT_FLAG_GENERIC) && fast_supported(Nf, Nt, Nr)) {
-    rc = fast_create(Nf, Nt, Nr, dr, n2, &ws->fast);
+  ws->precision = precision;
+  const bool fused = !(flags & ZDT_FLAG_GENERIC) && fast_supported(Nf, Nt, Nr);
+  if (rc == ZDT_OK && precision != ZDT_PRECISION_FP32 && !(fused && fast_precision_ok(Nf, n2, precision)))
+    rc = fail(ZDT_ENOTSUP, "precision %d needs the fused path (power-of-two Nf in [2^16, 2^20], Nt <= 16) "
+              "with an MFMA variant for Nf=%lld", precision, (long long)Nf);
+  if (rc == ZDT_OK && fused) {
+    rc = fast_create(Nf, Nt, Nr, dr, n2, precision, &ws->fast);
     if (rc == ZDT_OK) ws->bytes += fast_bytes(Nf, Nt, Nr);
   } else if (rc == ZDT_OK) {
     const bool fout = !(flags & ZDT_FLAG_GENERIC) && fast_out_supported(Nf, Nt, Nr);
@@ -477,7 +486,7 @@ int ws_path(const zdt_ws* ws) {
 extern "C" {
 
 int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, const int* factors,
-               int nfactors, int flags, void* stream, zdt_ws** out) {
+               int nfactors, int flags, int precision, void* stream, zdt_ws** out) {
   int n2 = 0;
   if (flags & ZDT_FLAG_N2_512) {
     if (!zdt::fast_n2_ok(Nf, 512) || (Nf & (Nf - 1)) != 0)
@@ -486,7 +495,7 @@ int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, in
   }
   const int maxr = (flags & ZDT_FLAG_RADIX8) ? 8 : 16;
   return zdt::create_ws_tuned(spec, Nf, Nt, r0, dr, Nr, factors, nfactors, flags & ~(ZDT_FLAG_N2_512 | ZDT_FLAG_RADIX8),
-                              stream, n2, maxr, out);
+                              precision, stream, n2, maxr, out);
 }
 
 int zdt_destroy(zdt_ws* ws) {
@@ -515,6 +524,13 @@ int zdt_info(const zdt_ws* ws, int64_t* Nf, int64_t* Nt, float* r0, float* dr, i
   if (dr) *dr = ws->dr;
   if (Nr) *Nr = ws->Nr;
   if (Nl) *Nl = ws->Nl;
+  return ZDT_OK;
+}
+
+int zdt_path(const zdt_ws* ws, int* path, int* precision) {
+  ZDT_TRY(check_ws(ws));
+  if (path) *path = ws_path(ws);
+  if (precision) *precision = ws->precision;
   return ZDT_OK;
 }
 

@@ -11,7 +11,9 @@ bool fast_supported(int64_t Nf, int64_t Nt, int64_t Nr);
 int64_t fast_bytes(int64_t Nf, int64_t Nt, int64_t Nr);
 // n2: the strided FFT length of k1/k3 (512 or 1024; 0 = the default for Nf)
 bool fast_n2_ok(int64_t Nf, int n2);
-int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, FastPlan** out);
+// precision: ZDT_PRECISION_* (the split modes where fast_precision_ok says so)
+bool fast_precision_ok(int64_t Nf, int n2, int precision);
+int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, int precision, FastPlan** out);
 void fast_destroy(FastPlan* p);
 // input!: spectrogram (device Nf x Nt) -> the plan's input-spectrum buffer
 int fast_input(FastPlan* p, const float* spec, hipStream_t st);

@@ -696,11 +696,20 @@ bool fast_n2_ok(int64_t Nf, int n2) {
   return (n2 == 1024 && N1 >= 64 && N1 <= 512) || (n2 == 512 && N1 >= 64 && N1 <= 512);
 }
 
-int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, FastPlan** out) {
+bool fast_precision_ok(int64_t Nf, int n2, int precision) {
+  (void)Nf;
+  (void)n2;
+  return precision == ZDT_PRECISION_FP32;
+}
+
+int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, int precision, FastPlan** out) {
   *out = nullptr;
   if (!fast_supported(Nf, Nt, Nr)) return fail(ZDT_ENOTSUP, "fused path: unsupported shape");
   if (!fast_n2_ok(Nf, n2)) return fail(ZDT_ENOTSUP, "fused path: N2=%d not available for Nf=%lld", n2, (long long)Nf);
+  if (!fast_precision_ok(Nf, n2, precision))
+    return fail(ZDT_ENOTSUP, "fused path: precision %d not available for Nf=%lld", precision, (long long)Nf);
   FastPlan* p = new FastPlan();
+  p->precision = precision;
   p->Nf = Nf;
   p->Nt = Nt;
   p->Nr = Nr;

@@ -32,6 +32,7 @@ struct FastPlan {
   bool paired = false;     // the last k2 wrote B in mirrored rate pairs
   bool have_B = false;
   int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
+  int precision = 0;              // ZDT_PRECISION_* of the DFT stages
 };
 
 int ilog2i(int64_t n);

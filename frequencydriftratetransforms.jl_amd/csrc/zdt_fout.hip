@@ -18,9 +18,10 @@ namespace zdt {
 
 // ---------------------------------------------------------------- fused output for Nt > 16
 //
-// For power-of-two Nf with Nt up to 64 (more coefficients than k2 keeps in
-// registers) the chirp-Z runs per bin (czt_reinsch_col) and the output irfft
-// reuses the second half of k2 and k3:
+// For power-of-two Nf with Nt up to FOUT_MAX_NT (more coefficients than k2
+// keeps in registers; above 64 the chirp-Z runs in 64-step chunks) the
+// chirp-Z runs per bin (czt_reinsch_col) and the output irfft reuses the
+// second half of k2 and k3:
 //   czt_reinsch_col  Ys[k, j] -> Yc[j][k2][k1] (k = k2 + N2 k1 < M), Ys[M, j] -> yM[j]
 //   k2_c2r           per Hermitian column pair: c2r pre-pass, inverse DFT_N1
 //                    over k1 (one wavefront per row) -> B, bin M/2 -> yhalf
@@ -769,7 +770,11 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, fl
   c.hz = (static_cast<double>(r0) - rc) / static_cast<double>(p->Nf);
   {
     const unsigned grid = static_cast<unsigned>(p->M / 256 + 1);
-    ProfScope ps(st, "czt_reinsch_col", 8.0 * (p->M + 1) * (Nt + p->Nr));
+    // bytes: F read once; Yc written by the first 64-step chunk and read and
+    // rewritten by each further chunk
+    const int nchunk = (Nt + 63) / 64;
+    ProfScope ps(st, Nt <= 64 ? "czt_reinsch_col" : "czt_reinsch_col_chunks",
+                 8.0 * (p->M + 1) * (Nt + p->Nr * (2.0 * nchunk - 1.0)));
     if (Nt <= 16) {
       czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);
     } else if (Nt <= 32) {

@@ -55,10 +55,15 @@ int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, doub
 // scratch: hits_scan_scratch(n) int64 words
 int64_t hits_scan_scratch(int64_t n);
 int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, int64_t* scratch, hipStream_t st);
+// offsets: the scanned tile counts, this matrix's tiles from tile0 on; hits
+// land at offsets + *pos_base (pos_base: device, NULL = 0) as linear indices
+// + base_index, at most cap of them
 int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr,
                            const double* thr_dev, const int64_t* offsets, int64_t ntiles_all, int64_t tile0,
                            const int64_t* total_dev, int64_t base_index, int64_t* hits, int64_t cap,
-                           hipStream_t st);
+                           hipStream_t st, const int64_t* pos_base = nullptr);
+// *acc = (init ? 0 : *acc) + *v on the device
+int launch_accumulate_i64(int64_t* acc, const int64_t* v, bool init, hipStream_t st);
 // *p = v by a kernel (graph-capturable, unlike a host-to-device copy)
 int launch_set_double(double* p, double v, hipStream_t st);
 // thr = snr * sqrt(min colvar) + colmean[0], on the device

@@ -206,7 +206,8 @@ __global__ __launch_bounds__(256) void czt_reinsch_chunks(CztArgs a) {
 int launch_czt(const CztArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>((a.Nk + 255) / 256);
   if (a.Nt > 64) {
-    ProfScope ps(st, "czt_reinsch_chunks", 8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr * ((a.Nt + 63) / 64)));
+    ProfScope ps(st, "czt_reinsch_chunks",
+                 8.0 * static_cast<double>(a.Nk) * (a.Nt + a.Nr * (2.0 * ((a.Nt + 63) / 64) - 1.0)));
     czt_reinsch_chunks<<<grid, 256, 0, st>>>(a);
     ZDT_CHECK_LAUNCH("czt_reinsch_chunks");
     return ZDT_OK;

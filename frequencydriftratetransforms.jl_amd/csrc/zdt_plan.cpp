@@ -75,7 +75,8 @@ size_t stats_doubles(const Shape& s) {
 // Times input! + one zdtfdr! batch of a candidate workspace (ms, best of 3).
 int time_candidate(const Shape& s, int flags, int n2, int maxr, const float* spec, float* dest, float* ms_out) {
   zdt_ws* ws = nullptr;
-  ZDT_TRY(create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, flags, s.st, n2, maxr, &ws));
+  ZDT_TRY(create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, flags, ZDT_PRECISION_FP32, s.st, n2, maxr,
+                          &ws));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = ZDT_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(ZDT_EHIP, "plan: event");
@@ -106,6 +107,9 @@ int autotune(Shape& s, int flags) {
   if (!(flags & ZDT_FLAG_GENERIC) && fast_supported(s.Nf, s.Nt, s.Nr)) {
     for (int n2 : {1024, 512})
       if (fast_n2_ok(s.Nf, n2)) cands.push_back({1, n2, 16});
+  } else if (!(flags & ZDT_FLAG_GENERIC) && fast_out_supported(s.Nf, s.Nt, s.Nr)) {
+    // the fused output has one shape per Nf and runs no radix-capped FFT
+    // in the transform: nothing to choose
   } else {
     for (int maxr : {16, 8}) cands.push_back({0, 0, maxr});
   }
@@ -259,8 +263,8 @@ int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* 
     s.fused = (!(wflags & ZDT_FLAG_GENERIC) && fast_supported(s.Nf, s.Nt, s.Nr)) ? 1 : 0;
     if (flags & ZDT_PLAN_AUTOTUNE) rc = autotune(s, wflags);
     if (rc != ZDT_OK) break;
-    rc = create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, wflags, s.st, s.fused ? s.n2 : 0, s.maxr,
-                         &s.ws);
+    rc = create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, wflags, ZDT_PRECISION_FP32, s.st,
+                         s.fused ? s.n2 : 0, s.maxr, &s.ws);
     if (rc != ZDT_OK) break;
     s.fused = ws_path(s.ws);
     rc = hip_alloc(reinterpret_cast<void**>(&s.stats), stats_doubles(s) * sizeof(double));
@@ -285,8 +289,9 @@ int zdt_plan_info(const zdt_plan* p, int ch, int* shape, int* path, int* n2, int
   const Shape& s = p->shapes[si];
   if (shape) *shape = si;
   if (path) *path = s.fused;
-  if (n2) *n2 = s.fused == 1 ? (s.n2 ? s.n2 : (s.Nf >= (int64_t(1) << 17) ? 1024 : 512)) : 0;
-  if (maxr) *maxr = s.fused == 1 ? 0 : s.maxr;
+  const int n2_default = s.Nf >= (int64_t(1) << 17) ? 1024 : 512;
+  if (n2) *n2 = s.fused == 1 ? (s.n2 ? s.n2 : n2_default) : (s.fused == 2 ? n2_default : 0);
+  if (maxr) *maxr = s.fused != 0 ? 0 : s.maxr;
   return ZDT_OK;
 }
 

@@ -12,7 +12,14 @@
 // budget allows; otherwise a first pass over the batches gathers the
 // statistics (fdrstats decomposes: mean of the first column, minimum of the
 // per-column deviations) and a second pass recomputes each batch for its
-// hits.
+// hits.  Statistics, threshold and hit extraction stay on the device: the
+// per-batch column statistics land side by side, one kernel forms
+// thr = snr*s + m over the requested columns, and each batch's hits are
+// counted, scanned and written after the previous batches' (a device-side
+// running total), so the host waits once per spectrogram, and only after the
+// next spectrogram's work is queued.  When Nr is not a multiple of the batch
+// length the last batch's extra columns (rates past r0 + (Nr-1)*dr) are
+// computed but take no part in the statistics or the hits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +29,7 @@
 #include <vector>
 
 #include "zdt_internal.h"
+#include "zdt_kernels.h"
 #include "zdt_ws.h"
 
 struct zdt_stream {
@@ -31,15 +39,16 @@ struct zdt_stream {
   bool resident = false;
   int64_t bytes = 0;
   zdt_ws* ws = nullptr;
-  hipStream_t compute = nullptr, copy = nullptr;
-  hipEvent_t copied[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
+  hipStream_t compute = nullptr, copy = nullptr, readback = nullptr;
+  hipEvent_t copied[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
   float* spec[2] = {nullptr, nullptr};
-  std::vector<float*> fdr;  // resident batch FDRs (or one reused buffer)
-  int64_t* dhits = nullptr;  // device hit staging
+  std::vector<float*> fdr;             // resident batch FDRs (or one reused buffer)
+  int64_t* dhits[2] = {nullptr, nullptr};  // device hit staging, one per spectrogram in flight
   int64_t dcap = 0;
-  double* dstats = nullptr;  // colmean, colvar (Nrb each), then generic-path scratch
+  int64_t* counts = nullptr;  // one batch's tile counts, scan scratch, batch total, running total
+  double* dstats = nullptr;   // colmean, colvar (nb*Nrb each), thr, then generic-path scratch
+  void* hsmall = nullptr;     // pinned: per slot {thr (double), total (int64)}
 };
-
 namespace zdt {
 namespace {
 
@@ -61,7 +70,9 @@ int64_t stream_bytes(int64_t Nf, int64_t Nt, int64_t Nr, int64_t Nrb, bool resid
   if (ws < 0) return -1;
   const int64_t nb = (Nr + Nrb - 1) / Nrb;
   const int64_t fdrs = (resident ? nb : 1) * Nf * Nrb * 4;
-  return ws + fdrs + 2 * Nf * Nt * 4 + dcap * 8 + (2 * Nrb + devstats_scratch_doubles(Nf, Nrb)) * 8;
+  const int64_t nt = hits_tiles(Nf, Nrb);
+  return ws + fdrs + 2 * Nf * Nt * 4 + 2 * dcap * 8 + (nt + hits_scan_scratch(nt) + 2) * 8 +
+         (2 * nb * Nrb + 1 + devstats_scratch_doubles(Nf, Nrb)) * 8;
 }
 
 }  // namespace
@@ -96,8 +107,15 @@ int zdt_stream_destroy(zdt_stream* s) {
     if (s->copied[i]) (void)hipEventDestroy(s->copied[i]);
     if (s->used[i]) (void)hipEventDestroy(s->used[i]);
   }
-  if (s->dhits) (void)hipFree(s->dhits);
+  if (s->readback) (void)hipStreamSynchronize(s->readback);
+  for (int i = 0; i < 2; ++i) {
+    if (s->dhits[i]) (void)hipFree(s->dhits[i]);
+    if (s->done[i]) (void)hipEventDestroy(s->done[i]);
+  }
+  if (s->counts) (void)hipFree(s->counts);
   if (s->dstats) (void)hipFree(s->dstats);
+  if (s->hsmall) (void)hipHostFree(s->hsmall);
+  if (s->readback) (void)hipStreamDestroy(s->readback);
   if (s->copy) (void)hipStreamDestroy(s->copy);
   if (s->compute) (void)hipStreamDestroy(s->compute);
   delete s;
@@ -113,7 +131,7 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
                 (long long)Nr, (long long)Nrb);
   if (flags & ~(ZDT_FLAG_UNNORMALIZED | ZDT_FLAG_GENERIC)) return fail(ZDT_EINVAL, "stream: unsupported flags");
   if (!std::isfinite(r0) || !std::isfinite(dr)) return fail(ZDT_EINVAL, "stream: r0 and dr must be finite");
-  const int64_t dcap = int64_t(1) << 20;  // device hit staging per batch pass
+  const int64_t dcap = int64_t(1) << 20;  // initial device hit staging per spectrogram (grows)
   int64_t budget = hbm_bytes;
   if (budget == 0) {
     size_t fr = 0, tot = 0;
@@ -158,15 +176,18 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
   int rc = ZDT_OK;
   if (hipGetDevice(&s->device) != hipSuccess) rc = fail(ZDT_EHIP, "stream: hipGetDevice");
   if (rc == ZDT_OK && (hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking) != hipSuccess ||
-                       hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess))
+                       hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess ||
+                       hipStreamCreateWithFlags(&s->readback, hipStreamNonBlocking) != hipSuccess))
     rc = fail(ZDT_EHIP, "stream: stream creation failed");
   for (int i = 0; i < 2 && rc == ZDT_OK; ++i) {
     if (hipEventCreateWithFlags(&s->copied[i], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s->used[i], hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&s->used[i], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->done[i], hipEventDisableTiming) != hipSuccess)
       rc = fail(ZDT_EHIP, "stream: event creation failed");
     if (rc == ZDT_OK) rc = salloc(s, reinterpret_cast<void**>(&s->spec[i]), static_cast<size_t>(Nf * Nt) * 4);
   }
-  if (rc == ZDT_OK) rc = zdt_create(nullptr, Nf, Nt, r0, dr, s->Nrb, nullptr, 0, flags, s->compute, &s->ws);
+  if (rc == ZDT_OK) rc = zdt_create(nullptr, Nf, Nt, r0, dr, s->Nrb, nullptr, 0, flags, ZDT_PRECISION_FP32, s->compute,
+                                      &s->ws);
   if (rc == ZDT_OK) {
     int64_t wb = 0;
     (void)zdt_sizeof(s->ws, &wb);
@@ -178,10 +199,19 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
     rc = salloc(s, reinterpret_cast<void**>(&f), static_cast<size_t>(Nf * s->Nrb) * 4);
     if (rc == ZDT_OK) s->fdr.push_back(f);
   }
-  if (rc == ZDT_OK) rc = salloc(s, reinterpret_cast<void**>(&s->dhits), static_cast<size_t>(dcap) * 8);
+  for (int i = 0; i < 2 && rc == ZDT_OK; ++i)
+    rc = salloc(s, reinterpret_cast<void**>(&s->dhits[i]), static_cast<size_t>(dcap) * 8);
+  const int64_t nt = hits_tiles(Nf, s->Nrb);
+  if (rc == ZDT_OK)
+    rc = salloc(s, reinterpret_cast<void**>(&s->counts), static_cast<size_t>(nt + hits_scan_scratch(nt) + 2) * 8);
   if (rc == ZDT_OK)
     rc = salloc(s, reinterpret_cast<void**>(&s->dstats),
-                static_cast<size_t>(2 * s->Nrb + devstats_scratch_doubles(Nf, s->Nrb)) * sizeof(double));
+                static_cast<size_t>(2 * s->nb * s->Nrb + 1 + devstats_scratch_doubles(Nf, s->Nrb)) * sizeof(double));
+  if (rc == ZDT_OK && hipHostMalloc(&s->hsmall, 2 * 16, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    s->hsmall = nullptr;
+    rc = fail(ZDT_ENOMEM, "stream: pinned staging allocation failed");
+  }
   if (rc != ZDT_OK) {
     std::string msg = zdt_last_error();
     zdt_stream_destroy(s);
@@ -210,7 +240,30 @@ int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, 
     if (host_specs[i] == nullptr) return fail(ZDT_EINVAL, "stream_search: spectrogram %d is NULL", i);
   ZDT_TRY_HIP(hipSetDevice(s->device));
   const size_t spec_bytes = static_cast<size_t>(s->Nf * s->Nt) * 4;
-  const int64_t Nf = s->Nf, Nrb = s->Nrb;
+  const int64_t Nf = s->Nf, Nrb = s->Nrb, nb = s->nb;
+  // device hit staging of cap entries per spectrogram in flight
+  if (cap > s->dcap) {
+    ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
+    ZDT_TRY_HIP(hipStreamSynchronize(s->readback));
+    for (int k = 0; k < 2; ++k) {
+      (void)hipFree(s->dhits[k]);
+      s->dhits[k] = nullptr;
+    }
+    s->bytes -= 2 * s->dcap * 8;
+    s->dcap = 0;
+    for (int k = 0; k < 2; ++k) ZDT_TRY(salloc(s, reinterpret_cast<void**>(&s->dhits[k]), static_cast<size_t>(cap) * 8));
+    s->dcap = cap;
+  }
+  double* colmean = s->dstats;
+  double* colvar = colmean + nb * Nrb;
+  double* thr_dev = colvar + nb * Nrb;
+  double* scratch = thr_dev + 1;
+  const int64_t nt_full = hits_tiles(Nf, Nrb);
+  int64_t* counts = s->counts;
+  int64_t* scan_scratch = counts + nt_full;
+  int64_t* batch_total = scan_scratch + hits_scan_scratch(nt_full);
+  int64_t* running = batch_total + 1;
+
   auto enqueue_copy = [&](int i) -> int {
     const int slot = i & 1;
     ZDT_TRY_HIP(hipStreamWaitEvent(s->copy, s->used[slot], 0));
@@ -218,79 +271,75 @@ int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, 
     ZDT_TRY_HIP(hipEventRecord(s->copied[slot], s->copy));
     return ZDT_OK;
   };
-  // the events start out "recorded" so the first waits pass
-  for (int i = 0; i < 2; ++i) ZDT_TRY_HIP(hipEventRecord(s->used[i], s->compute));
-  ZDT_TRY(enqueue_copy(0));
-  int rc_trunc = ZDT_OK;
-  for (int i = 0; i < nspec; ++i) {
+  auto r0b = [&](int64_t b) {
+    return static_cast<float>(static_cast<double>(s->r0) + static_cast<double>(b * Nrb) * static_cast<double>(s->dr));
+  };
+  // columns of batch b that belong to the requested rates
+  auto valid = [&](int64_t b) { return std::min(Nrb, s->Nr - b * Nrb); };
+  // the whole search of spectrogram i, enqueued on the compute stream
+  auto enqueue_search = [&](int i) -> int {
     const int slot = i & 1;
-    if (i + 1 < nspec) ZDT_TRY(enqueue_copy(i + 1));  // flies while spectrogram i is searched
     ZDT_TRY_HIP(hipStreamWaitEvent(s->compute, s->copied[slot], 0));
     ZDT_TRY(zdt_input(s->ws, s->spec[slot]));
     ZDT_TRY_HIP(hipEventRecord(s->used[slot], s->compute));
-    auto r0b = [&](int64_t b) {
-      return static_cast<float>(static_cast<double>(s->r0) + static_cast<double>(b * Nrb) * static_cast<double>(s->dr));
-    };
-    // statistics over all batches: mean of the first column, least deviation
-    double m = 0.0, sd = INFINITY;
-    bool nan = false;
-    std::vector<double> hst(static_cast<size_t>(2 * Nrb));
-    for (int64_t b = 0; b < s->nb; ++b) {
+    // statistics of every batch, side by side: the first Nr entries are the
+    // requested rates' columns
+    for (int64_t b = 0; b < nb; ++b) {
       float* f = s->fdr[s->resident ? b : 0];
-      ZDT_TRY(run_devstats(s->ws, r0b(b), f, s->dstats, s->dstats + Nrb, s->dstats + 2 * Nrb));
-      ZDT_TRY_HIP(hipMemcpyAsync(hst.data(), s->dstats, hst.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                 s->compute));
-      ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
-      if (b == 0) m = hst[0];
-      for (int64_t j = 0; j < Nrb; ++j) {
-        const double v = hst[static_cast<size_t>(Nrb + j)];
-        if (std::isnan(v)) nan = true;
-        sd = std::min(sd, std::sqrt(std::max(v, 0.0)));
-      }
+      ZDT_TRY(run_devstats(s->ws, r0b(b), f, colmean + b * Nrb, colvar + b * Nrb, scratch));
     }
-    const double t = snr * (nan ? NAN : sd) + m;  // fdrdenormalize (src/fdrutils.jl:104-107)
-    if (thr) thr[i] = t;
+    ZDT_TRY(launch_hits_threshold(colmean, colvar, s->Nr, snr, thr_dev, s->compute));
     // hits in column-major order of the hcat'd batches
-    int64_t total = 0;
-    int64_t* out = hits ? hits + static_cast<int64_t>(i) * cap : nullptr;
-    for (int64_t b = 0; b < s->nb; ++b) {
+    for (int64_t b = 0; b < nb; ++b) {
       float* f = s->fdr[s->resident ? b : 0];
       if (!s->resident) ZDT_TRY(zdt_run(s->ws, r0b(b), ZDT_WINDOW_RECT, nullptr, f));
-      const float* fl[1] = {f};
-      int64_t nb_hits = 0;
-      const int64_t left = std::max<int64_t>(0, cap - total);
-      int64_t room = std::min(left, s->dcap);
-      int rc = zdt_findprotohits(fl, 1, Nf, Nrb, Nf, t, s->dhits, room, &nb_hits, s->compute);
-      if (rc != ZDT_OK && rc != ZDT_ETRUNC) return rc;
-      if (nb_hits > room && room < left) {
-        // more hits than the staging buffer holds: grow it and extract again
-        const int64_t need = std::min(nb_hits, left);
-        ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
-        (void)hipFree(s->dhits);
-        s->bytes -= s->dcap * 8;
-        s->dhits = nullptr;
-        s->dcap = 0;
-        ZDT_TRY(salloc(s, reinterpret_cast<void**>(&s->dhits), static_cast<size_t>(need) * 8));
-        s->dcap = need;
-        room = need;
-        rc = zdt_findprotohits(fl, 1, Nf, Nrb, Nf, t, s->dhits, room, &nb_hits, s->compute);
-        if (rc != ZDT_OK && rc != ZDT_ETRUNC) return rc;
-      }
-      const int64_t got = std::min(nb_hits, room);
-      if (got > 0) {
-        std::vector<int64_t> tmp(static_cast<size_t>(got));
-        ZDT_TRY_HIP(hipMemcpyAsync(tmp.data(), s->dhits, static_cast<size_t>(got) * 8, hipMemcpyDeviceToHost,
-                                   s->compute));
-        ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
-        const int64_t off = b * Nrb * Nf;
-        for (int64_t h = 0; h < got; ++h) out[total + h] = tmp[static_cast<size_t>(h)] + off;
-      }
-      total += nb_hits;
+      const int64_t nv = valid(b);
+      const int64_t nt = hits_tiles(Nf, nv);
+      ZDT_TRY(launch_hits_count(f, Nf, nv, Nf, 0.0, thr_dev, counts, s->compute));
+      ZDT_TRY(launch_hits_scan(counts, nt, batch_total, scan_scratch, s->compute));
+      if (cap > 0)
+        ZDT_TRY(launch_hits_write_impl(f, Nf, nv, Nf, 0.0, thr_dev, counts, nt, 0, batch_total, b * Nrb * Nf,
+                                       s->dhits[slot], cap, s->compute, b == 0 ? nullptr : running));
+      ZDT_TRY(launch_accumulate_i64(running, batch_total, b == 0, s->compute));
     }
+    char* hs = static_cast<char*>(s->hsmall) + 16 * slot;
+    ZDT_TRY_HIP(hipMemcpyAsync(hs, thr_dev, sizeof(double), hipMemcpyDeviceToHost, s->compute));
+    ZDT_TRY_HIP(hipMemcpyAsync(hs + 8, running, sizeof(int64_t), hipMemcpyDeviceToHost, s->compute));
+    ZDT_TRY_HIP(hipEventRecord(s->done[slot], s->compute));
+    return ZDT_OK;
+  };
+  int rc_trunc = ZDT_OK;
+  // results of spectrogram i, once its work is done (the next one is queued)
+  auto finish = [&](int i) -> int {
+    const int slot = i & 1;
+    ZDT_TRY_HIP(hipEventSynchronize(s->done[slot]));
+    const char* hs = static_cast<const char*>(s->hsmall) + 16 * slot;
+    double t;
+    int64_t total;
+    std::memcpy(&t, hs, sizeof(double));
+    std::memcpy(&total, hs + 8, sizeof(int64_t));
+    if (thr) thr[i] = t;
     nhits[i] = total;
-    if (total > cap) rc_trunc = fail(ZDT_ETRUNC, "stream_search: spectrogram %d has %lld hits, capacity %lld", i,
-                                     (long long)total, (long long)cap);
+    const int64_t got = std::min(total, cap);
+    if (got > 0) {
+      ZDT_TRY_HIP(hipMemcpyAsync(hits + static_cast<int64_t>(i) * cap, s->dhits[slot], static_cast<size_t>(got) * 8,
+                                 hipMemcpyDeviceToHost, s->readback));
+      ZDT_TRY_HIP(hipStreamSynchronize(s->readback));
+    }
+    if (total > cap)
+      rc_trunc = fail(ZDT_ETRUNC, "stream_search: spectrogram %d has %lld hits, capacity %lld", i, (long long)total,
+                      (long long)cap);
+    return ZDT_OK;
+  };
+  // the events start out "recorded" so the first waits pass
+  for (int i = 0; i < 2; ++i) ZDT_TRY_HIP(hipEventRecord(s->used[i], s->compute));
+  ZDT_TRY(enqueue_copy(0));
+  for (int i = 0; i < nspec; ++i) {
+    if (i + 1 < nspec) ZDT_TRY(enqueue_copy(i + 1));  // flies while spectrogram i is searched
+    ZDT_TRY(enqueue_search(i));
+    if (i > 0) ZDT_TRY(finish(i - 1));
   }
+  ZDT_TRY(finish(nspec - 1));
   ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
   return rc_trunc;
 }

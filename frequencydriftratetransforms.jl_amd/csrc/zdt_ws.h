@@ -9,7 +9,7 @@ namespace zdt {
 // path's strided FFT length (512 or 1024; 0 = the default for Nf), and maxr,
 // the generic FFTs' largest power-of-two radix (16, 8 or 4).
 int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, const int* factors,
-                    int nfactors, int flags, void* stream, int n2, int maxr, zdt_ws** out);
+                    int nfactors, int flags, int precision, void* stream, int n2, int maxr, zdt_ws** out);
 
 // true when the workspace runs the fused kernels
 bool ws_is_fused(const zdt_ws* ws);

@@ -33,6 +33,17 @@ ZDT_FLAG_RADIX8 = 0x10
 
 ZDT_PLAN_AUTOTUNE = 0x100
 
+ZDT_PRECISION_FP32 = 0
+ZDT_PRECISION_SPLIT_BF16 = 1
+ZDT_PRECISION_SPLIT_FP16 = 2
+PRECISIONS = {"fp32": ZDT_PRECISION_FP32, "split-bf16": ZDT_PRECISION_SPLIT_BF16,
+              "split-fp16": ZDT_PRECISION_SPLIT_FP16}
+
+ZDT_PATH_GENERIC = 0
+ZDT_PATH_FUSED = 1
+ZDT_PATH_FUSED_OUTPUT = 2
+PATHS = {ZDT_PATH_GENERIC: "generic", ZDT_PATH_FUSED: "fused", ZDT_PATH_FUSED_OUTPUT: "fused-output"}
+
 ZDT_BUF_F = 0
 ZDT_BUF_Y = 1
 ZDT_BUF_V = 2
@@ -51,9 +62,10 @@ SIGNATURES = {
     "zdt_estimate_memory": (_i64, [_i64, _i64, _i64, _i64, _i64, C.POINTER(C.c_int), C.c_int]),
     "zdt_workspace_bytes": (_i64, [_i64, _i64, _i64, C.POINTER(C.c_int), C.c_int, C.c_int]),
     "zdt_create": (C.c_int, [_fp, _i64, _i64, C.c_float, C.c_float, _i64, C.POINTER(C.c_int), C.c_int,
-                             C.c_int, _vp, C.POINTER(_vp)]),
+                             C.c_int, C.c_int, _vp, C.POINTER(_vp)]),
     "zdt_destroy": (C.c_int, [_vp]),
     "zdt_info": (C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_float), C.POINTER(C.c_float), _i64p, _i64p]),
+    "zdt_path": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "zdt_sizeof": (C.c_int, [_vp, _i64p]),
     "zdt_set_stream": (C.c_int, [_vp, _vp]),
     "zdt_sync": (C.c_int, [_vp]),

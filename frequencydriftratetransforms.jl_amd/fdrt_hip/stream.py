@@ -32,8 +32,10 @@ class StreamSearch:
     ``Nrb=0`` and ``hbm_bytes=0`` let the library size the batches to 90% of
     the free device memory.  ``search(specs, snr)`` returns one (H, 2) array
     of 1-based (channel, rate-column) indices per spectrogram (columns of the
-    hcat'd batches, the last batch padded past Nr like ``batchrates``) and the
-    thresholds used.
+    hcat'd batches, i.e. rate index + 1) and the thresholds used.  When Nr is
+    not a multiple of the batch length the last batch computes rates past
+    r0 + (Nr-1)*dr, but only the Nr requested columns enter the statistics
+    and the hits.
     """
 
     def __init__(self, Nf: int, Nt: int, rates, *, Nrb: int = 0, hbm_bytes: int = 0, generic: bool = False,

@@ -61,11 +61,14 @@ class ZDTWorkspace:
     (test/runtests.jl:31 convention); ``generic=True`` forces the unfused
     kernels; ``n2=512`` selects the fused path's 512-point strided FFTs
     where 1024 is the default; ``radix_cap=8`` limits the generic FFTs to
-    radices up to 8 (16 is the default).
+    radices up to 8 (16 is the default); ``precision`` is "fp32" (default),
+    "split-bf16" or "split-fp16" (the fused path's DFT stages as MFMA
+    products of split hi+lo operands).
     """
 
     def __init__(self, spectrogram, *args, factors=None, output_aligned: bool = False,
-                 unnormalized: bool = False, generic: bool = False, n2: int = 0, radix_cap: int = 16):
+                 unnormalized: bool = False, generic: bool = False, n2: int = 0, radix_cap: int = 16,
+                 precision: str = "fp32"):
         if len(args) in (1, 2) and not isinstance(args[0], (int, float, np.floating, np.integer)):
             r0, dr, Nr = rates_params(args[0])
             if len(args) == 2:
@@ -90,11 +93,14 @@ class ZDTWorkspace:
                  | (_lib.ZDT_FLAG_N2_512 if n2 == 512 else 0) | (_lib.ZDT_FLAG_RADIX8 if radix_cap == 8 else 0))
         if radix_cap not in (8, 16):
             raise ValueError("radix_cap must be 8 or 16")
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
         arr, n = _lib.int_array(self.factors)
         h = C.c_void_p()
         self._h = None
         _lib.check(_lib.lib().zdt_create(spec.data_ptr(), self.Nf, self.Nt, float(self.r0), float(self.dr), self.Nr,
-                                         arr, n, flags, C.c_void_p(self._stream), C.byref(h)))
+                                         arr, n, flags, _lib.PRECISIONS[precision], C.c_void_p(self._stream),
+                                         C.byref(h)))
         self._h = h
         nl = C.c_int64()
         _lib.check(_lib.lib().zdt_info(self._h, None, None, None, None, None, C.byref(nl)))
@@ -127,6 +133,21 @@ class ZDTWorkspace:
     @property
     def Nk(self) -> int:
         return self.Nf // 2 + 1
+
+    @property
+    def path(self) -> str:
+        """Kernel path the library runs for this workspace: "fused",
+        "fused-output" or "generic" (zdt_path)."""
+        v = C.c_int()
+        _lib.check(_lib.lib().zdt_path(self.handle, C.byref(v), None))
+        return _lib.PATHS[v.value]
+
+    @property
+    def precision(self) -> str:
+        """Arithmetic of the DFT stages: "fp32", "split-bf16" or "split-fp16"."""
+        v = C.c_int()
+        _lib.check(_lib.lib().zdt_path(self.handle, None, C.byref(v)))
+        return {b: a for a, b in _lib.PRECISIONS.items()}[v.value]
 
     def _buffer(self, which: int) -> torch.Tensor:
         p = C.c_void_p()

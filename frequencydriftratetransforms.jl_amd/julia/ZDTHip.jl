@@ -151,9 +151,9 @@ function ZDTWorkspace(spec::HipMatrix{Float32}, r0::Real, δr::Real, Nr::Integer
     h = Ref{Ptr{Cvoid}}(C_NULL)
     flags = unnormalized ? FLAG_UNNORMALIZED : 0
     check(ccall((:zdt_create, LIBZDT), Cint,
-                (Ptr{Float32}, Int64, Int64, Cfloat, Cfloat, Int64, Ptr{Cint}, Cint, Cint, Ptr{Cvoid},
+                (Ptr{Float32}, Int64, Int64, Cfloat, Cfloat, Int64, Ptr{Cint}, Cint, Cint, Cint, Ptr{Cvoid},
                  Ref{Ptr{Cvoid}}),
-                spec.ptr, Nf, Nt, Float32(r0), Float32(δr), Nr, f, length(f), flags, stream, h))
+                spec.ptr, Nf, Nt, Float32(r0), Float32(δr), Nr, f, length(f), flags, Cint(0), stream, h))
     nl = Ref{Int64}(0)
     check(ccall((:zdt_info, LIBZDT), Cint,
                 (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Cfloat}, Ptr{Cfloat}, Ptr{Int64}, Ref{Int64}),

@@ -56,6 +56,19 @@ extern "C" {
 #define ZDT_FLAG_RADIX8 0x10      /* generic FFTs: radices up to 8 (16 is the  *
                                    * default; ragged plans time both)          */
 
+/* arithmetic of the fused path's frequency-axis DFT stages (zdt_create's
+ * precision argument); the chirp-Z, the phases and every other step stay FP32
+ * with FP64 phase arguments in all modes */
+#define ZDT_PRECISION_FP32 0       /* FP32 butterflies (the default)            */
+#define ZDT_PRECISION_SPLIT_BF16 1 /* DFT stages as BF16 MFMA products of split *
+                                    * hi+lo operands, FP32 accumulation         */
+#define ZDT_PRECISION_SPLIT_FP16 2 /* the same with FP16 hi+lo operands         */
+
+/* kernel paths reported by zdt_path */
+#define ZDT_PATH_GENERIC 0      /* Stockham / four-step FFTs + per-bin chirp-Z  */
+#define ZDT_PATH_FUSED 1        /* k1_input, k2_czt, k3_output                  */
+#define ZDT_PATH_FUSED_OUTPUT 2 /* per-bin chirp-Z + the fused c2r (k2_c2r, k3) */
+
 /* buffers exposed by zdt_buffer */
 #define ZDT_BUF_F 0  /* Nk x Nt ComplexF32: input FFT (natural layout)        */
 #define ZDT_BUF_Y 1  /* Nk x Nl ComplexF32: CZT work matrix (step functions)  */
@@ -94,10 +107,11 @@ int64_t zdt_workspace_bytes(int64_t Nf, int64_t Nt, int64_t Nr,
  * Computes Nl, allocates the device buffers, precomputes the chirp kernel and,
  * when spec is not NULL, runs input! on it.  stream is a hipStream_t (NULL =
  * the default stream).  The rates are r0 + j*dr, j < Nr, with r0/dr taken as
- * Float32 (src/zdtfdr.jl:13-14). */
+ * Float32 (src/zdtfdr.jl:13-14).  precision is one of ZDT_PRECISION_*; the
+ * split modes exist for the fused path only (ZDT_ENOTSUP elsewhere). */
 int zdt_create(const float* spec, int64_t Nf, int64_t Nt, float r0, float dr,
                int64_t Nr, const int* factors, int nfactors, int flags,
-               void* stream, zdt_ws** out);
+               int precision, void* stream, zdt_ws** out);
 
 /* Frees every device buffer of the workspace (the Julia finalizer). */
 int zdt_destroy(zdt_ws* ws);
@@ -105,6 +119,12 @@ int zdt_destroy(zdt_ws* ws);
 /* Shape of a workspace (Base.show, src/zdtfdr.jl:89-99). Any pointer may be NULL. */
 int zdt_info(const zdt_ws* ws, int64_t* Nf, int64_t* Nt, float* r0, float* dr,
              int64_t* Nr, int64_t* Nl);
+
+/* The kernel path (ZDT_PATH_*) and precision (ZDT_PRECISION_*) a workspace
+ * runs; either pointer may be NULL.  No counterpart in the reference, whose
+ * only GPU path is the generic one (ext/CUDAFrequencyDriftRateTransformsExt.jl):
+ * it lets hosts and tests confirm which kernels serve a shape. */
+int zdt_path(const zdt_ws* ws, int* path, int* precision);
 
 /* Device bytes held by the workspace (Base.sizeof, src/zdtfdr.jl:101-103). */
 int zdt_sizeof(const zdt_ws* ws, int64_t* bytes);
@@ -227,9 +247,12 @@ int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* 
 int zdt_plan_destroy(zdt_plan* p);
 
 /* The choices made for channel c: shape index (channels with one shape share
- * a workspace), path (1 = fused; 2 = per-bin chirp-Z with the fused output,
- * power-of-two Nf with 16 < Nt <= 64; 0 = generic), n2 (fused) and maxr
- * (the radix cap of the other paths' FFTs). */
+ * a workspace), path (ZDT_PATH_*: 1 = fused, power-of-two Nf in [2^16, 2^20]
+ * with Nt <= 16; 2 = per-bin chirp-Z with the fused output, power-of-two Nf in
+ * [2^16, 2^22] with Nt <= 1024 and Nr <= 65535 where path 1 does not apply;
+ * 0 = generic), n2 (the strided FFT length of paths 1 and 2) and maxr (the
+ * radix cap of the generic path's FFTs; 0 on paths 1 and 2, which do not use
+ * them for the transform). */
 int zdt_plan_info(const zdt_plan* p, int ch, int* shape, int* path, int* n2, int* maxr);
 
 /* Binds device buffers and captures the graph (no work is done yet).

@@ -16,8 +16,9 @@ zdt_oracle
     ``fftfdr``/``fdshift``.
 host_oracle
     Host-side arithmetic: ``calcNl``, ``growNr``, ``estimate_memory``,
-    ``driftrates``, ``batchrates``, ``fdrstats``, ``fdrnormalize``,
-    ``fdrdenormalize`` and ``findprotohits``.
+    ``batchrates``, ``fdrstats``, ``fdrnormalize``, ``fdrdenormalize`` and
+    ``findprotohits``.  (``driftrates`` needs no restatement: the GPU test
+    checks the product's ranges against hand-derived exact values.)
 synth
     Seeded synthetic spectrograms (Exponential(1) noise plus drifting tones).
 

@@ -64,11 +64,6 @@ def estimate_memory(Nf: int, Nt: int, Nr: int, Ni: int = 1, No: int = 1,
     return 4 * ((Nf * Nt) * Ni + (Nf * Nl) * Nz + (Nf * Nr) * No)
 
 
-def driftrates(r0, dr, Nr: int) -> List[float]:
-    """range(r0, step=dr, length=Nr) (src/zdtutils.jl:62-64), float64 values."""
-    return [float(r0) + j * float(dr) for j in range(Nr)]
-
-
 def _round_from_zero(x: float) -> int:
     return int(math.copysign(math.ceil(abs(x)), x)) if x != int(x) else int(x)
 

@@ -154,15 +154,24 @@ def fdr64(spec: np.ndarray, r0, dr=None, Nr: Optional[int] = None,
         rl = [exact(r) for r in rates]
     F = np.fft.rfft(spec.astype(np.float64), axis=0)
     k = np.arange(Nk, dtype=np.int64)
-    t = np.arange(Nt, dtype=np.int64)
-    kt = k[:, None] * t[None, :]
     w = window64(window, Nf)
     out = np.empty((Nf, len(rl)), dtype=np.float64)
     for j, r in enumerate(rl):
-        q = r.denominator
-        num = _mul_exact(kt, r.numerator)
-        ph = cis_revs(_revs(num, Nf * q))
-        S = np.sum(F * ph, axis=1)
+        den = Nf * r.denominator
+        S = F[:, 0].copy()  # t = 0: phase 1
+        if Nt > 1 and den < (1 << 61):
+            # the phase numerator of (k, t) is t*step_k mod den, stepped
+            # exactly in int64 (every value stays below 2*den < 2^62)
+            step = np.mod(_mul_exact(k, r.numerator), den).astype(np.int64)
+            acc = np.zeros(Nk, dtype=np.int64)
+            for t in range(1, Nt):
+                acc += step
+                acc[acc >= den] -= den
+                S += F[:, t] * cis_revs(_revs(acc, den))
+        elif Nt > 1:
+            t = np.arange(1, Nt, dtype=np.int64)
+            num = _mul_exact(k[:, None] * t[None, :], r.numerator)
+            S += np.sum(F[:, 1:] * cis_revs(_revs(num, den)), axis=1)
         if w is not None:
             S = S * w
         out[:, j] = np.fft.irfft(S, n=Nf)

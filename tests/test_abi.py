@@ -74,19 +74,22 @@ def test_bad_factor_rejected(L):
 def test_create_validation_errors_before_device(L):
     from fdrt_hip import _lib
     h = C.c_void_p()
-    rc = L.zdt_create(None, 0, 16, 0.0, 1.0, 10, None, 0, 0, None, C.byref(h))
+    rc = L.zdt_create(None, 0, 16, 0.0, 1.0, 10, None, 0, 0, 0, None, C.byref(h))
     assert rc == _lib.ZDT_EINVAL
     assert "invalid shape" in L.zdt_last_error().decode()
-    rc = L.zdt_create(None, 17 * 4, 16, 0.0, 1.0, 10, None, 0, 0, None, C.byref(h))
+    rc = L.zdt_create(None, 17 * 4, 16, 0.0, 1.0, 10, None, 0, 0, 0, None, C.byref(h))
     assert rc == _lib.ZDT_ENOTSUP
     assert "prime factor" in L.zdt_last_error().decode()
     arr, n = _f((0,))
-    rc = L.zdt_create(None, 64, 16, 0.0, 1.0, 10, arr, n, 0, None, C.byref(h))
+    rc = L.zdt_create(None, 64, 16, 0.0, 1.0, 10, arr, n, 0, 0, None, C.byref(h))
     assert rc == _lib.ZDT_EINVAL
-    rc = L.zdt_create(None, 64, 16, float("nan"), 1.0, 10, None, 0, 0, None, C.byref(h))
+    rc = L.zdt_create(None, 64, 16, float("nan"), 1.0, 10, None, 0, 0, 0, None, C.byref(h))
     assert rc == _lib.ZDT_EINVAL
-    rc = L.zdt_create(None, 64, 16, 0.0, 1.0, 10, None, 0, 0, None, None)
+    rc = L.zdt_create(None, 64, 16, 0.0, 1.0, 10, None, 0, 0, 0, None, None)
     assert rc == _lib.ZDT_EINVAL
+    rc = L.zdt_create(None, 1 << 16, 16, 0.0, 1.0, 10, None, 0, 0, 7, None, C.byref(h))
+    assert rc == _lib.ZDT_EINVAL
+    assert "precision" in L.zdt_last_error().decode()
 
 
 def test_null_workspace_errors(L):
@@ -94,6 +97,8 @@ def test_null_workspace_errors(L):
     assert L.zdt_run(None, 0.0, 0, None, None) == _lib.ZDT_EINVAL
     assert L.zdt_destroy(None) == _lib.ZDT_OK
     assert L.zdt_fdrstats(None, 0, 10, 10, 10, None, None, None) == _lib.ZDT_EINVAL
+    assert L.zdt_path(None, None, None) == _lib.ZDT_EINVAL
+    assert L.zdt_version() == 2
 
 
 def test_plan_validation_errors_before_device(L):

@@ -35,10 +35,6 @@ def F(gpu):
     return fdrt_hip
 
 
-def fused(F, ws):
-    return F.workspace_bytes(ws.Nf, ws.Nt, ws.Nr) < F.workspace_bytes(ws.Nf, ws.Nt, ws.Nr, generic=True) or True
-
-
 @pytest.mark.parametrize("Nf", [1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20])
 @pytest.mark.parametrize("Nt", [16, 7, 1])
 def test_fast_sizes(F, Nf, Nt):
@@ -46,6 +42,7 @@ def test_fast_sizes(F, Nf, Nt):
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.31, 1.5, 25.0), (Nf * 0.77, -2.75, 40.0)], seed=Nf % 1000 + Nt)
     say(f"fast Nf={Nf} Nt={Nt} Nr={Nr}")
     ws = F.ZDTWorkspace(s, -3.0, 0.1875, Nr)
+    assert ws.path == "fused"
     out = host(F.zdtfdr(ws))
     ref = Z.fdr64(s, -3.0, 0.1875, Nr)
     assert normwise(out, ref) <= 2e-6
@@ -89,8 +86,11 @@ def test_fast_vs_generic(F, Nt):
     Nf = 1 << 17
     s = synth.spectrogram(Nf, Nt, [(5000.0, 0.5, 30.0)], seed=40 + Nt)
     say(f"fast vs generic Nt={Nt}")
-    a = host(F.zdtfdr(F.ZDTWorkspace(s, -4.0, 0.0625, 129)))
-    b = host(F.zdtfdr(F.ZDTWorkspace(s, -4.0, 0.0625, 129, generic=True)))
+    wa = F.ZDTWorkspace(s, -4.0, 0.0625, 129)
+    wb = F.ZDTWorkspace(s, -4.0, 0.0625, 129, generic=True)
+    assert (wa.path, wb.path) == ("fused", "generic")
+    a = host(F.zdtfdr(wa))
+    b = host(F.zdtfdr(wb))
     assert normwise(a, b.astype(np.float64)) <= 2e-6
 
 
@@ -155,8 +155,36 @@ def test_fused_output_vs_fp64(F, Nf, Nt):
     Nr = 33
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.37, 1.25, 25.0), (Nf * 0.81, -2.0, 30.0)], seed=Nf % 991 + Nt)
     say(f"fused output Nf={Nf} Nt={Nt}")
-    out = host(F.zdtfdr(F.ZDTWorkspace(s, -2.0, 0.125, Nr)))
+    ws = F.ZDTWorkspace(s, -2.0, 0.125, Nr)
+    assert ws.path == "fused-output"
+    out = host(F.zdtfdr(ws))
     assert normwise(out, Z.fdr64(s, -2.0, 0.125, Nr)) <= 2e-6
+
+
+@pytest.mark.parametrize("Nf,Nt,r0,dr,Nr", [(1 << 16, 1024, -0.5, 1.0 / 64, 5), (1 << 18, 128, -1.0, 1.0 / 32, 9),
+                                            (1 << 22, 80, -3.0, 1.5, 3), (1 << 21, 65, 0.25, 0.5, 4)])
+def test_fused_output_long_integrations(F, Nf, Nt, r0, dr, Nr):
+    """The chunked column chirp-Z (64-step chunks) at its Nt = 1024 limit,
+    at 2^18 x 128, and at N1 = 2048 and 1024 (2^22 and 2^21 channels) with
+    more than 64 steps."""
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.29, 0.375, 8.0), (Nf * 0.71, -1.0, 6.0)], seed=Nt + Nr)
+    say(f"fused output long Nf={Nf} Nt={Nt} Nr={Nr}")
+    ws = F.ZDTWorkspace(s, r0, dr, Nr)
+    assert ws.path == "fused-output"
+    out = host(F.zdtfdr(ws))
+    assert normwise(out, Z.fdr64(s, r0, dr, Nr)) <= 2e-6
+
+
+@pytest.mark.parametrize("Nf,Nt", [(6000, 1100), (1 << 16, 1100)])
+def test_generic_many_steps(F, Nf, Nt):
+    """More than 1024 steps (past the fused output's limit) and a
+    non-power-of-two Nf: the generic path's chunked chirp-Z, 18 chunks."""
+    r0, dr, Nr = -0.25, 1.0 / 128, 7
+    s = synth.spectrogram(Nf, Nt, [(Nf * 0.4, 0.125, 5.0)], seed=Nf % 97)
+    say(f"generic many steps Nf={Nf} Nt={Nt}")
+    ws = F.ZDTWorkspace(s, r0, dr, Nr)
+    assert ws.path == "generic"
+    assert normwise(host(F.zdtfdr(ws)), Z.fdr64(s, r0, dr, Nr)) <= 2e-6
 
 
 def test_fused_output_chunked_asymmetric_rates(F):

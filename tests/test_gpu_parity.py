@@ -324,18 +324,89 @@ def test_hit_set_identity_vs_fp64(F):
     assert (g ^ r) <= band
 
 
-# ------------------------------------------------------------ full size, sampled
+# ------------------------------------------------------------ full size (the headline workload)
 
-def test_full_size_sampled_columns(F):
+CFG2 = json.loads((Path(__file__).parent / "golden" / "cfg2_hits_snr8.json").read_text())
+
+
+@pytest.fixture(scope="module")
+def cfg2(F):
+    """2^20 x 16 spectrogram (16 tones, seed 1234), rates -4:1/16:4: the
+    spectrogram, its tone rates and the device FDR of the fused path."""
+    from oracle.synth import random_tones
     s = synth.cfg2(seed=1234)
-    say("cfg2 2^20 x 16 x 129 sampled columns")
+    tones = random_tones(1 << 20, 16, (10, 50), (-4, 4), 1234)
+    say("cfg2 2^20 x 16 x 129")
     ws = F.ZDTWorkspace(s, -4.0, 0.0625, 129)
-    out = host(F.zdtfdr(ws))
-    cols = [0, 17, 64, 100, 128]
+    assert ws.path == "fused" and ws.precision == "fp32"
+    fdr = F.zdtfdr(ws)
+    return s, [r for _, r, _ in tones], fdr
+
+
+def test_full_size_columns_vs_fp64(F, cfg2):
+    """Normwise <= 2e-6 against fdr64 on 19 columns: the first, middle and
+    last rates and the rate nearest each of the 16 injected tones."""
+    s, tone_rates, fdr = cfg2
+    cols = sorted({0, 64, 128} | {int(round((r + 4.0) * 16)) for r in tone_rates})
+    assert len(cols) >= 16
+    out = host(fdr[:, cols])
     rates = [Fraction(-4) + Fraction(j, 16) for j in cols]
     ref = Z.fdr64(s, None, rates=rates)
-    assert normwise(out[:, cols], ref) <= 2e-6
-    # a checksum-of-checksums property: the DC of each column equals the sum
-    # of the spectrogram (every drift line visits each time step once)
-    colsum = out.astype(np.float64).sum(axis=0)
+    err = normwise(out, ref)
+    say(f"cfg2 {len(cols)} columns: normwise {err:.2e}")
+    assert err <= 2e-6
+    # a checksum-of-checksums property over all 129 columns: the DC of each
+    # column equals the sum of the spectrogram (every drift line visits each
+    # time step once)
+    colsum = host(fdr.double().sum(dim=0))
     np.testing.assert_allclose(colsum, s.astype(np.float64).sum(), rtol=1e-6)
+    np.testing.assert_allclose(CFG2["colsum_fdr64"], s.astype(np.float64).sum(), rtol=1e-9)
+
+
+def test_full_size_hit_set_vs_fp64(F, cfg2):
+    """findprotohits(fdr, 8; snr=true) at 2^20 x 16 x 129 against the FP64
+    oracle's hit set (tests/golden/cfg2_hits_snr8.json, written by
+    make_cfg2_hits.py): the symmetric difference lies inside the tie band
+    |fdr64 - thr64| <= eps, eps = 4 ulp(thr64) + 2e-6 max|fdr64| (the FDR
+    tolerance) + |thr - thr64|."""
+    s, _, fdr = cfg2
+    hits = F.findprotohits(fdr, 8, snr=True)
+    thr = F.fdrdenormalize(8, fdr)
+    thr64 = CFG2["thr64"]
+    assert abs(thr - thr64) <= 1e-5 * thr64
+    eps = 4 * float(np.spacing(np.float32(thr64))) + 2e-6 * CFG2["max_abs_fdr64"] + abs(thr - thr64)
+    assert eps <= CFG2["band_abs"]
+    band = {(i, j) for i, j, d in CFG2["band"] if abs(d) <= eps}
+    g = set(map(tuple, hits.tolist()))
+    r = set(map(tuple, CFG2["hits"]))
+    say(f"cfg2 hits: {len(g)} (FP64 {len(r)}), symmetric difference {len(g ^ r)}, band {len(band)}")
+    assert len(r) > 0 and (g ^ r) <= band
+    # the hits are listed in column-major order, as findall returns them
+    lin = (hits[:, 0] - 1) + (1 << 20) * (hits[:, 1] - 1)
+    assert np.all(np.diff(lin) > 0)
+
+
+def test_driftrates_and_fdrsynchronize(F):
+    """driftrates(ws; r0) is range(r0, step=ws.dr, length=ws.Nr)
+    (src/zdtutils.jl:62-64) with the workspace's Float32 r0/dr
+    (src/zdtfdr.jl:13-14); fdrsynchronize waits for the device
+    (ext/CUDAFrequencyDriftRateTransformsExt.jl:26-28)."""
+    s = synth.spectrogram(4096, 8, [(1000.0, 0.5, 20.0)], seed=2)
+    ws = F.ZDTWorkspace(s, -4.0, 0.0625, 129)
+    r = F.driftrates(ws)
+    assert len(r) == 129 and r[0] == -4 and r[-1] == 4 and r[64] == 0
+    assert r.step == Fraction(1, 16)
+    r2 = F.driftrates(ws, -1.5)
+    assert r2[0] == Fraction(-3, 2) and r2[128] == Fraction(13, 2)
+    # a step that is not dyadic is held as its Float32 value
+    ws3 = F.ZDTWorkspace(s, Fraction(-8, 3), Fraction(1, 15), 81)
+    r3 = F.driftrates(ws3)
+    assert r3.first == Fraction(float(np.float32(np.float32(-8) / np.float32(3))))
+    assert r3.step == Fraction(float(np.float32(np.float32(1) / np.float32(15))))
+    assert len(r3) == 81
+    # fdrsynchronize: after it, a result written by the device is visible on the host
+    dest = F.create_fdr(s, 129)
+    F.zdtfdr_(dest, ws)
+    F.fdrsynchronize(ws)
+    F.fdrsynchronize()
+    assert np.isfinite(host(dest)).all()

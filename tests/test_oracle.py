@@ -118,6 +118,102 @@ def test_ref32_close_to_fp64():
     assert normwise(a, b) < 5e-6
 
 
+# ---- the reference's Float32 phase arguments, derived by hand
+#
+# Julia evaluates `-k * l^2 * δr / Nf` (src/zdtfdr.jl:141) left to right:
+# the Int64 product -k*l^2, promoted to Float32 (round to nearest even) for
+# the product with the Float32 δr (rounded), then divided by Float32(Nf)
+# (exact for a power of two).  `k * l * (l * δr + 2 * r0) / Nf` (:197) and
+# `k * l * l * δr / Nf` (:269) likewise.  rne24 below rounds an exact
+# rational to Float32 with integer arithmetic only (no numpy float32).
+
+def rne24(x: Fraction) -> Fraction:
+    if x == 0:
+        return Fraction(0)
+    sgn = -1 if x < 0 else 1
+    x = abs(x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    if Fraction(2) ** e > x:
+        e -= 1
+    if Fraction(2) ** (e + 1) <= x:
+        e += 1
+    q = x / Fraction(2) ** (e - 23)  # in [2^23, 2^24)
+    m = q.numerator // q.denominator
+    rem = q - m
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and m % 2 == 1):
+        m += 1
+    return sgn * m * Fraction(2) ** (e - 23)
+
+
+def wrap2(x: Fraction) -> float:
+    """x mod 2 into [-1, 1): the cispi argument's angle over pi."""
+    y = x - 2 * ((x + 1) // 2)
+    return float(y)
+
+
+def angle_over_pi(z) -> float:
+    return float(np.angle(np.complex128(z)) / np.pi)
+
+
+def test_rne24_hand_values():
+    assert rne24(Fraction(2 ** 24 + 1)) == 2 ** 24          # tie to even
+    assert rne24(Fraction(2 ** 24 + 3)) == 2 ** 24 + 4      # tie to even (up)
+    assert rne24(Fraction(2 ** 25 + 3)) == 2 ** 25 + 4      # above the tie
+    assert rne24(Fraction(1, 3)) == Fraction(11184811, 2 ** 25)
+
+
+# (k, l, discriminating): where the Float32 argument differs from the exact
+# one by more than twice the check's tolerance, so an exact-phase oracle fails
+@pytest.mark.parametrize("k,l,disc", [(524287, 127, True), (524289, 128, False), (300001, 100, False),
+                                      (65535, 143, False), (499999, 111, True)])
+def test_vlow32_argument(k, l, disc):
+    dr, Nf = Fraction(1, 16), 1 << 20
+    arg32 = rne24(rne24(Fraction(-k * l * l)) * dr) / Nf
+    exact = Fraction(-k * l * l) * dr / Nf
+    got = angle_over_pi(Z.vlow32(k, l, np.float32(0.0625), Nf))
+    assert abs(got - wrap2(arg32)) < 2e-6
+    assert (abs(wrap2(arg32) - wrap2(exact)) > 4e-6) == disc
+
+
+@pytest.mark.parametrize("k,l", [(524287, 15), (400001, 13), (123457, 9)])
+def test_prephase32_argument(k, l):
+    r0, dr, Nf = Fraction(-4), Fraction(1, 16), 1 << 20
+    inner = rne24(rne24(Fraction(l)) * dr + 2 * r0)
+    arg32 = rne24(rne24(Fraction(k * l)) * inner) / Nf
+    got = angle_over_pi(Z.prephase32(k, l, np.float32(-4.0), np.float32(0.0625), Nf))
+    assert abs(got - wrap2(arg32)) < 2e-6
+    # k*l < 2^24 is exact in Float32; the rounding of the product with the
+    # inner factor is what the reference adds (a few 1e-7 of pi here)
+    exact = Fraction(k * l) * (l * dr + 2 * r0) / Nf
+    assert abs(wrap2(arg32) - wrap2(exact)) < 1e-5
+
+
+@pytest.mark.parametrize("k,l", [(524287, 127), (500001, 77)])
+def test_postphase32_argument(k, l):
+    dr, Nf = Fraction(1, 16), 1 << 20
+    arg32 = rne24(rne24(Fraction(k * l * l)) * dr) / Nf
+    got = angle_over_pi(Z.postphase32(k, l, np.float32(0.0625), Nf))
+    assert abs(got - wrap2(arg32)) < 2e-6
+    exact = Fraction(k * l * l) * dr / Nf
+    assert abs(wrap2(arg32) - wrap2(exact)) > 4e-6
+
+
+def test_ref32_distance_to_fp64_at_full_size():
+    """At 2^20 x 16 x 129 the reference's Float32 phase arguments put its
+    FP32 output 8e-6 (normwise) from the FP64 definition; FP32 FFTs with
+    exact phases land near 1e-7.  The bracket pins the restatement to the
+    reference's rounding: an exact-phase FP32 version fails it."""
+    s = synth.cfg2(seed=1234)
+    a = Z.zdt_ref32(s, -4.0, 0.0625, 129)
+    # two columns of the FP64 definition suffice for the scale of the error:
+    # the rounding grows with k*l^2, i.e. with the rate index
+    cols = [0, 128]
+    rates = [Fraction(-4) + Fraction(j, 16) for j in cols]
+    b = Z.fdr64(s, None, rates=rates)
+    err = float(np.max(np.abs(a[:, cols] - b)) / np.max(np.abs(b)))
+    assert 4e-6 <= err <= 1.2e-5, err
+
+
 # ------------------------------------------------------------ invariants
 
 def test_time_reversal_symmetry():
