@@ -139,6 +139,7 @@ def long_spectrogram(seed):
     return s, rate
 
 
+@pytest.mark.timeout(420)
 def test_long_integration_shape_vs_fp64(F):
     """2^22 x 64 with rates -8:1/64:8 (Nr = 1025): the search resident and
     with a budget that forces recomputed batches give the same thresholds and
