@@ -132,20 +132,54 @@ def copy_calibration() -> int:
     return 0
 
 
-def cpu_baseline(seed: int = 1234) -> dict:
-    """One batch of the FP32 CPU restatement of the reference, 1 thread."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+ACC_COLS = (0, 37, 64, 91, 128)  # rate columns whose FP64 oracle values the accuracy figures use
+
+
+def cpu_baseline(gpu_cols: dict, seed: int = 1234) -> tuple:
+    """One batch of the FP32 CPU restatement of the reference on one core
+    (scipy.fft workers=1: a default single-threaded Julia session runs FFTW
+    and the broadcasts on one thread) and, as an upper bound, with the FFTs on
+    every core of the host.  As the checker it also gives the accuracy of the
+    GPU FDRs (gpu_cols: precision -> the ACC_COLS columns, Nf x 5, and the
+    full FDR's column sums) against the FP64 oracle and against the
+    restatement's own FDR."""
+    import scipy.fft as sfft
     sys.path.insert(0, str(ROOT))
+    from oracle import host_oracle as H  # checker / CPU baseline only
     from oracle import zdt_oracle as Z  # checker / CPU baseline only
     s = np.asfortranarray(synthetic_spectrogram(NF, NT, seed).T)
     ws = Z.RefWorkspace32(s, R0, DR, NR)
-    t0 = time.perf_counter()
-    ws.input(s)
-    ws.zdtfdr()
-    dt = time.perf_counter() - t0
-    return {"value": NF * NR / dt, "unit": "FDR cells/s", "cores": 1, "kind": "port",
+    with sfft.set_workers(1):
+        t0 = time.perf_counter()
+        ws.input(s)
+        ref32 = ws.zdtfdr()
+        dt = time.perf_counter() - t0
+    ncpu = len(os.sched_getaffinity(0))
+    with sfft.set_workers(ncpu):
+        t0 = time.perf_counter()
+        ws.input(s)
+        ws.zdtfdr()
+        dt_all = time.perf_counter() - t0
+    base = {"value": NF * NR / dt, "unit": "FDR cells/s", "cores": 1, "kind": "port",
             "sample": f"one zdtfdr! batch (input!+preprocess!+convolve!+postprocess!+output!) at "
-                      f"2^20x16x129 by the numpy/scipy FP32 restatement, {dt:.2f} s"}
+                      f"2^20x16x129 by the numpy/scipy FP32 restatement, {dt:.2f} s",
+            "all_cores": {"value": NF * NR / dt_all, "cores": ncpu,
+                          "sample": f"the same batch with scipy.fft on {ncpu} threads (broadcasts single-threaded), "
+                                    f"{dt_all:.2f} s"}}
+    from fractions import Fraction
+    ref = Z.fdr64(s, None, rates=[Fraction(R0) + Fraction(DR) * j for j in ACC_COLS])
+    sig = H.fdrstats(ref)[1]
+    acc = {"basis": f"FP64 oracle (exact phases) on rate columns {list(ACC_COLS)}; normwise = max|d|/max|fdr64|, "
+                    f"cell = |d|/max(|fdr64|, s) with s = {sig:.3f} the columns' minimum std; vs_ref32 = normwise "
+                    "against the FP32 restatement of the reference (Float32 phase arguments) on the same columns"}
+    r32 = ref32[:, list(ACC_COLS)].astype(np.float64)
+    for name, cols in gpu_cols.items():
+        d = np.abs(cols.astype(np.float64) - ref)
+        cell = d / np.maximum(np.abs(ref), sig)
+        acc[name] = {"normwise": float(d.max() / np.abs(ref).max()), "cell_max": float(cell.max()),
+                     "cell_mean": float(cell.mean()),
+                     "vs_ref32": float(np.abs(cols - r32).max() / np.abs(r32).max())}
+    return base, acc
 
 
 def ragged(a) -> int:
@@ -255,6 +289,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--generic", action="store_true", help="force the unfused kernels")
+    ap.add_argument("--fp32-only", action="store_true", help="skip the split-precision modes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (to scale rocprof byte counters)")
     ap.add_argument("--ragged", action="store_true", help="the ragged multi-channel plan workload instead")
@@ -297,63 +332,78 @@ def main(argv=None) -> int:
     arr, n = _lib.int_array((2, 3, 5))
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
-    def create(flags):
-        h = C.c_void_p()
-        _lib.check(L.zdt_create(spec.data_ptr(), NF, NT, R0, DR, NR, arr, n, flags, 0, stream, C.byref(h)))
-        return h
-
-    hp, h = create(base), create(base | _lib.ZDT_FLAG_PROFILE)
     dest = F.create_fdr(spec, NR)
+    total = spec.double().sum()
 
-    def step(w):
-        _lib.check(L.zdt_input(w, spec.data_ptr()))
-        _lib.check(L.zdt_run(w, R0, _lib.ZDT_WINDOW_RECT, None, dest.data_ptr()))
+    def measure(precision: str) -> dict:
+        """Timed steps of one precision mode: per-kernel durations from a
+        workspace with the profiling hook (an event pair around every
+        launch serializes the queue, ~10 us each), then the timed steps on a
+        workspace without it.  Returns the timings, the kernel path the
+        library chose and the FDR columns the accuracy figures use."""
+        pc = _lib.PRECISIONS[precision]
 
-    # per-kernel durations for the roofline: the steps with an event pair
-    # around every kernel launch (the events serialize the queue, ~10 us
-    # each, so the timed steps below run without them)
-    for _ in range(a.warmup):
-        step(h)
-    torch.cuda.synchronize()
-    _lib.check(L.zdt_profile_reset(h))
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step(h)
-    torch.cuda.synchronize()
-    dt_prof = (time.perf_counter() - t0) / a.steps
-    # the timed steps
-    for _ in range(a.warmup):
-        step(hp)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step(hp)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / a.steps
+        def create(flags):
+            h = C.c_void_p()
+            _lib.check(L.zdt_create(spec.data_ptr(), NF, NT, R0, DR, NR, arr, n, flags, pc, stream, C.byref(h)))
+            return h
 
-    cnt = C.c_int()
-    _lib.check(L.zdt_profile_read(h, -1, C.byref(cnt), None, 0, None, None, None))
-    kern = []
-    for i in range(cnt.value):
-        name = C.create_string_buffer(64)
-        ms = C.c_double()
-        nl = C.c_int64()
-        by = C.c_double()
-        _lib.check(L.zdt_profile_read(h, i, None, name, 64, C.byref(ms), C.byref(nl), C.byref(by)))
-        kern.append((name.value.decode(), ms.value, nl.value, by.value))
-    kern.sort(key=lambda x: -x[1])
+        hp, h = create(base), create(base | _lib.ZDT_FLAG_PROFILE)
+
+        def step(w):
+            _lib.check(L.zdt_input(w, spec.data_ptr()))
+            _lib.check(L.zdt_run(w, R0, _lib.ZDT_WINDOW_RECT, None, dest.data_ptr()))
+
+        for _ in range(a.warmup):
+            step(h)
+        torch.cuda.synchronize()
+        _lib.check(L.zdt_profile_reset(h))
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            step(h)
+        torch.cuda.synchronize()
+        dt_prof = (time.perf_counter() - t0) / a.steps
+        for _ in range(a.warmup):
+            step(hp)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            step(hp)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        cnt = C.c_int()
+        _lib.check(L.zdt_profile_read(h, -1, C.byref(cnt), None, 0, None, None, None))
+        kern = []
+        for i in range(cnt.value):
+            name = C.create_string_buffer(64)
+            ms = C.c_double()
+            nl = C.c_int64()
+            by = C.c_double()
+            _lib.check(L.zdt_profile_read(h, i, None, name, 64, C.byref(ms), C.byref(nl), C.byref(by)))
+            kern.append((name.value.decode(), ms.value, nl.value, by.value))
+        kern.sort(key=lambda x: -x[1])
+        path = C.c_int()
+        _lib.check(L.zdt_path(hp, C.byref(path), None))
+        L.zdt_destroy(h)
+        L.zdt_destroy(hp)
+        # sanity: the result is the drift-line sum (column sums equal the total power)
+        colsum = dest.double().sum(dim=0)
+        ok = bool(torch.allclose(colsum, total.expand_as(colsum), rtol=1e-5))
+        cols = dest[:, list(ACC_COLS)].cpu().numpy()
+        return {"dt": dt, "dt_prof": dt_prof, "kern": kern, "path": _lib.PATHS[path.value], "ok": ok, "cols": cols}
+
+    runs = {"fp32": measure("fp32")}
+    if not a.generic and not a.fp32_only:
+        for prec in ("split-bf16", "split-fp16"):
+            runs[prec] = measure(prec)
+    if not all(r["ok"] for r in runs.values()):
+        print("bench.py: result check failed", file=sys.stderr)
+        return 1
+    r = runs["fp32"]
+    dt, dt_prof, kern = r["dt"], r["dt_prof"], r["kern"]
     dname, dms, dn, dbytes = kern[0]
     avg_s = dms / dn / 1e3
     achieved = dbytes / avg_s / 1e9
-    L.zdt_destroy(h)
-    L.zdt_destroy(hp)
-
-    # sanity: the result is the drift-line sum (column sums equal the total power)
-    colsum = dest.double().sum(dim=0)
-    total = spec.double().sum()
-    if not torch.allclose(colsum, total.expand_as(colsum), rtol=1e-5):
-        print("bench.py: result check failed", file=sys.stderr)
-        return 1
 
     # the VALU-bound kernel against the FP32 vector peak (algorithmic flops)
     compute = None
@@ -365,7 +415,7 @@ def main(argv=None) -> int:
                    "basis": "chirp-Z counted as complex multiply-adds (8 flops per term and rate), FFTs as "
                             "5 N log2 N; the FP32 vector peak, which non-packed VOP2 FMAs reach on gfx950"}
     out = {
-        "metric": "FDR cells/s (Nf x Nrates) at 2^20ch x 16t x 129 rates, FP32 ZDT",
+        "metric": "FDR cells/s (Nf\u00d7Nrates) at 2^20ch\u00d716t\u00d7129 rates, split-BF16 vs FP32 ZDT, %HBM",
         "value": NF * NR / dt,
         "unit": "FDR cells/s",
         "n_gpus": 1,
@@ -378,8 +428,7 @@ def main(argv=None) -> int:
         "dtype": "f32",
         "data": "synthetic (Exponential(1) noise + 16 drifting tones, seed 1234)",
         "config": {"workload": "zdtfdr! input+1 batch, Nf=2^20, Nt=16, rates -4:1/16:4 (Nr=129, Nl=144)",
-                   "Nf": NF, "Nt": NT, "Nr": NR, "r0": R0, "dr": DR,
-                   "path": "generic" if a.generic else "fused"},
+                   "Nf": NF, "Nt": NT, "Nr": NR, "r0": R0, "dr": DR, "path": r["path"], "precision": "fp32"},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dname),
                      "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": dbytes,
@@ -390,8 +439,16 @@ def main(argv=None) -> int:
         "kernels": [{"name": k, "ms_per_step": ms / a.steps, "launches_per_step": n_ / a.steps,
                      "alg_GBps": (by / (ms / n_ / 1e3) / 1e9) if ms > 0 else None} for k, ms, n_, by in kern],
     }
+    if len(runs) > 1:
+        # the split-precision modes on the same workload: the column FFTs of
+        # input! and output! (k1, k3) as BF16/FP16 matrix-core products
+        out["precision_modes"] = {
+            prec: {"ms_per_step": q["dt"] * 1e3, "value": NF * NR / q["dt"], "speedup_vs_fp32": dt / q["dt"],
+                   "path": q["path"],
+                   "kernels": [{"name": k, "ms_per_step": ms / a.steps} for k, ms, n_, by in q["kern"]]}
+            for prec, q in runs.items() if prec != "fp32"}
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"], out["accuracy"] = cpu_baseline({p: q["cols"] for p, q in runs.items()})
     print(json.dumps(out))
     return 0
 

@@ -409,9 +409,11 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
   ws->maxr = maxr;
   ws->precision = precision;
   const bool fused = !(flags & ZDT_FLAG_GENERIC) && fast_supported(Nf, Nt, Nr);
-  if (rc == ZDT_OK && precision != ZDT_PRECISION_FP32 && !(fused && fast_precision_ok(Nf, n2, precision)))
-    rc = fail(ZDT_ENOTSUP, "precision %d needs the fused path (power-of-two Nf in [2^16, 2^20], Nt <= 16) "
-              "with an MFMA variant for Nf=%lld", precision, (long long)Nf);
+  const bool fout_ok = !(flags & ZDT_FLAG_GENERIC) && !fused && fast_out_supported(Nf, Nt, Nr);
+  if (rc == ZDT_OK && precision != ZDT_PRECISION_FP32 &&
+      !((fused && fast_precision_ok(Nf, n2, precision)) || (fout_ok && fast_precision_ok(Nf, 0, precision))))
+    rc = fail(ZDT_ENOTSUP, "precision %d needs the fused path or the fused output with N2 = 1024 "
+              "(power-of-two Nf in [2^17, 2^22]); Nf=%lld Nt=%lld", precision, (long long)Nf, (long long)Nt);
   if (rc == ZDT_OK && fused) {
     rc = fast_create(Nf, Nt, Nr, dr, n2, precision, &ws->fast);
     if (rc == ZDT_OK) ws->bytes += fast_bytes(Nf, Nt, Nr);
@@ -420,7 +422,7 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
     if (fout) {
       // the natural-layout F is unpacked from Ft only for the step functions
       rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->Ft), static_cast<size_t>(ws->Nk * Nt) * sizeof(c32));
-      if (rc == ZDT_OK) rc = fast_out_create(Nf, Nr, &ws->fout);
+      if (rc == ZDT_OK) rc = fast_out_create(Nf, Nr, precision, &ws->fout);
       if (rc == ZDT_OK) ws->bytes += fast_out_bytes(Nf, Nr);
     } else {
       rc = dev_alloc(ws, reinterpret_cast<void**>(&ws->F), static_cast<size_t>(ws->Nk * Nt) * sizeof(c32));

@@ -40,7 +40,8 @@ constexpr int64_t FOUT_MAX_NT = 1024;
 // (bin k2 + N2 k1) followed by the Nt Nyquist bins.
 bool fast_out_supported(int64_t Nf, int64_t Nt, int64_t Nr);
 int64_t fast_out_bytes(int64_t Nf, int64_t Nr);
-int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out);
+// precision: as fast_create (the split modes change k1 and k3)
+int fast_out_create(int64_t Nf, int64_t Nr, int precision, FastPlan** out);
 // input!: spectrogram (device Nf x Nt) -> Ft (Nk x Nt entries, transposed layout)
 int fast_out_input(FastPlan* p, const float* spec, int64_t Nt, c32* Ft, hipStream_t st);
 // natural-layout F from Ft (for the step functions)

@@ -44,16 +44,6 @@ namespace zdt {
 
 // ---------------------------------------------------------------- k1: input pass A
 
-struct K1Args {
-  const float* spec;  // Nf x Nt, column stride ldspec floats
-  int64_t ldspec;
-  c32* G;             // [Nt][N2][N1]
-  const c32* tw2;     // W_N2^m
-  int logN1;
-  int64_t M;
-  int ntiles;         // Nt * N1/16
-};
-
 template <int R1>
 __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k1_input(K1Args a) {
   using C = CF<R1>;
@@ -99,47 +89,6 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k1_inp
 }
 
 // ---------------------------------------------------------------- k3: output pass 2
-
-struct K3Args {
-  const c32* B;  // [Nr][N1/16][N2][16]
-  const c32* yhalf;  // [Nr] bin M/2 term: B(j, k2 = 0, n1) += yhalf[j] (-1)^{n1}
-  float* dest;   // Nf x Nr, column stride ld floats
-  int64_t ld;
-  const c32* tw2;
-  int logN1;
-  int Nr;
-  int ntiles;    // Nr * N1/16
-  double* part;  // NULL, or per-wavefront FDR statistics [Nr][N1/16][waves][3] (see k3_stats)
-  int paired;    // k2 wrote the rates in mirrored pairs from the middle outwards
-};
-
-// k3's i-th rate: the reverse of k2's write order, so the spectra written
-// last are read first (Nr-1, Nr-2, ... or, for paired rates, Nr-1, 0, Nr-2, 1, ...)
-__device__ __forceinline__ int k3_rate(int i, int Nr, int paired) {
-  if (!paired) return Nr - 1 - i;
-  return (i & 1) ? (i >> 1) : Nr - 1 - (i >> 1);
-}
-
-// Each wavefront of a k3 tile writes (S1, S2, c): the FP64 sums of (x - c) and
-// (x - c)^2 over its 64 x 64 FDR values, c = lane 0's first value.  Each lane
-// sums with its own first value as shift and is re-centred to c before the
-// (fixed-order) wavefront reduction, so no sum cancels.
-__device__ __forceinline__ void k3_stats(double* dst, double s1, double s2, double ct) {
-  constexpr double n = 64.0;  // values per lane
-  const double c = __shfl(ct, 0, 64);
-  const double d = ct - c;
-  double S1 = s1 + n * d;
-  double S2 = s2 + d * (2.0 * s1 + n * d);
-  for (int off = 32; off > 0; off >>= 1) {
-    S1 += __shfl_xor(S1, off, 64);
-    S2 += __shfl_xor(S2, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    dst[0] = S1;
-    dst[1] = S2;
-    dst[2] = c;
-  }
-}
 
 template <int R1>
 __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_output(K3Args a) {
@@ -697,9 +646,10 @@ bool fast_n2_ok(int64_t Nf, int n2) {
 }
 
 bool fast_precision_ok(int64_t Nf, int n2, int precision) {
-  (void)Nf;
-  (void)n2;
-  return precision == ZDT_PRECISION_FP32;
+  if (precision == ZDT_PRECISION_FP32) return true;
+  if (precision != ZDT_PRECISION_SPLIT_BF16 && precision != ZDT_PRECISION_SPLIT_FP16) return false;
+  // the matrix-core column FFTs are the 32 x 32 split of N2 = 1024
+  return (n2 ? n2 : fast_n2(Nf)) == 1024;
 }
 
 int fast_create(int64_t Nf, int64_t Nt, int64_t Nr, float dr, int n2, int precision, FastPlan** out) {
@@ -786,6 +736,7 @@ int launch_k1(const FastPlan* p, const float* spec, c32* G, int64_t Nt, hipStrea
   a.M = p->M;
   a.ntiles = static_cast<int>(Nt * (p->N1 / 16));
   ProfScope ps(st, "k1_input", 4.0 * p->Nf * Nt + 8.0 * p->M * Nt);
+  if (p->precision != ZDT_PRECISION_FP32) return launch_k1_mfma(p, a, st);
   if (p->N2 == 1024) {
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
     k1_input<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
@@ -953,7 +904,9 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* 
   a.paired = p->paired ? 1 : 0;
   {
     ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
-    if (p->N2 == 1024) {
+    if (p->precision != ZDT_PRECISION_FP32) {
+      ZDT_TRY(launch_k3_mfma(p, a, st));
+    } else if (p->N2 == 1024) {
       const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
       k3_output<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
     } else {

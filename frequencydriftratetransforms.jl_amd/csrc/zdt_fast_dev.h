@@ -230,6 +230,59 @@ __device__ __forceinline__ void cf_load_twiddles(c32* twl, const c32* __restrict
   for (int i = threadIdx.x; i < N; i += nthreads) twl[i] = tw[i];
 }
 
+// ---------------------------------------------------------------- k1/k3 arguments
+
+struct K1Args {
+  const float* spec;  // Nf x Nt, column stride ldspec floats
+  int64_t ldspec;
+  c32* G;             // [Nt][N2][N1]
+  const c32* tw2;     // W_N2^m
+  int logN1;
+  int64_t M;
+  int ntiles;         // Nt * N1/16
+};
+
+struct K3Args {
+  const c32* B;  // [Nr][N1/16][N2][16]
+  const c32* yhalf;  // [Nr] bin M/2 term: B(j, k2 = 0, n1) += yhalf[j] (-1)^{n1}
+  float* dest;   // Nf x Nr, column stride ld floats
+  int64_t ld;
+  const c32* tw2;
+  int logN1;
+  int Nr;
+  int ntiles;    // Nr * N1/16
+  double* part;  // NULL, or per-wavefront FDR statistics [Nr][N1/16][waves][3] (see k3_stats)
+  int paired;    // k2 wrote the rates in mirrored pairs from the middle outwards
+};
+
+// k3's i-th rate: the reverse of k2's write order, so the spectra written
+// last are read first (Nr-1, Nr-2, ... or, for paired rates, Nr-1, 0, Nr-2, 1, ...)
+__device__ __forceinline__ int k3_rate(int i, int Nr, int paired) {
+  if (!paired) return Nr - 1 - i;
+  return (i & 1) ? (i >> 1) : Nr - 1 - (i >> 1);
+}
+
+// Each wavefront of a k3 tile writes (S1, S2, c): the FP64 sums of (x - c) and
+// (x - c)^2 over its 64 x 64 FDR values, c = lane 0's first value.  Each lane
+// sums with its own first value as shift and is re-centred to c before the
+// (fixed-order) wavefront reduction, so no sum cancels.
+__device__ __forceinline__ void k3_stats(double* dst, double s1, double s2, double ct) {
+  constexpr double n = 64.0;  // values per lane
+  const double c = __shfl(ct, 0, 64);
+  const double d = ct - c;
+  double S1 = s1 + n * d;
+  double S2 = s2 + d * (2.0 * s1 + n * d);
+  for (int off = 32; off > 0; off >>= 1) {
+    S1 += __shfl_xor(S1, off, 64);
+    S2 += __shfl_xor(S2, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    dst[0] = S1;
+    dst[1] = S2;
+    dst[2] = c;
+  }
+}
+
 __device__ __forceinline__ float fast_window(int wkind, const float* wcustom, int64_t k, int64_t Nf) {
   if (wkind == ZDT_WINDOW_HAMMING) {
     const float p = static_cast<float>(static_cast<double>(k) / static_cast<double>(Nf));

@@ -45,4 +45,12 @@ void alloc_part(FastPlan* p, int& rc);
 // k1_input over Nt time steps of spec into G
 int launch_k1(const FastPlan* p, const float* spec, c32* G, int64_t Nt, hipStream_t st);
 
+// split-precision (matrix-core) k1/k3 for p->precision != ZDT_PRECISION_FP32
+// (zdt_mfma.hip); they need N2 = 1024
+struct K1Args;
+struct K3Args;
+bool mfma_supported(const FastPlan* p);
+int launch_k1_mfma(const FastPlan* p, const K1Args& a, hipStream_t st);
+int launch_k3_mfma(const FastPlan* p, const K3Args& a, hipStream_t st);
+
 }  // namespace zdt

@@ -637,10 +637,13 @@ int64_t fast_out_bytes(int64_t Nf, int64_t Nr) {
   return 16 * M * Nr + 8 * (M / N2 + N2 + 2 * Nr) + 8 * fast_stats_doubles(Nf, Nr);
 }
 
-int fast_out_create(int64_t Nf, int64_t Nr, FastPlan** out) {
+int fast_out_create(int64_t Nf, int64_t Nr, int precision, FastPlan** out) {
   *out = nullptr;
   if (!fast_out_supported(Nf, 1, Nr)) return fail(ZDT_ENOTSUP, "fused output: unsupported shape");
+  if (!fast_precision_ok(Nf, 0, precision))
+    return fail(ZDT_ENOTSUP, "fused output: precision %d not available for Nf=%lld", precision, (long long)Nf);
   FastPlan* p = new FastPlan();
+  p->precision = precision;
   p->Nf = Nf;
   p->Nt = 0;
   p->Nr = Nr;
