@@ -182,20 +182,16 @@ def cpu_baseline(gpu_cols: dict, seed: int = 1234) -> tuple:
     return base, acc
 
 
-def ragged(a) -> int:
-    """Many coarse channels in one plan launch: 64 channels, Nf drawn from
-    {2^18, 2^19, 2^20}, Nt from {16, 32, 60}, Exponential(1) noise + 8 tones
-    each, rates -4:1/Nt:4, proto-hits at SNR 10 compacted on the device.
-    Prints one JSON line (aggregate FDR cells/s over all channels)."""
+def ragged_channels(nch: int, seed: int, dev):
+    """The ragged workload's channels, generated on the device: Nf drawn from
+    {2^18, 2^19, 2^20}, Nt from {16, 32, 60}, Exponential(1) noise plus 8
+    tones (start channel uniform, rate U(-4, 4), amplitude U(10, 50)) each,
+    rates -4:1/Nt:4.  Returns the shapes, the (Nf, Nt) column-major device
+    spectrograms and the (r0, dr, Nr) rate ranges."""
     import torch
-
-    import fdrt_hip as F
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    rng = np.random.default_rng(4321)
-    nch = a.channels
+    rng = np.random.default_rng(seed)
     shapes = [(int(rng.choice([1 << 18, 1 << 19, 1 << 20])), int(rng.choice([16, 32, 60]))) for _ in range(nch)]
-    gen = torch.Generator(device=dev).manual_seed(4321)
+    gen = torch.Generator(device=dev).manual_seed(seed)
     specs, rates = [], []
     for Nf, Nt in shapes:
         sp = torch.empty((Nt, Nf), device=dev).exponential_(1.0, generator=gen)  # row t = time step
@@ -209,6 +205,21 @@ def ragged(a) -> int:
             np.repeat(amp, Nt).astype(np.float32)).to(dev)
         specs.append(sp.t())
         rates.append((-4.0, 1.0 / Nt, 8 * Nt + 1))
+    return shapes, specs, rates
+
+
+def ragged(a) -> int:
+    """Many coarse channels in one plan launch: 64 channels, Nf drawn from
+    {2^18, 2^19, 2^20}, Nt from {16, 32, 60}, Exponential(1) noise + 8 tones
+    each, rates -4:1/Nt:4, proto-hits at SNR 10 compacted on the device.
+    Prints one JSON line (aggregate FDR cells/s over all channels)."""
+    import torch
+
+    import fdrt_hip as F
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    nch = a.channels
+    shapes, specs, rates = ragged_channels(nch, 4321, dev)
     t0 = time.perf_counter()
     plan = F.RaggedPlan(specs, rates, autotune=not a.no_autotune)
     plan.bind(10.0, snr=True, cap=1 << 16, keep_fdrs=False)

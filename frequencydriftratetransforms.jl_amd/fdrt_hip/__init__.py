@@ -3,7 +3,8 @@ over libzdt_hip.so (MI355X / gfx950).
 
 Names follow the reference's exports (src/FrequencyDriftRateTransforms.jl:3-24)
 restricted to the ZDT path; a trailing underscore stands for Julia's ``!``.
-The same C-ABI is bound from Julia by ``julia/ZDTHip.jl``.
+The same C-ABI is bound from Julia by the ``julia/ZDTHip`` package and its
+extension of FrequencyDriftRateTransforms.
 """
 import torch  # noqa: F401  (loads the HIP runtime libzdt_hip.so binds to)
 

@@ -1,8 +1,8 @@
 """ZDTWorkspace and the ZDT step functions over libzdt_hip.so.
 
-Python mirror of src/zdtfdr.jl (and of the Julia host module
-julia/ZDTHip.jl): same names with ``!`` spelled as a trailing underscore,
-same argument meaning, same errors.  Device arrays are torch tensors of the
+Python mirror of src/zdtfdr.jl (and of the Julia binding julia/ZDTHip, a
+package extension of FrequencyDriftRateTransforms): same names with ``!``
+spelled as a trailing underscore, same argument meaning, same errors.  Device arrays are torch tensors of the
 Julia shape with column-major strides (see _device.py).
 """
 from __future__ import annotations

@@ -1,31 +1,25 @@
 """
     ZDTHip
 
-Julia host module for libzdt_hip.so, the MI355X (gfx950) implementation of the
-Chirp-Z De-Doppler Transform path of FrequencyDriftRateTransforms.jl.  It keeps
-the package's call signatures (`ZDTWorkspace`, `input!`, `preprocess!`,
-`convolve!`, `postprocess!`, `output!`, `zdtfdr!`, `zdtfdr`, `findprotohits`,
-`fdrstats`, `fdrnormalize!`, `fdrdenormalize`, `fdrsynchronize`, `calcNl`,
-`growNr`, `estimate_memory`, `driftrates`, `batchrates`) and reaches the
-device through `ccall` only: device memory is a small `HipMatrix` handle
-(pointer + size) managed through the HIP runtime's C API, so no GPU package is
-needed.  Every library call returns a status; a non-zero status throws
-`ErrorException(zdt_last_error())`.
+Device side of FrequencyDriftRateTransforms.jl's Chirp-Z De-Doppler Transform
+path on MI355X (gfx950): the `HipMatrix` device array and thin `ccall`
+wrappers over the C-ABI of libzdt_hip.so (include/zdt.h).  It defines no
+function of the package; `using FrequencyDriftRateTransforms, ZDTHip` loads
+the extension `FrequencyDriftRateTransformsZDTHipExt`, which adds the
+package's own methods (`ZDTWorkspace`, `input!`, `zdtfdr!`, `findprotohits`,
+...) for `HipMatrix` spectrograms, as the package's CUDA extension does for
+`CuArray` (ext/CUDAFrequencyDriftRateTransformsExt.jl:1-28).
 
-Reference interfaces replaced: src/zdtfdr.jl:10-421, src/zdtutils.jl:7-64,
-src/batchrates.jl:22-60, src/fdrutils.jl:11-169,
-ext/CUDAFrequencyDriftRateTransformsExt.jl:26-103.
+The objects with no counterpart in the package (`RaggedPlan`, `StreamSearch`)
+live here.  Every library call returns a status; a non-zero status throws
+`ErrorException(zdt_last_error())`.
 """
 module ZDTHip
 
-export HipMatrix, ZDTWorkspace, input!, output!, preprocess!, convolve!, postprocess!,
-       zdtfdr, zdtfdr!, calcNl, growNr, estimate_memory, driftrates, batchrates,
-       create_fdr, fdrstats, fdrsynchronize, fdrnormalize!, fdrnormalize, fdrdenormalize,
-       findprotohits, RaggedPlan, bind!, launch!, planhits,
-       fftfdr_workspace, fftfdr_workspace!, fftfdr, fftfdr!, fdshift, fdshift!, fdshiftsum, fdshiftsum!,
-       StreamSearch, search
+export HipMatrix, HipZDTWorkspace, zdtpath, zdtprecision,
+       RaggedPlan, bind!, launch!, planhits, StreamSearch, search
 
-const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "lib", "libzdt_hip.so"))
+const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "libzdt_hip.so"))
 const LIBHIP = "libamdhip64.so"
 
 const ZDT_OK = 0
@@ -34,6 +28,9 @@ const WINDOW_RECT = 0
 const WINDOW_HAMMING = 1
 const WINDOW_CUSTOM = 2
 const FLAG_UNNORMALIZED = 0x1
+const PLAN_AUTOTUNE = 0x100
+const PRECISIONS = Dict(:fp32 => 0, :split_bf16 => 1, :split_fp16 => 2)
+const PATHS = Dict(0 => :generic, 1 => :fused, 2 => :fused_output)
 
 zdt_error() = unsafe_string(ccall((:zdt_last_error, LIBZDT), Cstring, ()))
 check(rc::Integer) = rc == ZDT_OK ? nothing : error(zdt_error())
@@ -41,10 +38,12 @@ check(rc::Integer) = rc == ZDT_OK ? nothing : error(zdt_error())
 # ----------------------------------------------------------------- device memory
 
 """
-    HipMatrix{T}(undef, m, n) / HipMatrix(A::Matrix)
+    HipMatrix{T}(undef, m, n) / HipMatrix(A::AbstractMatrix)
 
-Column-major device matrix: the stand-in for a `CuMatrix`.  `Array(H)` copies
-back to the host.  Memory is released by a finalizer (`hipFree`).
+Column-major device matrix, the counterpart of a `CuMatrix` for this path.
+`Array(H)` copies back to the host; `similar` allocates another one (which
+makes the package's `create_fdr`, src/fdrutils.jl:11-26, work unchanged).
+Memory is released by a finalizer (`hipFree`).
 """
 mutable struct HipMatrix{T} <: AbstractMatrix{T}
     ptr::Ptr{T}
@@ -74,6 +73,7 @@ end
 Base.size(H::HipMatrix) = H.dims
 Base.getindex(::HipMatrix, ::Int...) = error("HipMatrix elements live on the device; use Array(H)")
 Base.pointer(H::HipMatrix) = H.ptr
+Base.similar(::HipMatrix, ::Type{T}, dims::Dims{2}) where {T} = HipMatrix{T}(undef, dims...)
 
 function Base.Array(H::HipMatrix{T}) where {T}
     A = Matrix{T}(undef, H.dims...)
@@ -83,55 +83,21 @@ function Base.Array(H::HipMatrix{T}) where {T}
     return A
 end
 
-# ----------------------------------------------------------------- sizing (src/zdtutils.jl)
+device_synchronize() = ccall((:hipDeviceSynchronize, LIBHIP), Cint, ())
 
-_factors(f) = Cint[Int(x) for x in f]
-
-"calcNl(Nt, Nr, factors=(2,3,5)): nextprod(factors, Nt+Nr-1) -- src/zdtutils.jl:7-9"
-function calcNl(Nt, Nr, factors=(2, 3, 5))
-    f = _factors(factors)
-    v = ccall((:zdt_calc_nl, LIBZDT), Int64, (Int64, Int64, Ptr{Cint}, Cint), Nt, Nr, f, length(f))
-    v < 0 && error("calcNl: invalid arguments")
-    return Int(v)
-end
-
-"growNr(Nt, Nr, factors=(2,3,5)) -- src/zdtutils.jl:17-19"
-growNr(Nt, Nr, factors=(2, 3, 5)) = calcNl(Nt, Nr, factors) - Nt + 1
-
-"estimate_memory(Nf, Nt, Nr, Ni=1, No=1; factors) -- src/zdtutils.jl:35-53"
-function estimate_memory(Nf, Nt, Nr, Ni=1, No=1; factors=(2, 3, 5))
-    f = _factors(factors)
-    ccall((:zdt_estimate_memory, LIBZDT), Int64,
-          (Int64, Int64, Int64, Int64, Int64, Ptr{Cint}, Cint), Nf, Nt, Nr, Ni, No, f, length(f))
-end
-
-"batchrates(Nt, δhzps, r1, r2=-r1; Nrb=360) -- src/batchrates.jl:22-60"
-function batchrates(Nt::Integer, δhzps, r1, r2=-r1; Nrb=360)
-    if r2 < r1
-        r1, r2 = r2, r1
-    end
-    nc1 = round(Int, r1 / δhzps)
-    nc2 = round(Int, r2 / δhzps)
-    Nrspan = abs(nc2 - nc1) + 1
-    Nrb = growNr(Nt, min(Nrb, Nrspan))
-    Nr = cld(Nrspan, Nrb) * Nrb
-    Nrbonus = Nr - Nrspan
-    nc1 -= round(Int, sign(δhzps) * Nrbonus / 2, RoundFromZero)
-    nc2 += round(Int, sign(δhzps) * Nrbonus / 2, RoundToZero)
-    rs = range(nc1 // (Nt - 1), stop=nc2 // (Nt - 1), length=Nr)
-    @assert length(rs) % Nrb == 0
-    collect(Iterators.partition(rs, Nrb))
-end
-
-# ----------------------------------------------------------------- workspace
+# ----------------------------------------------------------------- workspace handle
 
 """
-    ZDTWorkspace(spectrogram::HipMatrix{Float32}, rates[, factors]; unnormalized=false)
-    ZDTWorkspace(spectrogram::HipMatrix{Float32}, r0, δr, Nr[, factors]; unnormalized=false)
+    HipZDTWorkspace(spectrogram::HipMatrix{Float32}, r0, δr, Nr[, factors];
+                    unnormalized=false, precision=:fp32, stream=C_NULL)
 
-src/zdtfdr.jl:37-87.  Allocates the device buffers and runs `input!`.
+The library's workspace (zdt_create): device buffers, chirp kernel and, from
+`spectrogram`, the input spectrum.  `precision` is `:fp32`, `:split_bf16` or
+`:split_fp16` (the split modes run the column DFTs of input!/output! as
+matrix-core products; power-of-two Nf in [2^17, 2^22]).  The package's
+`ZDTWorkspace(spectrogram::HipMatrix{Float32}, ...)` returns one of these.
 """
-mutable struct ZDTWorkspace
+mutable struct HipZDTWorkspace
     handle::Ptr{Cvoid}
     Nf::Int
     Nt::Int
@@ -139,56 +105,62 @@ mutable struct ZDTWorkspace
     δr::Float32
     Nr::Int
     Nl::Int
-    factors::Tuple
+    factors::Union{Tuple,AbstractVector}
     stream::Ptr{Cvoid}
     window::Union{Nothing,HipMatrix{Float32}}
 end
 
-function ZDTWorkspace(spec::HipMatrix{Float32}, r0::Real, δr::Real, Nr::Integer,
-                      factors=(2, 3, 5); unnormalized::Bool=false, stream::Ptr{Cvoid}=C_NULL)
+_factors(f) = Cint[Int(x) for x in f]
+
+function HipZDTWorkspace(spec::HipMatrix{Float32}, r0::Real, δr::Real, Nr::Integer,
+                         factors::Union{Tuple,AbstractVector}=(2, 3, 5); unnormalized::Bool=false,
+                         precision::Symbol=:fp32, stream::Ptr{Cvoid}=C_NULL)
     Nf, Nt = size(spec)
     f = _factors(factors)
+    haskey(PRECISIONS, precision) || error("precision must be one of $(sort(collect(keys(PRECISIONS))))")
     h = Ref{Ptr{Cvoid}}(C_NULL)
     flags = unnormalized ? FLAG_UNNORMALIZED : 0
     check(ccall((:zdt_create, LIBZDT), Cint,
                 (Ptr{Float32}, Int64, Int64, Cfloat, Cfloat, Int64, Ptr{Cint}, Cint, Cint, Cint, Ptr{Cvoid},
                  Ref{Ptr{Cvoid}}),
-                spec.ptr, Nf, Nt, Float32(r0), Float32(δr), Nr, f, length(f), flags, Cint(0), stream, h))
+                spec.ptr, Nf, Nt, Float32(r0), Float32(δr), Nr, f, length(f), flags, PRECISIONS[precision],
+                stream, h))
     nl = Ref{Int64}(0)
     check(ccall((:zdt_info, LIBZDT), Cint,
                 (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Cfloat}, Ptr{Cfloat}, Ptr{Int64}, Ref{Int64}),
                 h[], C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, nl))
-    ws = ZDTWorkspace(h[], Nf, Nt, Float32(r0), Float32(δr), Nr, Int(nl[]), Tuple(factors), stream, nothing)
+    ws = HipZDTWorkspace(h[], Nf, Nt, Float32(r0), Float32(δr), Nr, Int(nl[]), factors, stream, nothing)
     finalizer(ws) do w
         ccall((:zdt_destroy, LIBZDT), Cint, (Ptr{Cvoid},), w.handle)
     end
     return ws
 end
 
-function ZDTWorkspace(spec::HipMatrix{Float32}, rates::AbstractRange, factors=(2, 3, 5); kw...)
-    ZDTWorkspace(spec, first(rates), step(rates), length(rates), factors; kw...)
+function _path(ws::HipZDTWorkspace)
+    p, q = Ref{Cint}(0), Ref{Cint}(0)
+    check(ccall((:zdt_path, LIBZDT), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}), ws.handle, p, q))
+    p[], q[]
 end
 
-function Base.show(io::IO, ws::ZDTWorkspace)
-    print(io, "ZDTWorkspace(Nf=", ws.Nf, ",Nt=", ws.Nt, ",r0=", ws.r0, ",δr=", ws.δr,
+"zdtpath(ws): the kernel path serving the workspace (:fused, :fused_output or :generic)."
+zdtpath(ws::HipZDTWorkspace) = PATHS[_path(ws)[1]]
+
+"zdtprecision(ws): the arithmetic of its column DFTs (:fp32, :split_bf16 or :split_fp16)."
+zdtprecision(ws::HipZDTWorkspace) = Dict(v => k for (k, v) in PRECISIONS)[_path(ws)[2]]
+
+function Base.show(io::IO, ws::HipZDTWorkspace)
+    print(io, "HipZDTWorkspace(Nf=", ws.Nf, ",Nt=", ws.Nt, ",r0=", ws.r0, ",δr=", ws.δr,
           ",Nr=", ws.Nr, ",Nl=", ws.Nl, ",", ws.factors, ")")
 end
 
-function Base.sizeof(ws::ZDTWorkspace)
+function Base.sizeof(ws::HipZDTWorkspace)
     b = Ref{Int64}(0)
     check(ccall((:zdt_sizeof, LIBZDT), Cint, (Ptr{Cvoid}, Ref{Int64}), ws.handle, b))
     Int(b[])
 end
 
-"driftrates(ws, r0=ws.r0) -- src/zdtutils.jl:62-64"
-driftrates(ws::ZDTWorkspace, r0=ws.r0) = range(r0, step=ws.δr, length=ws.Nr)
-
-"fdrsynchronize -- ext/CUDAFrequencyDriftRateTransformsExt.jl:26-28"
-fdrsynchronize(ws::ZDTWorkspace) = check(ccall((:zdt_sync, LIBZDT), Cint, (Ptr{Cvoid},), ws.handle))
-fdrsynchronize(::Type{<:HipMatrix}) = ccall((:hipDeviceSynchronize, LIBHIP), Cint, ())
-
-# window -> (kind, device pointer); src/zdtfdr.jl:297-321
-function _window(ws::ZDTWorkspace, w)
+# window -> (kind, device pointer); the window forms of src/zdtfdr.jl:297-321
+function _window(ws::HipZDTWorkspace, w)
     w isa Val && (w = typeof(w).parameters[1])
     if w === :rect
         return WINDOW_RECT, Ptr{Float32}(C_NULL)
@@ -197,6 +169,8 @@ function _window(ws::ZDTWorkspace, w)
     elseif w isa Symbol
         error("unsupported window type ($w)")
     elseif w isa Function
+        # evaluated on the host in Float64 (as Julia promotes w's result with
+        # the ComplexF32 phase), stored as Float32
         v = Float32[w(n, ws.Nf) for n in 0:ws.Nf÷2]
         ws.window = HipMatrix(reshape(v, :, 1))
         return WINDOW_CUSTOM, ws.window.ptr
@@ -204,82 +178,59 @@ function _window(ws::ZDTWorkspace, w)
     error("unsupported window argument $(typeof(w))")
 end
 
-"input!(workspace, spectrogram) -- src/zdtfdr.jl:209-212"
-function input!(ws::ZDTWorkspace, spec::HipMatrix{Float32})
+function _input!(ws::HipZDTWorkspace, spec::HipMatrix{Float32})
     size(spec) == (ws.Nf, ws.Nt) || error("input spectrogram has unexpected size")
     check(ccall((:zdt_input, LIBZDT), Cint, (Ptr{Cvoid}, Ptr{Float32}), ws.handle, spec.ptr))
+    ws
 end
 
-"preprocess!(workspace, r0=workspace.r0) -- src/zdtfdr.jl:221-239"
-preprocess!(ws::ZDTWorkspace, r0::Real=ws.r0) =
-    check(ccall((:zdt_preprocess, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat), ws.handle, Float32(r0)))
+_preprocess!(ws::HipZDTWorkspace, r0::Float32) =
+    check(ccall((:zdt_preprocess, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat), ws.handle, r0))
 
-"convolve!(workspace) -- src/zdtfdr.jl:249-253"
-convolve!(ws::ZDTWorkspace) = check(ccall((:zdt_convolve, LIBZDT), Cint, (Ptr{Cvoid},), ws.handle))
+_convolve!(ws::HipZDTWorkspace) = check(ccall((:zdt_convolve, LIBZDT), Cint, (Ptr{Cvoid},), ws.handle))
 
-"postprocess!([w,] workspace) -- src/zdtfdr.jl:297-321"
-function postprocess!(w, ws::ZDTWorkspace)
+function _postprocess!(w, ws::HipZDTWorkspace)
     kind, ptr = _window(ws, w)
     check(ccall((:zdt_postprocess, LIBZDT), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}), ws.handle, kind, ptr))
 end
-postprocess!(ws::ZDTWorkspace) = postprocess!(:rect, ws)
 
-"output!(dest, workspace) -- src/zdtfdr.jl:328-331"
-function output!(dest::HipMatrix{Float32}, ws::ZDTWorkspace)
+function _output!(dest::HipMatrix{Float32}, ws::HipZDTWorkspace)
     size(dest) == (ws.Nf, ws.Nr) || error("dest has unexpected size")
     check(ccall((:zdt_output, LIBZDT), Cint, (Ptr{Cvoid}, Ptr{Float32}), ws.handle, dest.ptr))
-    return dest
+    dest
 end
 
-"zdtfdr!([w,] dests, workspace[, spectrogram]; r0) -- src/zdtfdr.jl:349-362"
-function zdtfdr!(w::Union{Function,Symbol,Val}, dests, ws::ZDTWorkspace, spectrogram=nothing; r0=ws.r0)
-    spectrogram === nothing || input!(ws, spectrogram)
-    kind, ptr = _window(ws, w)
-    for (dest, rate) in zip(dests, Iterators.cycle(r0))
-        size(dest) == (ws.Nf, ws.Nr) || error("dest has unexpected size")
-        check(ccall((:zdt_run, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat, Cint, Ptr{Float32}, Ptr{Float32}),
-                    ws.handle, Float32(rate), kind, ptr, dest.ptr))
-    end
-    return dests
-end
-zdtfdr!(dests, ws::ZDTWorkspace, spectrogram=nothing; r0=ws.r0) = zdtfdr!(:rect, dests, ws, spectrogram; r0)
-
-function zdtfdr!(w::Union{Function,Symbol,Val}, dest::HipMatrix{Float32}, ws::ZDTWorkspace, spectrogram=nothing;
-                 r0::Real=ws.r0)
-    zdtfdr!(w, (dest,), ws, spectrogram; r0)
-    return dest
-end
-zdtfdr!(dest::HipMatrix{Float32}, ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0) =
-    zdtfdr!(:rect, dest, ws, spectrogram; r0)
-
-# no dest: the result stays in the workspace for output! (src/zdtfdr.jl:382-396)
-function zdtfdr!(w::Union{Function,Symbol,Val}, ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0)
-    spectrogram === nothing || input!(ws, spectrogram)
+# one fused batch: zdt_run (dest may be C_NULL: the spectra stay for output!)
+function _run!(ws::HipZDTWorkspace, w, r0::Real, dest::Ptr{Float32})
     kind, ptr = _window(ws, w)
     check(ccall((:zdt_run, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat, Cint, Ptr{Float32}, Ptr{Float32}),
-                ws.handle, Float32(r0), kind, ptr, C_NULL))
-    return nothing
+                ws.handle, Float32(r0), kind, ptr, dest))
 end
-zdtfdr!(ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0) = zdtfdr!(:rect, ws, spectrogram; r0)
 
-"zdtfdr([w,] workspace[, spectrogram]; r0) -- src/zdtfdr.jl:412-421"
-function zdtfdr(w::Union{Function,Symbol,Val}, ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0)
-    dest = HipMatrix{Float32}(undef, ws.Nf, ws.Nr)
-    zdtfdr!(w, dest, ws, spectrogram; r0)
+_sync(ws::HipZDTWorkspace) = check(ccall((:zdt_sync, LIBZDT), Cint, (Ptr{Cvoid},), ws.handle))
+
+function _fftfdr!(fdr::HipMatrix{Float32}, ws::HipZDTWorkspace, rates)
+    r = Float32.(collect(rates))
+    size(fdr) == (ws.Nf, length(r)) || error("fdr must be $(ws.Nf) x $(length(r))")
+    check(ccall((:zdt_fftfdr, LIBZDT), Cint, (Ptr{Cvoid}, Ptr{Cfloat}, Int64, Ptr{Float32}),
+                ws.handle, r, length(r), fdr.ptr))
+    fdr
 end
-zdtfdr(ws::ZDTWorkspace, spectrogram=nothing; r0::Real=ws.r0) = zdtfdr(:rect, ws, spectrogram; r0)
 
-# ----------------------------------------------------------------- statistics (src/fdrutils.jl)
+function _fdshift!(dest::HipMatrix{Float32}, ws::HipZDTWorkspace, rate)
+    size(dest) == (ws.Nf, ws.Nt) || error("dest must be $(ws.Nf) x $(ws.Nt)")
+    check(ccall((:zdt_fdshift, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat, Ptr{Float32}), ws.handle, Float32(rate),
+                dest.ptr))
+    dest
+end
 
-"create_fdr(spectrogram, Nr_or_rates) -- src/fdrutils.jl:11-26"
-create_fdr(spec::HipMatrix{Float32}, Nr::Integer) = HipMatrix{Float32}(undef, size(spec, 1), Nr)
-create_fdr(spec::HipMatrix{Float32}, rates) = create_fdr(spec, length(rates))
+# ----------------------------------------------------------------- statistics and hits
+
+const HipFdrs = Union{Tuple{Vararg{HipMatrix{Float32}}},AbstractVector{<:HipMatrix{Float32}}}
 
 _ptrs(fdrs) = Ptr{Float32}[f.ptr for f in fdrs]
 
-"fdrstats(fdr) / fdrstats(fdrs) -- src/fdrutils.jl:39-50"
-fdrstats(fdr::HipMatrix{Float32}) = fdrstats((fdr,))
-function fdrstats(fdrs)
+function _stats(fdrs)
     Nf, Nr = size(first(fdrs))
     m = Ref{Cdouble}(0)
     s = Ref{Cdouble}(0)
@@ -289,32 +240,20 @@ function fdrstats(fdrs)
     (m[], s[])
 end
 
-fdrnormalize(x::Number, m, s) = (x - m) / s
-
-"fdrnormalize!(fdr[s][, m, s]) -- src/fdrutils.jl:65-90"
-fdrnormalize!(fdr::HipMatrix{Float32}, m, s) = (fdrnormalize!((fdr,), m, s); fdr)
-fdrnormalize!(fdr::HipMatrix{Float32}) = fdrnormalize!(fdr, fdrstats(fdr)...)
-function fdrnormalize!(fdrs, m, s)
+function _normalize!(fdrs, m, s)
     Nf, Nr = size(first(fdrs))
     check(ccall((:zdt_fdrnormalize, LIBZDT), Cint,
                 (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cdouble, Cdouble, Ptr{Cvoid}),
                 _ptrs(fdrs), length(fdrs), Nf, Nr, Nf, m, s, C_NULL))
     fdrs
 end
-fdrnormalize!(fdrs) = fdrnormalize!(fdrs, fdrstats(fdrs)...)
 
-"fdrdenormalize(snr, m, s) / fdrdenormalize(snr, fdr[s]) -- src/fdrutils.jl:104-119"
-fdrdenormalize(snr, m, s) = snr * s + m
-fdrdenormalize(snr, fdrs) = fdrdenormalize(snr, fdrstats(fdrs)...)
-
-"findprotohits(fdr[s], threshold; snr=false) -- src/fdrutils.jl:148-169"
-findprotohits(fdr::HipMatrix{Float32}, threshold; snr::Bool=false) = findprotohits((fdr,), threshold; snr)
-function findprotohits(fdrs, threshold; snr::Bool=false)
-    snr && (threshold = fdrdenormalize(threshold, fdrs))
+# every element >= threshold, column-major over the hcat'd matrices, as
+# 1-based CartesianIndex{2} (the device returns 0-based i + Nf*(b*Nr + j))
+function _hits(fdrs, threshold)
     Nf, Nr = size(first(fdrs))
     n = Ref{Int64}(0)
     ptrs = _ptrs(fdrs)
-    # count, then fetch the 0-based linear indices i + Nf*(b*Nr + j)
     rc = ccall((:zdt_findprotohits, LIBZDT), Cint,
                (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cdouble, Ptr{Int64}, Int64, Ref{Int64}, Ptr{Cvoid}),
                ptrs, length(fdrs), Nf, Nr, Nf, threshold, C_NULL, 0, n, C_NULL)
@@ -327,48 +266,7 @@ function findprotohits(fdrs, threshold; snr::Bool=false)
     [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
 end
 
-# ----------------------------------------------------------------- Fourier-shift de-doppler
-
-"fftfdr_workspace(spectrogram) -- src/fftfdr.jl:39-59 (any ZDT workspace holds the input spectrum)"
-fftfdr_workspace(spec::HipMatrix{Float32}; bunaligned=true) = ZDTWorkspace(spec, 0f0, 1f0, 1)
-
-"fftfdr_workspace!(workspace, spectrogram) -- src/fftfdr.jl:69-84"
-function fftfdr_workspace!(ws::ZDTWorkspace, spec::HipMatrix{Float32}; bunaligned=true)
-    size(spec) == (ws.Nf, ws.Nt) || error("input spectrogram has unexpected size")
-    input!(ws, spec)
-    ws
-end
-fftfdr_workspace!(::Nothing, spec::HipMatrix{Float32}; bunaligned=true) = fftfdr_workspace(spec)
-
-"fftfdr!(fdr, workspace, rates) -- src/fftfdr.jl:126-134"
-function fftfdr!(fdr::HipMatrix{Float32}, ws::ZDTWorkspace, rates)
-    r = Float32.(collect(rates))
-    size(fdr) == (ws.Nf, length(r)) || error("fdr must be $(ws.Nf) x $(length(r))")
-    check(ccall((:zdt_fftfdr, LIBZDT), Cint, (Ptr{Cvoid}, Ptr{Cfloat}, Int64, Ptr{Float32}),
-                ws.handle, r, length(r), fdr.ptr))
-    fdr
-end
-
-"fftfdr(workspace, rates) -- src/fftfdr.jl:143-148"
-fftfdr(ws::ZDTWorkspace, rates) = fftfdr!(HipMatrix{Float32}(undef, ws.Nf, length(rates)), ws, rates)
-
-"fdshiftsum!(dest, workspace, rate) / fdshiftsum(workspace, rate) -- src/fftfdr.jl:86-102"
-fdshiftsum!(dest::HipMatrix{Float32}, ws::ZDTWorkspace, rate) = fftfdr!(dest, ws, (rate,))
-fdshiftsum(ws::ZDTWorkspace, rate) = fftfdr(ws, (rate,))
-
-"fdshift!(dest, workspace, rate) -- src/fftfdr.jl:104-111"
-function fdshift!(dest::HipMatrix{Float32}, ws::ZDTWorkspace, rate)
-    size(dest) == (ws.Nf, ws.Nt) || error("dest must be $(ws.Nf) x $(ws.Nt)")
-    check(ccall((:zdt_fdshift, LIBZDT), Cint, (Ptr{Cvoid}, Cfloat, Ptr{Float32}), ws.handle, Float32(rate), dest.ptr))
-    dest
-end
-
-"fdshift(workspace, rate) -- src/fftfdr.jl:113-118"
-fdshift(ws::ZDTWorkspace, rate) = fdshift!(HipMatrix{Float32}(undef, ws.Nf, ws.Nt), ws, rate)
-
 # ----------------------------------------------------------------- ragged plans
-
-const PLAN_AUTOTUNE = 0x100
 
 """
     RaggedPlan(specs::Vector{HipMatrix{Float32}}, rates::Vector; autotune=true)
@@ -376,7 +274,7 @@ const PLAN_AUTOTUNE = 0x100
 Many coarse channels of uneven Nf and Nt through `input!`, one `zdtfdr!`
 batch, `fdrdenormalize` and `findprotohits`, replayed as one hipGraph launch
 (include/zdt.h, zdt_plan_*).  `rates[c]` is a range (`r0:δr:r1`) or an
-`(r0, δr, Nr)` tuple.  No counterpart in the reference: it replaces the
+`(r0, δr, Nr)` tuple.  No counterpart in the package: it replaces the
 user's loop over channels around src/zdtfdr.jl:349-421 and
 src/fdrutils.jl:148-169.
 """
