@@ -74,7 +74,9 @@ def test_stream_search_vs_fp64(F, mode):
     rates and 7 extra ones that must not enter the statistics or the hits."""
     import torch
     specs = specs3()
-    budget = 0 if mode == "resident" else 30 << 20
+    # 36 MiB: one 2^16 x 10 FDR and the workspace fit (34.6 MB), all four
+    # batch FDRs (42.5 MB) do not
+    budget = 0 if mode == "resident" else 36 << 20
     st = F.StreamSearch(NF, NT, (R0, DR, NR), Nrb=10, hbm_bytes=budget)
     say(f"stream {mode}: Nrb={st.Nrb} nbatch={st.nbatch} resident={st.resident}")
     assert (st.Nrb, st.nbatch) == (10, 4) and st.resident == (mode == "resident")
