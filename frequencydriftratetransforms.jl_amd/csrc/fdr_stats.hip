@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "zdt_kernels.h"
 
@@ -418,3 +419,89 @@ int launch_hits_threshold(const double* colmean, const double* colvar, int64_t N
 }
 
 }  // namespace zdt
+
+// ---------------------------------------------------------------- allocation-free findprotohits
+
+namespace zdt {
+
+// The tile counts, the scan's block totals and the total of
+// zdt_findprotohits_noalloc live in this file-scope array instead of a
+// per-call allocation: ZDT_HITS_NOALLOC_MAX_TILES counts, one total, and the
+// scan's ceil(max / SCAN_CHUNK) block totals (16 MiB + 8 KiB per device, held
+// until the process ends).  Host code never passes the array's name to a
+// kernel or a copy: it uses the device address hipGetSymbolAddress returns,
+// resolved once per device on the first call there (each device holds its
+// own copy of the array).
+constexpr int64_t NOALLOC_TILES = ZDT_HITS_NOALLOC_MAX_TILES;
+constexpr int64_t NOALLOC_WORDS = NOALLOC_TILES + 1 + (NOALLOC_TILES + SCAN_CHUNK - 1) / SCAN_CHUNK;
+static_assert(NOALLOC_TILES * HIT_TILE == ZDT_HITS_NOALLOC_MAX_CELLS, "header and tile size disagree");
+
+static __device__ __align__(256) int64_t scratch[NOALLOC_WORDS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int64_t* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch(int64_t** out) {
+  int dev = 0;
+  ZDT_TRY_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(ZDT_ENOTSUP, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = static_cast<int64_t*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return ZDT_OK;
+}
+}  // namespace
+
+}  // namespace zdt
+
+extern "C" int zdt_findprotohits_noalloc(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr, int64_t ld,
+                                         double threshold, int64_t* hits, int64_t cap, int64_t* nhits,
+                                         void* stream) {
+  using namespace zdt;
+  if (fdrs == nullptr || nb < 1) return fail(ZDT_EINVAL, "need at least one FDR matrix");
+  if (Nf < 1 || Nr < 1 || ld < Nf)
+    return fail(ZDT_EINVAL, "bad FDR shape Nf=%lld Nr=%lld ld=%lld", (long long)Nf, (long long)Nr, (long long)ld);
+  for (int b = 0; b < nb; ++b)
+    if (fdrs[b] == nullptr) return fail(ZDT_EINVAL, "FDR matrix %d is NULL", b);
+  if (cap < 0 || (cap > 0 && hits == nullptr)) return fail(ZDT_EINVAL, "findprotohits: bad output buffer");
+  // the size limit, checked before anything is launched
+  if (Nf > ZDT_HITS_NOALLOC_MAX_CELLS / Nr)
+    return fail(ZDT_ENOTSUP, "findprotohits (allocation-free): %lld x %lld cells exceed its limit of %lld tiles "
+                "of %lld cells", (long long)Nf, (long long)Nr, (long long)NOALLOC_TILES, (long long)HIT_TILE);
+  const int64_t nt = hits_tiles(Nf, Nr);
+  if (nt > NOALLOC_TILES / nb)
+    return fail(ZDT_ENOTSUP, "findprotohits (allocation-free): %d x %lld tiles exceed its limit of %lld tiles", nb,
+                (long long)nt, (long long)NOALLOC_TILES);
+  int64_t* buf = nullptr;
+  ZDT_TRY(noalloc_scratch(&buf));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t ntall = nt * nb;
+  int64_t* counts = buf;              // every tile's count is written by hits_count
+  int64_t* total_dev = buf + ntall;   // written by hits_scan
+  int64_t* scan_tmp = total_dev + 1;  // hits_scan_scratch(ntall) block totals, written by hits_scan_local
+  for (int b = 0; b < nb; ++b) ZDT_TRY(launch_hits_count(fdrs[b], Nf, Nr, ld, threshold, nullptr, counts + b * nt, st));
+  ZDT_TRY(launch_hits_scan(counts, ntall, total_dev, scan_tmp, st));
+  int64_t total = 0;
+  {
+    hipError_t e = hipMemcpyAsync(&total, total_dev, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "findprotohits count copy");
+  }
+  if (cap > 0 && total > 0) {
+    for (int b = 0; b < nb; ++b)
+      ZDT_TRY(launch_hits_write_impl(fdrs[b], Nf, Nr, ld, threshold, nullptr, counts, ntall, b * nt, total_dev,
+                                     static_cast<int64_t>(b) * Nf * Nr, hits, cap, st));
+  }
+  if (nhits) *nhits = total;
+  if (total > cap)
+    return fail(ZDT_ETRUNC, "findprotohits: %lld hits exceed capacity %lld", (long long)total, (long long)cap);
+  return ZDT_OK;
+}
+

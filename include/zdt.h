@@ -221,6 +221,22 @@ int zdt_findprotohits(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr,
                       int64_t ld, double threshold, int64_t* hits, int64_t cap,
                       int64_t* nhits, void* stream);
 
+/* zdt_findprotohits without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops and harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors as zdt_findprotohits; its tile counts and scan totals live in one
+ * library-owned device array per GPU of ZDT_HITS_NOALLOC_MAX_TILES tiles
+ * (16 MiB, held until the process ends), so nb * ceil(Nf*Nr/4096) must not
+ * exceed ZDT_HITS_NOALLOC_MAX_TILES (ZDT_ENOTSUP otherwise, before any
+ * launch).  Calls on one GPU share that array: calls that overlap in time on
+ * the same GPU (other host threads, or other streams) race and must not be
+ * issued. */
+#define ZDT_HITS_NOALLOC_MAX_TILES (1LL << 21)
+#define ZDT_HITS_NOALLOC_MAX_CELLS (ZDT_HITS_NOALLOC_MAX_TILES * 4096LL)
+int zdt_findprotohits_noalloc(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr,
+                              int64_t ld, double threshold, int64_t* hits, int64_t cap,
+                              int64_t* nhits, void* stream);
+
 /* --------------------------------------------------------- ragged plans
  * Many coarse channels of uneven Nf and Nt, every one through input!, one
  * zdtfdr! batch, fdrdenormalize and findprotohits, replayed as ONE hipGraph
