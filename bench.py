@@ -135,6 +135,27 @@ def copy_calibration() -> int:
 ACC_COLS = (0, 37, 64, 91, 128)  # rate columns whose FP64 oracle values the accuracy figures use
 
 
+def valu_calibration() -> int:
+    """FP32 VALU issue rates (zdt_calib_valu): four instruction forms with 1,
+    2, 4 and 8 independent dependency chains per lane over a full-device
+    launch.  Prints one JSON line, rates in TFLOP/s counting an FMA as two
+    flops (an add as two as well, for comparison)."""
+    import torch  # noqa: F401  (the HIP runtime the library binds to)
+
+    from fdrt_hip import _lib
+    L = _lib.lib()
+    forms = ("vop3_sgpr", "vop3_vgpr", "vop2_fmac", "vop2_add")
+    out = {}
+    for f, name in enumerate(forms):
+        out[name] = {}
+        for nch in (1, 2, 4, 8):
+            r = C.c_double()
+            _lib.check(L.zdt_calib_valu(f, nch, C.byref(r)))
+            out[name][f"chains_{nch}"] = 2.0 * r.value / 1e12
+    print(json.dumps({"valu_calibration_tflops": out, "peak": FP32_PEAK_TFLOPS}))
+    return 0
+
+
 def cpu_baseline(gpu_cols: dict, seed: int = 1234) -> tuple:
     """One batch of the FP32 CPU restatement of the reference on one core
     (scipy.fft workers=1: a default single-threaded Julia session runs FFTW
@@ -303,6 +324,8 @@ def main(argv=None) -> int:
     ap.add_argument("--fp32-only", action="store_true", help="skip the split-precision modes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (to scale rocprof byte counters)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only measure FP32 VALU issue rates (instruction forms x independent chains)")
     ap.add_argument("--ragged", action="store_true", help="the ragged multi-channel plan workload instead")
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
@@ -311,6 +334,8 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     if a.copy_calibration:
         return copy_calibration()
+    if a.valu_calibration:
+        return valu_calibration()
     if a.long:
         if a.gpus != 1:
             print("bench.py measures one GPU (N=1 only)", file=sys.stderr)

@@ -417,8 +417,9 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
           zq[i] = cmul(zq[i], gam[i]);
           const c64 zB = cmul(q, c64{zA.x, -zA.y});
           c32 YA[2], YB[2];  // [0]: rate jA, [1]: rate Pm - jA
-          reinsch2<NT>(aA[i], zA, YA[0], YA[1]);
-          reinsch2<NT>(aB[i], zB, YB[0], YB[1]);
+          reinsch_quad<NT>([&](int t) { return aA[i][t].x; }, [&](int t) { return aA[i][t].y; }, zA,
+                           [&](int t) { return aB[i][t].x; }, [&](int t) { return aB[i][t].y; }, zB, YA[0], YA[1],
+                           YB[0], YB[1]);
 #pragma unroll
           for (int m = 0; m < 2; ++m) {
             if (m == 1 && !twin) break;

@@ -398,6 +398,61 @@ __device__ __forceinline__ void reinsch_pair(const AR& ar, const AI& ai, c64 z, 
   Yzc = {Cr + Si, Ci - Sr};
 }
 
+// Two reinsch_pair evaluations, coefficients (ar, ai) at za and (br, bi) at
+// zb, advanced term by term in one loop: the four recurrences are
+// independent, so each term issues four independent VALU chains instead of
+// two (a lone pair of dependent chains leaves the VALU waiting on results).
+template <int NT, typename AR, typename AI, typename BR, typename BI>
+__device__ __forceinline__ void reinsch_quad(const AR& ar, const AI& ai, c64 za, const BR& br, const BI& bi,
+                                             c64 zb, c32& Ya, c32& Yac, c32& Yb, c32& Ybc) {
+  const float sa = za.x >= 0.0 ? 1.0f : -1.0f;
+  const float sb = zb.x >= 0.0 ? 1.0f : -1.0f;
+  const float la = static_cast<float>(2.0 * (za.x - static_cast<double>(sa))) * sa;
+  const float lb = static_cast<float>(2.0 * (zb.x - static_cast<double>(sb))) * sb;
+  constexpr bool ODD = ((NT - 1) & 1) != 0;
+  float Dar = ODD ? sa * ar(NT - 1) : ar(NT - 1);
+  float Dai = ODD ? sa * ai(NT - 1) : ai(NT - 1);
+  float Dbr = ODD ? sb * br(NT - 1) : br(NT - 1);
+  float Dbi = ODD ? sb * bi(NT - 1) : bi(NT - 1);
+  float Uar = Dar, Uai = Dai, Ubr = Dbr, Ubi = Dbi;
+#pragma unroll
+  for (int t = NT - 2; t >= 1; --t) {
+    if (t & 1) {
+      Dar = fmaf(sa, ar(t), Dar);
+      Dai = fmaf(sa, ai(t), Dai);
+      Dbr = fmaf(sb, br(t), Dbr);
+      Dbi = fmaf(sb, bi(t), Dbi);
+    } else {
+      Dar += ar(t);
+      Dai += ai(t);
+      Dbr += br(t);
+      Dbi += bi(t);
+    }
+    Dar = fmaf(la, Uar, Dar);
+    Dai = fmaf(la, Uai, Dai);
+    Dbr = fmaf(lb, Ubr, Dbr);
+    Dbi = fmaf(lb, Ubi, Dbi);
+    Uar += Dar;
+    Uai += Dai;
+    Ubr += Dbr;
+    Ubi += Dbi;
+  }
+  {
+    const float ssn = static_cast<float>(za.y) * sa, hl = 0.5f * la;
+    const float Cr = fmaf(hl, Uar, ar(0)) + Dar, Ci = fmaf(hl, Uai, ai(0)) + Dai;
+    const float Sr = ssn * Uar, Si = ssn * Uai;
+    Ya = {Cr - Si, Sr + Ci};
+    Yac = {Cr + Si, Ci - Sr};
+  }
+  {
+    const float ssn = static_cast<float>(zb.y) * sb, hl = 0.5f * lb;
+    const float Cr = fmaf(hl, Ubr, br(0)) + Dbr, Ci = fmaf(hl, Ubi, bi(0)) + Dbi;
+    const float Sr = ssn * Ubr, Si = ssn * Ubi;
+    Yb = {Cr - Si, Sr + Ci};
+    Ybc = {Cr + Si, Ci - Sr};
+  }
+}
+
 template <int NT>
 __device__ __forceinline__ void reinsch2(const c32 (&c)[NT], c64 z, c32& Yz, c32& Yzc) {
   reinsch_pair<NT>([&](int t) { return c[t].x; }, [&](int t) { return c[t].y; }, z, Yz, Yzc);

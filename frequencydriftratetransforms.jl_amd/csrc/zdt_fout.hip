@@ -115,64 +115,121 @@ __global__ __launch_bounds__(256) void czt_reinsch_col(CztColArgs a) {
   const c64 g = cis64f(kd * a.hd2);
   c64 z = cis64f(kd * a.hz);  // e^{2 pi i k (r_0 - rc)/Nf}
   const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
-  for (int j = 0; j < npair; ++j) {
+  // two rate pairs (j, j+1) per iteration: four independent recurrences
+  int j = 0;
+  for (; j + 1 < npair; j += 2) {
+    const c64 z1 = cmul(z, g);
+    c32 y0, y0c, y1, y1c;
+    reinsch_quad<NT>([&](int t) { return cr[t]; }, [&](int t) { return ci[t]; }, z, [&](int t) { return cr[t]; },
+                     [&](int t) { return ci[t]; }, z1, y0, y0c, y1, y1c);
+    const int jp = a.Nr - 1 - j;  // > j + 1: both partners are distinct
+    bin.dst[static_cast<int64_t>(j) * bin.stride] = y0;
+    bin.dst[static_cast<int64_t>(jp) * bin.stride] = y0c;
+    bin.dst[static_cast<int64_t>(j + 1) * bin.stride] = y1;
+    if (jp - 1 != j + 1) bin.dst[static_cast<int64_t>(jp - 1) * bin.stride] = y1c;
+    z = cmul(z1, g);
+  }
+  if (j < npair) {
     const int jp = a.Nr - 1 - j;
     c32 yz, yc;
     reinsch_split<NT>(cr, ci, z, yz, yc);
     bin.dst[static_cast<int64_t>(j) * bin.stride] = yz;
     if (jp != j) bin.dst[static_cast<int64_t>(jp) * bin.stride] = yc;
-    z = cmul(z, g);
   }
 }
 
-// Nt > 64: czt_reinsch_col over chunks of 64 time steps, sum_t c_t z^t =
-// sum_q z^{64q} P_q(z), with z^{64q} (conj for the partner rate) from an FP64
-// recurrence over the rates; chunk 0 writes Yc, later chunks add to it.
-__global__ __launch_bounds__(256) void czt_reinsch_col_chunks(CztColArgs a) {
+// Nt > 64: the sum in chunks of 64 time steps, sum_t c_t z^t =
+// sum_q z^{64q} P_q(z), with NC (a power of two >= ceil(Nt/64)) adjacent
+// lanes per bin: lane q evaluates P_q for every rate pair (the recurrence of
+// czt_reinsch_col, two pairs at a time), scales it by z^{64q} (conj for the
+// partner rate; FP64 recurrence over the rates) and the NC partial sums are
+// added across the lanes (butterfly shuffles), so each Yc entry is written
+// once.  Bins are numbered g = 0 .. M (g = M: the Nyquist bin) in col_bin's
+// tile order.
+template <int NC>
+__global__ __launch_bounds__(256) void czt_reinsch_col_split(CztColArgs a) {
   constexpr int NT = 64;
+  constexpr int BPB = 256 / NC;  // bins per block
+  const int q = threadIdx.x % NC;
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * BPB + threadIdx.x / NC;
+  if (g > a.M) return;  // whole groups of NC lanes leave together
   ColBin bin;
-  if (!col_bin(a, bin)) return;
+  if (g == a.M) {
+    bin.k = a.M;
+    bin.dst = a.yM;
+    bin.stride = 1;
+    bin.fo = a.ftrans ? a.M * a.Nt : a.M;
+    bin.fs = a.ftrans ? 1 : a.ldF;
+  } else {
+    const int ntk1 = (1 << a.logN1) >> 4;
+    const int64_t tile = g >> 8, w = g & 255;
+    const int k2 = static_cast<int>(((tile / ntk1) << 4) + (w >> 4));
+    const int k1 = static_cast<int>(((tile % ntk1) << 4) + (w & 15));
+    bin.k = k2 + (static_cast<int64_t>(k1) << a.logN2);
+    bin.dst = a.Yc + (static_cast<int64_t>(k2) << a.logN1) + k1;
+    bin.stride = a.M;
+    bin.fo = a.ftrans ? (static_cast<int64_t>(k2) << a.logN1) + k1 : bin.k;
+    bin.fs = a.ftrans ? a.M : a.ldF;
+  }
   const double kd = static_cast<double>(bin.k);
   const float w = fast_window(a.wkind, a.wcustom, bin.k, a.Nf) * a.scale;
-  const c64 g = cis64f(kd * a.hd2);
-  const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
-  for (int t0 = 0; t0 < a.Nt; t0 += NT) {
-    float cr[NT], ci[NT];
+  const int t0 = NT * q;
+  float cr[NT], ci[NT];
 #pragma unroll
-    for (int u = 0; u < NT; ++u) {
-      const int t = t0 + u;
-      if (t < a.Nt) {
-        c32 f = a.F[bin.fo + static_cast<int64_t>(t) * bin.fs];
-        if (a.hc != 0.0) f = cmul(f, cis_nh(kd * static_cast<double>(t), a.hc));  // uniform
-        cr[u] = f.x * w;
-        ci[u] = f.y * w;
-      } else {
-        cr[u] = 0.0f;
-        ci[u] = 0.0f;
-      }
+  for (int u = 0; u < NT; ++u) {
+    const int t = t0 + u;
+    if (t < a.Nt) {
+      c32 f = a.F[bin.fo + static_cast<int64_t>(t) * bin.fs];
+      if (a.hc != 0.0) f = cmul(f, cis_nh(kd * static_cast<double>(t), a.hc));  // uniform
+      cr[u] = f.x * w;
+      ci[u] = f.y * w;
+    } else {
+      cr[u] = 0.0f;
+      ci[u] = 0.0f;
     }
-    const double td = static_cast<double>(t0);
-    c64 z = cis64f(kd * a.hz);            // e^{2 pi i k (r_0 - rc)/Nf}
-    c64 zq = cis64f(kd * a.hz * td);      // z^{t0} of rate 0
-    const c64 gq = cis64f(kd * a.hd2 * td);  // gamma^{t0}
-    for (int j = 0; j < npair; ++j) {
-      const int jp = a.Nr - 1 - j;
-      c32 yz, yc;
-      reinsch_split<NT>(cr, ci, z, yz, yc);
-      const c32 q = {static_cast<float>(zq.x), static_cast<float>(zq.y)};
-      yz = cmul(yz, q);
-      yc = cmulc(yc, q);
-      c32* dj = bin.dst + static_cast<int64_t>(j) * bin.stride;
-      c32* dp = bin.dst + static_cast<int64_t>(jp) * bin.stride;
-      if (t0 > 0) {
-        yz = cadd(yz, *dj);
-        if (jp != j) yc = cadd(yc, *dp);
-      }
-      *dj = yz;
-      if (jp != j) *dp = yc;
-      z = cmul(z, g);
-      zq = cmul(zq, gq);
+  }
+  const c64 g1 = cis64f(kd * a.hd2);                              // z step between rates
+  const c64 gq = cis64f(kd * a.hd2 * static_cast<double>(t0));   // gamma^{t0}
+  c64 z = cis64f(kd * a.hz);                                      // e^{2 pi i k (r_0 - rc)/Nf}
+  c64 zq = cis64f(kd * a.hz * static_cast<double>(t0));           // z^{t0} of rate 0
+  auto reduce = [](c32 v) {
+#pragma unroll
+    for (int m = 1; m < NC; m <<= 1) {
+      v.x += __shfl_xor(v.x, m, 64);
+      v.y += __shfl_xor(v.y, m, 64);
     }
+    return v;
+  };
+  auto store = [&](int j, c32 y) {
+    if (q == 0) bin.dst[static_cast<int64_t>(j) * bin.stride] = y;
+  };
+  const int npair = (a.Nr + 1) / 2;  // j < npair; partner Nr-1-j
+  int j = 0;
+  for (; j + 1 < npair; j += 2) {
+    const c64 z1 = cmul(z, g1);
+    const c64 zq1 = cmul(zq, gq);
+    c32 y0, y0c, y1, y1c;
+    reinsch_quad<NT>([&](int t) { return cr[t]; }, [&](int t) { return ci[t]; }, z, [&](int t) { return cr[t]; },
+                     [&](int t) { return ci[t]; }, z1, y0, y0c, y1, y1c);
+    const c32 s0 = {static_cast<float>(zq.x), static_cast<float>(zq.y)};
+    const c32 s1 = {static_cast<float>(zq1.x), static_cast<float>(zq1.y)};
+    const int jp = a.Nr - 1 - j;  // > j + 1
+    store(j, reduce(cmul(y0, s0)));
+    store(jp, reduce(cmulc(y0c, s0)));
+    store(j + 1, reduce(cmul(y1, s1)));
+    const c32 r1c = reduce(cmulc(y1c, s1));
+    if (jp - 1 != j + 1) store(jp - 1, r1c);
+    z = cmul(z1, g1);
+    zq = cmul(zq1, gq);
+  }
+  if (j < npair) {
+    const int jp = a.Nr - 1 - j;
+    c32 yz, yc;
+    reinsch_split<NT>(cr, ci, z, yz, yc);
+    const c32 s0 = {static_cast<float>(zq.x), static_cast<float>(zq.y)};
+    store(j, reduce(cmul(yz, s0)));
+    const c32 rc = reduce(cmulc(yc, s0));
+    if (jp != j) store(jp, rc);
   }
 }
 
@@ -398,9 +455,9 @@ __device__ __forceinline__ void wave_ifft_2048(c32* x, const c32* tw, int lane, 
 // pairs each, groups of 4 rates (8 rows, one per wavefront) in one LDS
 // buffer; each wavefront runs the inverse DFT_N1 of its row in place
 // (wave_fft_row) and stores it, so a group costs two barriers.  The next
-// group's spectra are loaded into registers before the DFTs (at N1 = 2048
-// this spills a few registers outside the DFTs, and is still 7% faster than
-// loading after the barrier).
+// group's spectra are loaded into registers before the DFTs, so their
+// latency overlaps them (their addresses are formed inside the rate loop:
+// hoisted out of it, they spilled).
 template <int N1, int N2>
 __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
   constexpr int T = 512;
@@ -455,8 +512,12 @@ __global__ __launch_bounds__(512) void k2_c2r_wave(K2cArgs a) {
         const c32* y = a.Yc + static_cast<int64_t>(j0n + jj) * a.M;  // uniform
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
-          ya[jj][i] = y[offA[i]];
-          yb[jj][i] = nyq[i] ? a.yM[j0n + jj] : y[offB[i]];
+          // opaque offsets: the 64-bit addresses are formed here, not hoisted
+          // out of the rate loop (where they would be spilled)
+          int oa = offA[i], ob = offB[i];
+          asm volatile("" : "+v"(oa), "+v"(ob));
+          ya[jj][i] = y[oa];
+          yb[jj][i] = nyq[i] ? a.yM[j0n + jj] : y[ob];
         }
       }
     }
@@ -771,23 +832,41 @@ int fast_out_run(FastPlan* p, const c32* F, int64_t ldF, bool ftrans, int Nt, fl
   const double rc = static_cast<double>(r0) + 0.5 * static_cast<double>(p->Nr - 1) * static_cast<double>(dr);
   c.hc = rc / static_cast<double>(p->Nf);
   c.hz = (static_cast<double>(r0) - rc) / static_cast<double>(p->Nf);
-  {
+  if (Nt <= 64) {
     const unsigned grid = static_cast<unsigned>(p->M / 256 + 1);
-    // bytes: F read once; Yc written by the first 64-step chunk and read and
-    // rewritten by each further chunk
-    const int nchunk = (Nt + 63) / 64;
-    ProfScope ps(st, Nt <= 64 ? "czt_reinsch_col" : "czt_reinsch_col_chunks",
-                 8.0 * (p->M + 1) * (Nt + p->Nr * (2.0 * nchunk - 1.0)));
+    ProfScope ps(st, "czt_reinsch_col", 8.0 * (p->M + 1) * (Nt + p->Nr));  // F read, Yc written, once each
     if (Nt <= 16) {
       czt_reinsch_col<16><<<grid, 256, 0, st>>>(c);
     } else if (Nt <= 32) {
       czt_reinsch_col<32><<<grid, 256, 0, st>>>(c);
-    } else if (Nt <= 64) {
-      czt_reinsch_col<64><<<grid, 256, 0, st>>>(c);
     } else {
-      czt_reinsch_col_chunks<<<grid, 256, 0, st>>>(c);
+      czt_reinsch_col<64><<<grid, 256, 0, st>>>(c);
     }
     ZDT_CHECK_LAUNCH("czt_reinsch_col");
+  } else {
+    // NC lanes per bin: the 64-step chunks, a power of two (FOUT_MAX_NT = 1024: at most 16)
+    int nc = 2;
+    while (nc * 64 < Nt) nc *= 2;
+    const int64_t bins = p->M + 1;
+    const unsigned grid = static_cast<unsigned>((bins + 256 / nc - 1) / (256 / nc));
+    ProfScope ps(st, "czt_reinsch_col_split", 8.0 * (p->M + 1) * (Nt + p->Nr));
+    switch (nc) {
+      case 2:
+        czt_reinsch_col_split<2><<<grid, 256, 0, st>>>(c);
+        break;
+      case 4:
+        czt_reinsch_col_split<4><<<grid, 256, 0, st>>>(c);
+        break;
+      case 8:
+        czt_reinsch_col_split<8><<<grid, 256, 0, st>>>(c);
+        break;
+      case 16:
+        czt_reinsch_col_split<16><<<grid, 256, 0, st>>>(c);
+        break;
+      default:
+        return fail(ZDT_ENOTSUP, "fused output: Nt=%d above %lld", Nt, (long long)FOUT_MAX_NT);
+    }
+    ZDT_CHECK_LAUNCH("czt_reinsch_col_split");
   }
   K2cArgs a;
   a.Yc = p->Yc;

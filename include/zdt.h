@@ -197,6 +197,13 @@ int zdt_profile_reset(zdt_ws* ws);
 int zdt_profile_read(zdt_ws* ws, int idx, int* count, char* name, int name_len,
                      double* total_ms, int64_t* launches, double* bytes_per_launch);
 
+/* FP32 VALU issue rate of one instruction form (0: VOP3 FMA with SGPR
+ * operands, 1: VOP3 FMA, 2: VOP2 FMAC, 3: VOP2 add) with `chains` independent
+ * dependency chains per lane (1, 2, 4 or 8), in lane operations per second
+ * over a full-device launch.  No counterpart in the reference: a measurement
+ * for the compute roofline of bench.py (bench.py --valu-calibration). */
+int zdt_calib_valu(int form, int chains, double* lane_ops_per_s);
+
 /* ---------------------------------------------------------- statistics */
 
 /* fdrstats(fdr) / fdrstats(fdrs)  -- src/fdrutils.jl:39-50.
