@@ -297,14 +297,24 @@ def long_integration(a) -> int:
         specs.append(p)
     st = F.StreamSearch(Nf, Nt, (r0, dr, Nr))
     st.search(specs[:1], snr=10.0)  # warm-up
+    # one spectrogram alone: its host copy is not overlapped with anything
+    t0 = time.perf_counter()
+    st.search(specs[:1], snr=10.0)
+    dt1 = time.perf_counter() - t0
     t0 = time.perf_counter()
     hits, thr = st.search(specs, snr=10.0)
     dt = time.perf_counter() - t0
-    cells = a.spectrograms * Nf * Nr
+    n = a.spectrograms
+    cells = n * Nf * Nr
+    # each further spectrogram's copy overlaps the search of the one before:
+    # (T(n) - T(1)) / (n - 1) is the per-spectrogram time of a long stream
+    steady = (dt - dt1) / (n - 1) if n > 1 else None
     print(json.dumps({
         "metric": "FDR cells/s, long-integration search streamed from pinned host memory",
-        "value": cells / dt, "unit": "FDR cells/s", "n_gpus": 1, "spectrograms": a.spectrograms,
-        "ms_per_spectrogram": dt / a.spectrograms * 1e3, "higher_is_better": True, "dtype": "f32",
+        "value": cells / dt, "unit": "FDR cells/s", "n_gpus": 1, "spectrograms": n,
+        "ms_per_spectrogram": dt / n * 1e3, "higher_is_better": True, "dtype": "f32",
+        "steady_ms_per_spectrogram": steady * 1e3 if steady else None,
+        "steady_value": Nf * Nr / steady if steady else None, "single_spectrogram_ms": dt1 * 1e3,
         "data": "synthetic (seed 2024)",
         "config": {"workload": "Nf=2^22, Nt=64, rates -8:1/64:8 (Nr=1025), SNR 10", "Nrb": st.Nrb,
                    "nbatch": st.nbatch, "resident": st.resident, "device_bytes": st.nbytes},
@@ -330,7 +340,7 @@ def main(argv=None) -> int:
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--long", action="store_true", help="the long-integration streaming search instead")
-    ap.add_argument("--spectrograms", type=int, default=4)
+    ap.add_argument("--spectrograms", type=int, default=6)
     a = ap.parse_args(argv)
     if a.copy_calibration:
         return copy_calibration()
