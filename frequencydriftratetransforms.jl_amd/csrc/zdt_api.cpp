@@ -400,7 +400,7 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
   ws->r0 = r0;
   ws->dr = dr;
   ws->factors = f;
-  ws->flags = flags;
+  ws->flags = flags & ~ZDT_FLAG_NO_FUSED_INTERNAL;
   ws->stream = static_cast<hipStream_t>(stream);
   ws->hd = static_cast<double>(dr) / (2.0 * static_cast<double>(Nf));
   int rc = ZDT_OK;
@@ -408,7 +408,7 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
   if (flags & ZDT_FLAG_PROFILE) ws->prof = new Profiler();
   ws->maxr = maxr;
   ws->precision = precision;
-  const bool fused = !(flags & ZDT_FLAG_GENERIC) && fast_supported(Nf, Nt, Nr);
+  const bool fused = !(flags & (ZDT_FLAG_GENERIC | ZDT_FLAG_NO_FUSED_INTERNAL)) && fast_supported(Nf, Nt, Nr);
   const bool fout_ok = !(flags & ZDT_FLAG_GENERIC) && !fused && fast_out_supported(Nf, Nt, Nr);
   if (rc == ZDT_OK && precision != ZDT_PRECISION_FP32 &&
       !((fused && fast_precision_ok(Nf, n2, precision)) || (fout_ok && fast_precision_ok(Nf, 0, precision))))

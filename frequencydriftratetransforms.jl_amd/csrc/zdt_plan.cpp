@@ -13,7 +13,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <map>
+#include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -33,6 +37,8 @@ struct Shape {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
   int fused = 0, n2 = 0, maxr = 16;
+  int tune_src = 0;      // 0: defaults, 1: timed at plan creation, 2: from the tuning table
+  float tune_ms = 0.0f;  // the chosen candidate's time (input! + one batch), when known
   // detection scratch (reused by the shape's channels, which run in order)
   double* stats = nullptr;  // partial sums, colmean, colvar, threshold
   int64_t* counts = nullptr;  // tile counts / offsets
@@ -77,6 +83,10 @@ int time_candidate(const Shape& s, int flags, int n2, int maxr, const float* spe
   zdt_ws* ws = nullptr;
   ZDT_TRY(create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, flags, ZDT_PRECISION_FP32, s.st, n2, maxr,
                           &ws));
+  if (!(flags & (ZDT_FLAG_GENERIC | ZDT_FLAG_NO_FUSED_INTERNAL)) && n2 != 0 && ws_path(ws) != 1) {
+    zdt_destroy(ws);
+    return fail(ZDT_EHIP, "plan: fused candidate did not get the fused path");
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = ZDT_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(ZDT_EHIP, "plan: event");
@@ -98,27 +108,115 @@ int time_candidate(const Shape& s, int flags, int n2, int maxr, const float* spe
   return rc;
 }
 
-// Chooses n2 (fused) or maxr (generic) for a shape by timing the candidates.
+// ---------------------------------------------------------------- tuning table
+//
+// zdt_tune_table names a text file that keeps the autotuner's choices across
+// processes: one line per (device, shape),
+//   zdt-tune-v1 <gcnArchName> <CUs> <Nf> <Nt> <Nr> <forced generic 0/1> <path> <n2> <maxr> <ms>
+// Plans look their shapes up before timing anything and append what they
+// time.  Lines that do not parse are ignored; the last entry for a key wins.
+
+std::mutex g_tune_mu;
+std::string g_tune_path;  // empty: no table
+
+struct TuneKey {
+  std::string arch;
+  int cus = 0;
+  int64_t Nf = 0, Nt = 0, Nr = 0;
+  int generic = 0;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(arch, cus, Nf, Nt, Nr, generic) < std::tie(o.arch, o.cus, o.Nf, o.Nt, o.Nr, o.generic);
+  }
+};
+struct TuneVal {
+  int path = 0, n2 = 0, maxr = 16;
+  float ms = 0.0f;
+};
+
+int device_key(TuneKey& k) {
+  int dev = 0;
+  ZDT_TRY_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  ZDT_TRY_HIP(hipGetDeviceProperties(&prop, dev));
+  k.arch = prop.gcnArchName;
+  for (char& ch : k.arch)
+    if (ch == ' ') ch = '_';
+  k.cus = prop.multiProcessorCount;
+  return ZDT_OK;
+}
+
+bool tune_lookup(const TuneKey& key, TuneVal& out) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (g_tune_path.empty()) return false;
+  FILE* f = std::fopen(g_tune_path.c_str(), "r");
+  if (f == nullptr) return false;
+  char line[512];
+  bool found = false;
+  while (std::fgets(line, sizeof line, f)) {
+    char tag[32], arch[128];
+    int cus, gen, path, n2, maxr;
+    long long Nf, Nt, Nr;
+    float ms;
+    if (std::sscanf(line, "%31s %127s %d %lld %lld %lld %d %d %d %d %f", tag, arch, &cus, &Nf, &Nt, &Nr, &gen, &path,
+                    &n2, &maxr, &ms) != 11)
+      continue;
+    if (std::strcmp(tag, "zdt-tune-v1") != 0 || key.arch != arch || key.cus != cus || key.Nf != Nf ||
+        key.Nt != Nt || key.Nr != Nr || key.generic != gen)
+      continue;
+    if (path < 0 || path > 2 || (path == 1 && n2 != 512 && n2 != 1024) || (path == 0 && maxr != 8 && maxr != 16))
+      continue;
+    out = {path, n2, maxr, ms};
+    found = true;
+  }
+  std::fclose(f);
+  return found;
+}
+
+void tune_store(const TuneKey& key, const TuneVal& v) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (g_tune_path.empty()) return;
+  FILE* f = std::fopen(g_tune_path.c_str(), "a");
+  if (f == nullptr) return;  // a table that cannot be written only costs the next plan a retune
+  std::fprintf(f, "zdt-tune-v1 %s %d %lld %lld %lld %d %d %d %d %.4f\n", key.arch.c_str(), key.cus, (long long)key.Nf,
+               (long long)key.Nt, (long long)key.Nr, key.generic, v.path, v.n2, v.maxr, v.ms);
+  std::fclose(f);
+}
+
+// workspace flags that select a candidate's path
+int path_flags(int wflags, int path) {
+  if (path == 0) return wflags | ZDT_FLAG_GENERIC;
+  if (path == 2) return wflags | ZDT_FLAG_NO_FUSED_INTERNAL;
+  return wflags;
+}
+
+// Chooses the kernel path and its shape parameters for a shape by timing the
+// candidates (or from the tuning table): the fused kernels with strided FFTs
+// of 1024 or 512 points (k1/k3 tiles of 32 or 16 rows, 138 or 70 KB of LDS),
+// the fused output, and the generic kernels with radices up to 16 or 8.
 int autotune(Shape& s, int flags) {
   struct Cand {
-    int fused, n2, maxr;
+    int path, n2, maxr;
   };
+  const bool generic_only = (flags & ZDT_FLAG_GENERIC) != 0;
   std::vector<Cand> cands;
-  if (!(flags & ZDT_FLAG_GENERIC) && fast_supported(s.Nf, s.Nt, s.Nr)) {
+  if (!generic_only && fast_supported(s.Nf, s.Nt, s.Nr))
     for (int n2 : {1024, 512})
       if (fast_n2_ok(s.Nf, n2)) cands.push_back({1, n2, 16});
-  } else if (!(flags & ZDT_FLAG_GENERIC) && fast_out_supported(s.Nf, s.Nt, s.Nr)) {
-    // the fused output has one shape per Nf and runs no radix-capped FFT
-    // in the transform: nothing to choose
-  } else {
-    for (int maxr : {16, 8}) cands.push_back({0, 0, maxr});
-  }
-  if (cands.size() < 2) {
-    if (!cands.empty()) {
-      s.fused = cands[0].fused;
-      s.n2 = cands[0].n2;
-      s.maxr = cands[0].maxr;
-    }
+  if (!generic_only && fast_out_supported(s.Nf, s.Nt, s.Nr)) cands.push_back({2, 0, 16});
+  for (int maxr : {16, 8}) cands.push_back({0, 0, maxr});
+  TuneKey key;
+  ZDT_TRY(device_key(key));
+  key.Nf = s.Nf;
+  key.Nt = s.Nt;
+  key.Nr = s.Nr;
+  key.generic = generic_only ? 1 : 0;
+  TuneVal hit;
+  if (tune_lookup(key, hit)) {
+    s.fused = hit.path;
+    s.n2 = hit.n2;
+    s.maxr = hit.maxr;
+    s.tune_src = 2;
+    s.tune_ms = hit.ms;
     return ZDT_OK;
   }
   void *spec = nullptr, *dest = nullptr;
@@ -131,10 +229,11 @@ int autotune(Shape& s, int flags) {
   for (const Cand& c : cands) {
     if (rc != ZDT_OK) break;
     float ms = INFINITY;
-    rc = time_candidate(s, flags, c.n2, c.maxr, static_cast<float*>(spec), static_cast<float*>(dest), &ms);
+    rc = time_candidate(s, path_flags(flags, c.path), c.n2, c.maxr, static_cast<float*>(spec),
+                        static_cast<float*>(dest), &ms);
     if (rc == ZDT_OK && ms < best) {
       best = ms;
-      s.fused = c.fused;
+      s.fused = c.path;
       s.n2 = c.n2;
       s.maxr = c.maxr;
     }
@@ -142,6 +241,11 @@ int autotune(Shape& s, int flags) {
   (void)hipStreamSynchronize(s.st);
   if (spec) (void)hipFree(spec);
   if (dest) (void)hipFree(dest);
+  if (rc == ZDT_OK) {
+    s.tune_src = 1;
+    s.tune_ms = best;
+    tune_store(key, {s.fused, s.n2, s.maxr, best});
+  }
   return rc;
 }
 
@@ -261,10 +365,14 @@ int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* 
       break;
     }
     s.fused = (!(wflags & ZDT_FLAG_GENERIC) && fast_supported(s.Nf, s.Nt, s.Nr)) ? 1 : 0;
-    if (flags & ZDT_PLAN_AUTOTUNE) rc = autotune(s, wflags);
-    if (rc != ZDT_OK) break;
-    rc = create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, wflags, ZDT_PRECISION_FP32, s.st,
-                         s.fused ? s.n2 : 0, s.maxr, &s.ws);
+    int cflags = wflags;
+    if (flags & ZDT_PLAN_AUTOTUNE) {
+      rc = autotune(s, wflags);
+      if (rc != ZDT_OK) break;
+      cflags = path_flags(wflags, s.fused);
+    }
+    rc = create_ws_tuned(nullptr, s.Nf, s.Nt, s.r0, s.dr, s.Nr, nullptr, 0, cflags, ZDT_PRECISION_FP32, s.st,
+                         s.fused == 1 ? s.n2 : 0, s.maxr, &s.ws);
     if (rc != ZDT_OK) break;
     s.fused = ws_path(s.ws);
     rc = hip_alloc(reinterpret_cast<void**>(&s.stats), stats_doubles(s) * sizeof(double));
@@ -292,6 +400,21 @@ int zdt_plan_info(const zdt_plan* p, int ch, int* shape, int* path, int* n2, int
   const int n2_default = s.Nf >= (int64_t(1) << 17) ? 1024 : 512;
   if (n2) *n2 = s.fused == 1 ? (s.n2 ? s.n2 : n2_default) : (s.fused == 2 ? n2_default : 0);
   if (maxr) *maxr = s.fused != 0 ? 0 : s.maxr;
+  return ZDT_OK;
+}
+
+int zdt_plan_tuning(const zdt_plan* p, int ch, int* source, float* ms) {
+  if (p == nullptr) return fail(ZDT_EINVAL, "plan is NULL");
+  if (ch < 0 || ch >= static_cast<int>(p->ch_shape.size())) return fail(ZDT_EINVAL, "plan: channel %d out of range", ch);
+  const Shape& s = p->shapes[p->ch_shape[ch]];
+  if (source) *source = s.tune_src;
+  if (ms) *ms = s.tune_ms;
+  return ZDT_OK;
+}
+
+int zdt_tune_table(const char* path) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune_path = path ? path : "";
   return ZDT_OK;
 }
 

@@ -5,6 +5,10 @@
 
 namespace zdt {
 
+// internal flag of create_ws_tuned: skip the fused path (the fused output or
+// the generic kernels serve the shape); a bit above every public ZDT_FLAG_*
+constexpr int ZDT_FLAG_NO_FUSED_INTERNAL = 0x10000;
+
 // zdt_create with the tunables a plan's autotuner chooses: n2, the fused
 // path's strided FFT length (512 or 1024; 0 = the default for Nf), and maxr,
 // the generic FFTs' largest power-of-two radix (16, 8 or 4).

@@ -13,7 +13,7 @@ from .fdrutils import (create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_, 
                        findprotohits)
 from .fftfdr import (FftFdrWorkspace, fdshift, fdshift_, fdshiftsum, fdshiftsum_, fftfdr, fftfdr_,
                      fftfdr_workspace, fftfdr_workspace_)
-from .plan import RaggedPlan
+from .plan import RaggedPlan, set_tune_table
 from .stream import StreamSearch, pinned_spectrogram
 from .zdt import (ZDTWorkspace, convolve_, fdrsynchronize, input_, output_, postprocess_,
                   preprocess_, zdtfdr, zdtfdr_)
@@ -25,6 +25,6 @@ __all__ = [
     "zdtfdr", "zdtfdr_", "calcNl", "growNr", "estimate_memory", "driftrates", "batchrates",
     "RateRange", "workspace_bytes", "create_fdr", "fdrstats", "fdrsynchronize",
     "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "ZDTError", "lib",
-    "RaggedPlan", "FftFdrWorkspace", "fftfdr_workspace", "fftfdr_workspace_", "fftfdr", "fftfdr_",
+    "RaggedPlan", "set_tune_table", "FftFdrWorkspace", "fftfdr_workspace", "fftfdr_workspace_", "fftfdr", "fftfdr_",
     "fdshift", "fdshift_", "fdshiftsum", "fdshiftsum_", "StreamSearch", "pinned_spectrogram",
 ]

@@ -82,6 +82,14 @@ class RaggedPlan:
         return {"shape": v[0].value, "path": {1: "fused", 2: "fused-output"}.get(v[1].value, "generic"),
                 "n2": v[2].value, "maxr": v[3].value}
 
+    def tuning(self, c: int) -> dict:
+        """Where channel c's choices came from ("default", "timed" or "table")
+        and the chosen candidate's time in ms (input! + one batch)."""
+        src = C.c_int()
+        ms = C.c_float()
+        _lib.check(_lib.lib().zdt_plan_tuning(self._h, int(c), C.byref(src), C.byref(ms)))
+        return {"source": {0: "default", 1: "timed", 2: "table"}[src.value], "ms": ms.value}
+
     def bind(self, thresholds=10.0, *, snr: bool = True, cap: int = 4096, keep_fdrs: bool = True) -> None:
         """Captures the graph.  thresholds: one value or one per channel (SNRs
         when snr=True).  keep_fdrs=False lets channels of one shape share a
@@ -122,3 +130,9 @@ class RaggedPlan:
         if self.dests is None:
             raise RuntimeError("bind(keep_fdrs=True) keeps the FDRs")
         return self.dests[c]
+
+
+def set_tune_table(path: Optional[str]) -> None:
+    """Names the tuning table autotuned plans read and append to (None turns it
+    off; process-wide, zdt_tune_table)."""
+    _lib.check(_lib.lib().zdt_tune_table(None if path is None else str(path).encode()))

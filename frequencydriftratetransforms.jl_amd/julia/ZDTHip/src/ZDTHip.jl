@@ -17,7 +17,7 @@ live here.  Every library call returns a status; a non-zero status throws
 module ZDTHip
 
 export HipMatrix, HipZDTWorkspace, zdtpath, zdtprecision,
-       RaggedPlan, bind!, launch!, planhits, StreamSearch, search
+       RaggedPlan, bind!, launch!, planhits, plantuning, tune_table!, StreamSearch, search
 
 const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "libzdt_hip.so"))
 const LIBHIP = "libamdhip64.so"
@@ -336,6 +336,22 @@ end
 
 "launch!(plan): one hipGraphLaunch of the bound plan."
 launch!(P::RaggedPlan) = (check(ccall((:zdt_plan_launch, LIBZDT), Cint, (Ptr{Cvoid},), P.handle)); P)
+
+"""
+    plantuning(plan, c) -> (source, ms)
+
+Where channel c's kernel choice came from (`:default`, `:timed` or `:table`)
+and the chosen candidate's time for input! plus one batch (zdt_plan_tuning).
+"""
+function plantuning(P::RaggedPlan, c::Integer)
+    src, ms = Ref{Cint}(0), Ref{Cfloat}(0)
+    check(ccall((:zdt_plan_tuning, LIBZDT), Cint, (Ptr{Cvoid}, Cint, Ref{Cint}, Ref{Cfloat}), P.handle, c - 1, src, ms))
+    (Dict(0 => :default, 1 => :timed, 2 => :table)[src[]], ms[])
+end
+
+"tune_table!(path): the tuning table autotuned plans read and append to (`nothing` turns it off)."
+tune_table!(path::Union{Nothing,AbstractString}) =
+    check(ccall((:zdt_tune_table, LIBZDT), Cint, (Cstring,), path === nothing ? C_NULL : path))
 
 "planhits(plan, c): channel c's proto-hits as `CartesianIndex`es (like `findprotohits`)."
 function planhits(P::RaggedPlan, c::Integer)

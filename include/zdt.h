@@ -254,9 +254,12 @@ int zdt_findprotohits_noalloc(const float* const* fdrs, int nb, int64_t Nf, int6
  *
  * Channels with equal (Nf, Nt, r0, dr, Nr) share one workspace and run one
  * after another; distinct shapes run on parallel branches of the graph.
- * With ZDT_PLAN_AUTOTUNE, plan creation times the candidate kernel shapes of
- * each workspace (the fused path's strided FFT length N2 in {512, 1024}, or
- * the generic FFTs' radix cap in {16, 8}) and keeps the fastest. */
+ * With ZDT_PLAN_AUTOTUNE, plan creation times the candidate kernel paths and
+ * shapes of each workspace and keeps the fastest: the fused kernels with a
+ * strided FFT length N2 of 1024 or 512 (k1/k3 tiles of 32 or 16 rows), the
+ * fused output, and the generic kernels with FFT radices up to 16 or 8 (each
+ * where the shape allows it).  With a tuning table (zdt_tune_table) the
+ * choices persist across processes. */
 typedef struct zdt_plan zdt_plan;
 
 #define ZDT_PLAN_AUTOTUNE 0x100
@@ -277,6 +280,19 @@ int zdt_plan_destroy(zdt_plan* p);
  * radix cap of the generic path's FFTs; 0 on paths 1 and 2, which do not use
  * them for the transform). */
 int zdt_plan_info(const zdt_plan* p, int ch, int* shape, int* path, int* n2, int* maxr);
+
+/* Where channel c's choices came from: source 0 = defaults (no autotuning),
+ * 1 = timed at plan creation, 2 = read from the tuning table; ms = the chosen
+ * candidate's time for input! plus one batch (0 when not tuned). */
+int zdt_plan_tuning(const zdt_plan* p, int ch, int* source, float* ms);
+
+/* Names the tuning table (a text file, created on first write) that
+ * ZDT_PLAN_AUTOTUNE plans read before timing a shape and append their
+ * timings to, keyed by device (gcnArchName, CU count) and shape (Nf, Nt, Nr,
+ * forced generic); NULL or "" turns the table off (the default).
+ * Process-wide.  No counterpart in the reference (FFTW and cuFFT plan by
+ * themselves): it keeps a plan's start-up from re-timing known shapes. */
+int zdt_tune_table(const char* path);
 
 /* Binds device buffers and captures the graph (no work is done yet).
  * specs[c]: channel c's spectrogram.  dests[c]: its Nf x Nr FDR, or NULL (or
