@@ -295,7 +295,7 @@ def long_integration(a) -> int:
             np.add.at(rows[t], (np.round(c0 + rate * t).astype(np.int64) % Nf), amp)
         p.t().copy_(torch.from_numpy(rows))
         specs.append(p)
-    st = F.StreamSearch(Nf, Nt, (r0, dr, Nr))
+    st = F.StreamSearch(Nf, Nt, (r0, dr, Nr), precision=a.precision)
     st.search(specs[:1], snr=10.0)  # warm-up
     # one spectrogram alone: its host copy is not overlapped with anything
     t0 = time.perf_counter()
@@ -316,7 +316,8 @@ def long_integration(a) -> int:
         "steady_ms_per_spectrogram": steady * 1e3 if steady else None,
         "steady_value": Nf * Nr / steady if steady else None, "single_spectrogram_ms": dt1 * 1e3,
         "data": "synthetic (seed 2024)",
-        "config": {"workload": "Nf=2^22, Nt=64, rates -8:1/64:8 (Nr=1025), SNR 10", "Nrb": st.Nrb,
+        "config": {"workload": "Nf=2^22, Nt=64, rates -8:1/64:8 (Nr=1025), SNR 10", "precision": a.precision,
+                   "Nrb": st.Nrb,
                    "nbatch": st.nbatch, "resident": st.resident, "device_bytes": st.nbytes},
         "hits": [int(len(h)) for h in hits],
     }))
@@ -341,6 +342,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--long", action="store_true", help="the long-integration streaming search instead")
     ap.add_argument("--spectrograms", type=int, default=6)
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "split-bf16", "split-fp16"),
+                    help="--long: the column DFTs' arithmetic")
     a = ap.parse_args(argv)
     if a.copy_calibration:
         return copy_calibration()

@@ -123,7 +123,7 @@ int zdt_stream_destroy(zdt_stream* s) {
 }
 
 int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, int64_t Nrb, int64_t hbm_bytes,
-                      int flags, zdt_stream** out) {
+                      int flags, int precision, zdt_stream** out) {
   if (out == nullptr) return fail(ZDT_EINVAL, "stream: out is NULL");
   *out = nullptr;
   if (Nf < 2 || Nt < 1 || Nr < 1 || Nrb < 0 || hbm_bytes < 0)
@@ -186,7 +186,7 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
       rc = fail(ZDT_EHIP, "stream: event creation failed");
     if (rc == ZDT_OK) rc = salloc(s, reinterpret_cast<void**>(&s->spec[i]), static_cast<size_t>(Nf * Nt) * 4);
   }
-  if (rc == ZDT_OK) rc = zdt_create(nullptr, Nf, Nt, r0, dr, s->Nrb, nullptr, 0, flags, ZDT_PRECISION_FP32, s->compute,
+  if (rc == ZDT_OK) rc = zdt_create(nullptr, Nf, Nt, r0, dr, s->Nrb, nullptr, 0, flags, precision, s->compute,
                                       &s->ws);
   if (rc == ZDT_OK) {
     int64_t wb = 0;

@@ -39,7 +39,7 @@ class StreamSearch:
     """
 
     def __init__(self, Nf: int, Nt: int, rates, *, Nrb: int = 0, hbm_bytes: int = 0, generic: bool = False,
-                 unnormalized: bool = False):
+                 unnormalized: bool = False, precision: str = "fp32"):
         if isinstance(rates, tuple) and len(rates) == 3:
             r0, dr, Nr = float32_of(rates[0]), float32_of(rates[1]), int(rates[2])
         else:
@@ -47,10 +47,13 @@ class StreamSearch:
         self.Nf, self.Nt, self.Nr = int(Nf), int(Nt), int(Nr)
         self.r0, self.dr = np.float32(r0), np.float32(dr)
         flags = (_lib.ZDT_FLAG_GENERIC if generic else 0) | (_lib.ZDT_FLAG_UNNORMALIZED if unnormalized else 0)
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
+        self.precision = precision
         h = C.c_void_p()
         self._h = None
         _lib.check(_lib.lib().zdt_stream_create(self.Nf, self.Nt, float(self.r0), float(self.dr), self.Nr, int(Nrb),
-                                                int(hbm_bytes), flags, C.byref(h)))
+                                                int(hbm_bytes), flags, _lib.PRECISIONS[precision], C.byref(h)))
         self._h = h
         nrb, nb, res, by = C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()
         _lib.check(_lib.lib().zdt_stream_info(self._h, C.byref(nrb), C.byref(nb), C.byref(res), C.byref(by)))

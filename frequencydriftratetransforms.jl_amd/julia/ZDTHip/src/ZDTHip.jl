@@ -365,7 +365,7 @@ end
 # ----------------------------------------------------------------- long-integration search
 
 """
-    StreamSearch(Nf, Nt, rates; Nrb=0, hbm_bytes=0)
+    StreamSearch(Nf, Nt, rates; Nrb=0, hbm_bytes=0, precision=:fp32)
 
 Spectrograms of one shape streamed from host memory, rates cut into equal
 HBM-sized batches (include/zdt.h, zdt_stream_*).  Replaces the loop around
@@ -381,12 +381,14 @@ mutable struct StreamSearch
     nbatch::Int
 end
 
-function StreamSearch(Nf::Integer, Nt::Integer, rates; Nrb::Integer=0, hbm_bytes::Integer=0)
+function StreamSearch(Nf::Integer, Nt::Integer, rates; Nrb::Integer=0, hbm_bytes::Integer=0,
+                      precision::Symbol=:fp32)
     r0, δr, Nr = _rate_params(rates)
+    haskey(PRECISIONS, precision) || error("precision must be one of $(sort(collect(keys(PRECISIONS))))")
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:zdt_stream_create, LIBZDT), Cint,
-                (Int64, Int64, Cfloat, Cfloat, Int64, Int64, Int64, Cint, Ref{Ptr{Cvoid}}),
-                Nf, Nt, r0, δr, Nr, Nrb, hbm_bytes, 0, h))
+                (Int64, Int64, Cfloat, Cfloat, Int64, Int64, Int64, Cint, Cint, Ref{Ptr{Cvoid}}),
+                Nf, Nt, r0, δr, Nr, Nrb, hbm_bytes, 0, PRECISIONS[precision], h))
     nrb, nb = Ref{Int64}(0), Ref{Int64}(0)
     check(ccall((:zdt_stream_info, LIBZDT), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ptr{Cint}, Ptr{Int64}),
                 h[], nrb, nb, C_NULL, C_NULL))

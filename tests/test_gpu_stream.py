@@ -44,11 +44,12 @@ def specs3():
     return out
 
 
-def check_hits(hits, ref64, thr, cols=None):
+def check_hits(hits, ref64, thr, cols=None, tol=2e-6):
     """hits (1-based (i, j)) against findall(ref64 >= thr); cols: the 0-based
-    rate columns ref64 holds (None: all of them, in order)."""
+    rate columns ref64 holds (None: all of them, in order); tol: the FDR's
+    normwise tolerance."""
     scale = float(np.max(np.abs(ref64)))
-    eps = 4 * float(np.spacing(np.float32(thr))) + 2e-6 * scale
+    eps = 4 * float(np.spacing(np.float32(thr))) + tol * scale
     if cols is None:
         cols = list(range(ref64.shape[1]))
     colset = {c + 1: q for q, c in enumerate(cols)}
@@ -125,6 +126,29 @@ def test_stream_fused_output_and_generic_paths(F, Nf, Nt):
         assert abs(thr[i] - oracle_thr(ref, 7.0)) <= 1e-5 * abs(thr[i])
         check_hits(hits[i], ref, thr[i])
     st.close()
+
+
+@pytest.mark.parametrize("precision", ["split-bf16", "split-fp16"])
+@pytest.mark.parametrize("Nf,Nt", [(1 << 17, 16), (1 << 17, 20)])
+def test_stream_split_precision(F, Nf, Nt, precision):
+    """The search with its column DFTs on the matrix cores (fused path at
+    Nt <= 16, fused output above): thresholds within 1e-4 of the oracle's,
+    hits the FP64 oracle's up to cells within the split modes' 1e-4 normwise
+    tolerance of the threshold."""
+    r0, dr, Nr, Nrb = -2.0, 0.125, 33, 11
+    specs = [synth.spectrogram(Nf, Nt, [(Nf * 0.3, 1.5 - i, 30.0), (Nf * 0.7, -1.0, 25.0)], seed=500 + i)
+             for i in range(2)]
+    st = F.StreamSearch(Nf, Nt, (r0, dr, Nr), Nrb=Nrb, precision=precision)
+    say(f"stream {precision} Nf={Nf} Nt={Nt}: Nrb={st.Nrb} nbatch={st.nbatch}")
+    assert st.precision == precision
+    hits, thr = st.search(specs, snr=7.0, cap=1 << 16)
+    for i, s in enumerate(specs):
+        ref = Z.fdr64(s, r0, dr, Nr)
+        assert abs(thr[i] - oracle_thr(ref, 7.0)) <= 1e-4 * abs(thr[i])
+        assert check_hits(hits[i], ref, thr[i], tol=1e-4) > 0
+    st.close()
+    with pytest.raises(ValueError):
+        F.StreamSearch(Nf, Nt, (r0, dr, Nr), precision="bf16")
 
 
 def long_spectrogram(seed):
