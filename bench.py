@@ -6,8 +6,8 @@ spectrogram of Nf=2^20 channels x Nt=16 time steps (Exponential(1) noise plus
 One step = zdtfdr!(dest, ws, spectrogram) = input! + one ZDT batch writing the
 2^20 x 129 Float32 FDR, with the spectrogram already resident in HBM.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
-    python bench.py --ragged [--channels 64] [--steps 3] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 100] [--no-cpu-baseline]
+    python bench.py --ragged [--channels 64] [--steps 20] [--warmup 5]
     python bench.py --long [--spectrograms 4]
 
 The second form times the ragged multi-channel plan (one hipGraph launch
@@ -15,6 +15,10 @@ over 64 channels of uneven Nf and Nt), the third the long-integration
 search streamed from pinned host memory; each prints its own JSON line.
 
 Prints ONE JSON line.  value = FDR cells/s = Nf*Nr / (seconds per step).
+The default 100 warm-up steps (~45 ms) let the shader clock settle: the
+first ~60 back-to-back steps after an idle GPU run up to 17% slower (0.51 ms
+falling to a steady 0.43 ms per step), and a long search runs at the
+steady rate.
 The roofline object covers the kernel with the largest share of the step
 time, timed live with HIP events on the workspace stream (library profiling
 hook) over the same number of steps run right before the timed ones: an
@@ -328,8 +332,8 @@ def long_integration(a) -> int:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200; --ragged 20)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before them (default 100; --ragged 5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--generic", action="store_true", help="force the unfused kernels")
     ap.add_argument("--fp32-only", action="store_true", help="skip the split-precision modes")
@@ -345,6 +349,10 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", default="fp32", choices=("fp32", "split-bf16", "split-fp16"),
                     help="--long: the column DFTs' arithmetic")
     a = ap.parse_args(argv)
+    if a.steps is None:
+        a.steps = 20 if a.ragged else 200
+    if a.warmup is None:
+        a.warmup = 5 if a.ragged else 100
     if a.copy_calibration:
         return copy_calibration()
     if a.valu_calibration:

@@ -233,12 +233,21 @@ template <int N1, int N2, int NT, int ROWS, bool PAIRED = false>
 __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_czt(K2Args a) {
   constexpr int T = N1 < 512 ? N1 : 512;
   constexpr int PP = N1 / T;
-  constexpr int P = N1 + N1 / 8;  // swizzled row pitch (see lsw)
+  // N1 = 512: rows in layout 1 (rsw) with per-pass twiddle tables; other
+  // N1: rows padded one in eight (lsw), twiddle table swizzled alike
+  constexpr int LAY = N1 == 512 ? 1 : 0;
+  constexpr int P = N1 + N1 / 8;  // row pitch
   constexpr int HALF = N1 / 2;
+  auto ix = [](int i) { return row_index<LAY>(i); };
   __shared__ c32 buf[ROWS * P];
   __shared__ c32 buf2[ROWS * P];
-  __shared__ c32 twl[N1 + N1 / 8];  // W_N1^m at lsw(m)
-  for (int i = threadIdx.x; i < N1; i += T) twl[lsw(i)] = a.tw1[i];
+  __shared__ c32 twl[N1 + N1 / 8];
+  static_assert(LAY == 0 || LAY1_TW <= N1 + N1 / 8, "twiddle table size");
+  if constexpr (LAY == 1) {
+    lay1_load_twiddles(twl, a.tw1, T);
+  } else {
+    for (int i = threadIdx.x; i < N1; i += T) twl[lsw(i)] = a.tw1[i];  // W_N1^m at lsw(m)
+  }
   __shared__ c32 ahalf[NT];  // a_t of the bin M/2 (block 0 only)
 
   const int tid = threadIdx.x;
@@ -246,6 +255,26 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   const bool special = (p == 0);
   const int colA = special ? 0 : p;
   const int colB = special ? N2 / 2 : N2 - p;
+
+  // G rows of the input phase, loaded in batches of RB rows (row 2u + s:
+  // time step t0 + u, column A for s = 0, B for s = 1).  The first batch of
+  // each row group is loaded one group ahead (the first one here, before
+  // the set-up arithmetic), so its latency overlaps the previous group's FFT.
+  constexpr int TG = ROWS / 2;
+  constexpr int RB = 8 / PP;
+  auto load_batch = [&](int t0, int nt, int r0, c32 (&v)[RB][PP]) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      if (r0 + r < 2 * nt) {
+        const int col = ((r0 + r) & 1) ? colB : colA;
+        const c32* src = a.G + (static_cast<int64_t>(t0 + ((r0 + r) >> 1)) * N2 + col) * N1;
+#pragma unroll
+        for (int i = 0; i < PP; ++i) v[r][i] = src[tid + i * T];
+      }
+    }
+  };
+  c32 pf[RB][PP];
+  load_batch(0, min(TG, a.Nt), 0, pf);
 
   int rowA[PP], rowB[PP], k1A[PP], k1B[PP];
   bool nyq[PP];
@@ -286,44 +315,36 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
   }
 
   // ---- input: W_M^{n1 k2}, DFT_N1 over n1, r2c untangle, pre-rotation -> aA, aB
+  // (entries u >= Nt are zeroed after the loop, so that no register is held
+  // for a time step before its group is transformed)
   c32 aA[PP][NT], aB[PP][NT];
-#pragma unroll
-  for (int i = 0; i < PP; ++i)
-#pragma unroll
-    for (int u = 0; u < NT; ++u) {
-      aA[i][u] = {0.0f, 0.0f};
-      aB[i][u] = {0.0f, 0.0f};
-    }
-  constexpr int TG = ROWS / 2;
 #pragma unroll
   for (int t0 = 0; t0 < NT; t0 += TG) {
     if (t0 >= a.Nt) break;  // uniform
     const int nt = min(TG, a.Nt - t0);
-    // loads issued in batches of 8 before their LDS stores
-    constexpr int RB = 8 / PP;
 #pragma unroll
     for (int r0 = 0; r0 < ROWS; r0 += RB) {
       c32 v[RB][PP];
+      if (r0 == 0) {
 #pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        if (r0 + r < 2 * nt) {
-          const int col = ((r0 + r) & 1) ? colB : colA;
-          const c32* src = a.G + (static_cast<int64_t>(t0 + ((r0 + r) >> 1)) * N2 + col) * N1;
+        for (int r = 0; r < RB; ++r)
 #pragma unroll
-          for (int i = 0; i < PP; ++i) v[r][i] = src[tid + i * T];
-        }
+          for (int i = 0; i < PP; ++i) v[r][i] = pf[r][i];
+      } else {
+        load_batch(t0, nt, r0, v);
       }
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         if (r0 + r < 2 * nt) {
 #pragma unroll
           for (int i = 0; i < PP; ++i)
-            buf[(r0 + r) * P + lsw(tid + i * T)] = cmulc(v[r][i], ((r0 + r) & 1) ? twS1[i] : twS0[i]);
+            buf[(r0 + r) * P + ix(tid + i * T)] = cmulc(v[r][i], ((r0 + r) & 1) ? twS1[i] : twS0[i]);
         }
       }
     }
+    if (t0 + TG < NT && t0 + TG < a.Nt) load_batch(t0 + TG, min(TG, a.Nt - t0 - TG), 0, pf);  // one group ahead
     __syncthreads();
-    const c32* fz = lds_fft_pp<N1, T, false, 1, true>(buf, buf2, P, 2 * nt, twl);
+    const c32* fz = lds_fft_pp<N1, T, false, 1, true, LAY>(buf, buf2, P, 2 * nt, twl);
 #pragma unroll
     for (int uu = 0; uu < TG; ++uu) {
       const int u = t0 + uu;
@@ -331,13 +352,13 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
         const double ud = static_cast<double>(u);
 #pragma unroll
         for (int i = 0; i < PP; ++i) {
-          const c32 Za = fz[(2 * uu + rowA[i]) * P + lsw(k1A[i])];
+          const c32 Za = fz[(2 * uu + rowA[i]) * P + ix(k1A[i])];
           c32 XA, XB;
           if (nyq[i]) {
             XA = {Za.x + Za.y, 0.0f};
             XB = {Za.x - Za.y, 0.0f};
           } else {
-            const c32 Zb = fz[(2 * uu + rowB[i]) * P + lsw(k1B[i])];
+            const c32 Zb = fz[(2 * uu + rowB[i]) * P + ix(k1B[i])];
             const c32 E = {0.5f * (Za.x + Zb.x), 0.5f * (Za.y - Zb.y)};
             const c32 O = {0.5f * (Za.y + Zb.y), -0.5f * (Za.x - Zb.x)};
             const c32 WO = cmul(WA[i], O);
@@ -358,7 +379,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
         if (special && tid == 0) {
           // bin M/2: X = conj(Z[M/2]); rotation e^{2 pi i (M/2) u r0/Nf} = e^{i pi u r0/2}
           const float wH = fast_window(a.wkind, a.wcustom, a.M / 2, a.Nf) * a.oscale;
-          const c32 Zh = fz[(2 * uu) * P + lsw(HALF)];
+          const c32 Zh = fz[(2 * uu) * P + ix(HALF)];
           const c32 h = cmul(cconj(Zh), cis_nh(ud, 0.5 * a.r0half));
           ahalf[u] = {h.x * wH, h.y * wH};
         }
@@ -366,6 +387,15 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int i = 0; i < PP; ++i)
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      if (u >= a.Nt) {
+        aA[i][u] = {0.0f, 0.0f};
+        aB[i][u] = {0.0f, 0.0f};
+      }
+    }
 
   if (special) {
     // bin M/2 (self-paired): gamma = e^{i pi dr/2}; its c2r pre-pass value is
@@ -387,11 +417,10 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
     const int wv = tid >> 6, lane = tid & 63;
     const int Pm = a.Nr - 1;        // rate j pairs with Pm - j
     const int npairs = Pm / 2 + 1;  // the last is the middle rate when Pm is even
-    c64 gam[PP], gamrg[PP], zp[PP], zq[PP];
+    c64 gam[PP], zp[PP], zq[PP];
 #pragma unroll
     for (int i = 0; i < PP; ++i) {
       gam[i] = cis64f(static_cast<double>(kA[i]) * a.hd2);
-      gamrg[i] = cis64f(-static_cast<double>(NPG) * static_cast<double>(kA[i]) * a.hd2);
     }
     // groups from the middle rates outwards, so that the rates k3 reads
     // first (Nr-1, 0, Nr-2, 1, ...) are the ones written last
@@ -426,13 +455,13 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
             const int slot = 2 * pp + m;
             if (nyq[i]) {
               cur[(2 * slot + rowA[i]) * P] = {YA[m].x + YB[m].x, YA[m].x - YB[m].x};
-              cur[(2 * slot + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
+              cur[(2 * slot + rowA[i]) * P + ix(HALF)] = {0.0f, 0.0f};
             } else {
               const c32 E = {YA[m].x + YB[m].x, YA[m].y - YB[m].y};
               const c32 D = {YA[m].x - YB[m].x, YA[m].y + YB[m].y};
               const c32 O = cmulc(D, WA[i]);
-              cur[(2 * slot + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
-              cur[(2 * slot + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};
+              cur[(2 * slot + rowA[i]) * P + ix(k1A[i])] = {E.x - O.y, E.y + O.x};
+              cur[(2 * slot + rowB[i]) * P + ix(k1B[i])] = {E.x + O.y, -E.y + O.x};
             }
           }
         }
@@ -441,7 +470,7 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       // slots 2pp, 2pp+1 hold rates p0+pp, Pm-p0-pp (the middle rate once, last)
       const int nslots = 2 * np - ((Pm & 1) == 0 && p0 + np == npairs ? 1 : 0);
       const int nrw = min(RPW, max(0, 2 * nslots - wv * RPW));  // uniform per wavefront
-      wave_fft<N1, true, 1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+      wave_fft<N1, true, 1, true, LAY>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
         constexpr int RL = sizeof(v) / sizeof(c32);
         const int row = wv * RPW + f;
         const int slot = row >> 1;
@@ -457,7 +486,12 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
       cur = nxt;
       nxt = t;
 #pragma unroll
-      for (int i = 0; i < PP; ++i) zp[i] = cmul(zp[i], gamrg[i]);  // back one group
+      for (int i = 0; i < PP; ++i) {  // back one group: conj(gamma)^NPG, formed here to spare registers
+        c64 g = {gam[i].x, -gam[i].y};
+#pragma unroll
+        for (int s = 1; s < NPG; ++s) g = cmul(g, c64{gam[i].x, -gam[i].y});
+        zp[i] = cmul(zp[i], g);
+      }
     }
   } else {
     // ---- chirp-Z for every rate, c2r pre-pass, inverse DFT_N1, store
@@ -498,19 +532,19 @@ __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_c
             // pair (0, M): only the real parts enter the c2r; the bin M/2 slot
             // of the row is zero here (k3 adds it)
             cur[(2 * jj + rowA[i]) * P] = {YA.x + YB.x, YA.x - YB.x};
-            cur[(2 * jj + rowA[i]) * P + lsw(HALF)] = {0.0f, 0.0f};
+            cur[(2 * jj + rowA[i]) * P + ix(HALF)] = {0.0f, 0.0f};
           } else {
             const c32 E = {YA.x + YB.x, YA.y - YB.y};  // (YA + conj YB)/2
             const c32 D = {YA.x - YB.x, YA.y + YB.y};  // (YA - conj YB)/2
             const c32 O = cmulc(D, WA[i]);  // W^{-k} D
-            cur[(2 * jj + rowA[i]) * P + lsw(k1A[i])] = {E.x - O.y, E.y + O.x};
-            cur[(2 * jj + rowB[i]) * P + lsw(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
+            cur[(2 * jj + rowA[i]) * P + ix(k1A[i])] = {E.x - O.y, E.y + O.x};
+            cur[(2 * jj + rowB[i]) * P + ix(k1B[i])] = {E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
           }
         }
       }
       __syncthreads();
       const int nrw = min(RPW, max(0, 2 * nj - wv * RPW));  // uniform per wavefront
-      wave_fft<N1, true, 1, true>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
+      wave_fft<N1, true, 1, true, LAY>(cur + wv * RPW * P, P, nrw, twl, lane, [&](int f, int jx, auto& v) {
         constexpr int RL = sizeof(v) / sizeof(c32);
         const int row = wv * RPW + f;
         // B is blocked by 16 n1: B[j][n1/16][k2][n1%16]

@@ -27,6 +27,38 @@ constexpr int fft_radix() {
 // N + N/8 entries.
 __device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
 
+// Second row layout (LAY = 1), for 512-point rows of the radix 8*8*8 passes:
+// entry i at rsw(i) = i + i/32 + 4 (i/64), pitch 576.  Measured on gfx950,
+// 8-byte LDS accesses serve 32 lanes per cycle over 64 banks; under that
+// model the contiguous reads (j + 64 r) and runs, and the first-pass writes
+// (8 j + r), are conflict-free and the second-pass writes (64 (j/8) +
+// j%8 + 8 r) are two-way, where the one-in-eight padding of lsw costs two
+// extra cycles on every contiguous access.  The offsets stay immediate:
+// rsw(j + 64 r) = rsw(j) + 70 r, rsw(8 j + r) = rsw(8 j) + r and, for
+// d0 % 64 < 8, rsw(d0 + 8 r) = rsw(d0) + 8 r + r/4.  Twiddles come from
+// per-pass tables (lay1_load_twiddles), read at contiguous or broadcast
+// addresses.
+__device__ __forceinline__ int rsw(int i) { return i + (i >> 5) + ((i >> 6) << 2); }
+
+template <int LAY>
+__device__ __forceinline__ int row_index(int i) {
+  return LAY == 1 ? rsw(i) : lsw(i);
+}
+
+// Per-pass twiddle table of the LAY = 1 passes (N = 512): the pass at stride
+// Ns (8 or 64) reads W^{(r k) * 512 / (8 Ns)} at (Ns - 8) + (r - 1) Ns + k,
+// k < Ns, r = 1..7 (504 entries), W^m = tw[m].
+constexpr int LAY1_TW = 504;
+
+__device__ __forceinline__ void lay1_load_twiddles(c32* twp, const c32* __restrict__ tw, int nthreads) {
+  for (int e = threadIdx.x; e < LAY1_TW; e += nthreads) {
+    const int Ns = e < 56 ? 8 : 64;
+    const int q = e - (Ns - 8);
+    const int r = q / Ns + 1, k = q % Ns;
+    twp[e] = tw[(r * k) * (64 / Ns)];
+  }
+}
+
 // Twiddle tables in LDS are read at (r k) << sh, a stride of 2^sh entries
 // across the lanes of a pass; with SW the passes read a table stored
 // swizzled like the rows (entry m at lsw(m), N + N/8 entries), which spreads
@@ -38,7 +70,7 @@ __device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
 // every butterfly needs the swizzle only once for its reads and once for its
 // writes.  Twiddles come from the LDS table tw (W_N^m, m < N); INV selects the
 // inverse butterflies and conjugate twiddles.
-template <int N, int R, int T, bool INV, bool SW = false>
+template <int N, int R, int T, bool INV, bool SW = false, int LAY = 0>
 __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int nrows, int Ns,
                                             const c32* __restrict__ tw) {
   constexpr int NR = N / R;
@@ -46,6 +78,7 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
   constexpr int LOGN = ilog2c(N);
   constexpr int LOGR = ilog2c(R);
   static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
+  static_assert(LAY == 0 || (N == 512 && R == 8), "layout 1 is the 512 = 8*8*8 schedule");
   const int logNs = 31 - __builtin_clz(Ns);
   const int total = nrows * NR;
   const int sh = LOGN - logNs - LOGR;
@@ -56,14 +89,14 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
     // Opaque to the optimizer: keeps the R-1 twiddle addresses from being
     // hoisted out of the caller's rate loop, where they would be spilled.
     asm volatile("" : "+v"(k));
-    const c32* x = in + f * P + lsw(j);
+    const c32* x = in + f * P + row_index<LAY>(j);
     c32 v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = x[r * (NR + NR / 8)];
+    for (int r = 0; r < R; ++r) v[r] = x[r * (LAY == 1 ? 70 : NR + NR / 8)];
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const c32 w = tw[SW ? lsw((r * k) << sh) : (r * k) << sh];
+        const c32 w = LAY == 1 ? tw[(Ns - 8) + (r - 1) * Ns + k] : tw[SW ? lsw((r * k) << sh) : (r * k) << sh];
         v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
       }
     }
@@ -71,15 +104,21 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
     const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
     c32* y = out + f * P;
     if (Ns == 1) {
-      // d0 = j*R is a multiple of 8 (R >= 8): lsw(d0 + r) = lsw(d0) + r + r/8
-      c32* yy = y + lsw(d0);
+      // d0 = j*R is a multiple of 8 (R >= 8): lsw(d0 + r) = lsw(d0) + r + r/8,
+      // rsw(d0 + r) = rsw(d0) + r
+      c32* yy = y + row_index<LAY>(d0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
+      for (int r = 0; r < R; ++r) yy[LAY == 1 ? r : r + (r >> 3)] = v[r];
     } else {
       // Ns >= 8: the stride Ns keeps the swizzle offset linear
-      c32* yy = y + lsw(d0);
+      c32* yy = y + row_index<LAY>(d0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (LAY == 1)
+          yy[Ns == 8 ? 8 * r + (r >> 2) : 70 * r] = v[r];  // d0 % 64 < 8 (Ns = 8) or d0 < 64 (Ns = 64)
+        else
+          yy[r * (Ns + (Ns >> 3))] = v[r];
+      }
     }
   }
   __syncthreads();
@@ -87,12 +126,12 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
 
 // radix schedule: 1024 = 16*8*8, 512 = 8*8*8, 256 = 16*16, 128 = 16*8,
 // 64 = 8*8; returns the buffer holding the result
-template <int N, int T, bool INV, int NS = 1, bool SW = false>
+template <int N, int T, bool INV, int NS = 1, bool SW = false, int LAY = 0>
 __device__ __forceinline__ c32* lds_fft_pp(c32* a, c32* b, int P, int nrows, const c32* __restrict__ tw) {
   if constexpr (NS < N) {
     constexpr int R = fft_radix<N, NS>();
-    lds_pass_pp<N, R, T, INV, SW>(a, b, P, nrows, NS, tw);
-    return lds_fft_pp<N, T, INV, NS * R, SW>(b, a, P, nrows, tw);
+    lds_pass_pp<N, R, T, INV, SW, LAY>(a, b, P, nrows, NS, tw);
+    return lds_fft_pp<N, T, INV, NS * R, SW, LAY>(b, a, P, nrows, tw);
   } else {
     return a;
   }
@@ -111,7 +150,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int N, int R, bool INV, bool SW = false>
+template <int N, int R, bool INV, bool SW = false, int LAY = 0>
 __device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, const c32* __restrict__ tw, int lane,
                                           c32 (&v)[R], bool last) {
   constexpr int NR = N / R;
@@ -119,34 +158,44 @@ __device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, con
   constexpr int LOGN = ilog2c(N);
   constexpr int LOGR = ilog2c(R);
   static_assert(NR % 8 == 0, "pass layout assumes N/R is a multiple of 8");
+  static_assert(LAY == 0 || (N == 512 && R == 8), "layout 1 is the 512 = 8*8*8 schedule");
   if (lane < nrw * NR) {
     const int logNs = 31 - __builtin_clz(Ns);
-    const int f = lane >> LOGNR;
-    const int j = lane & (NR - 1);
+    int ln = lane;
+    // the lane's row and twiddle addresses depend on nothing else: opaque,
+    // they are formed here, not hoisted out of the caller's loop and spilled
+    if constexpr (LAY == 1) asm volatile("" : "+v"(ln));
+    const int f = ln >> LOGNR;
+    const int j = ln & (NR - 1);
     int k = j & (Ns - 1);
     asm volatile("" : "+v"(k));  // keeps twiddle addresses out of the caller's loop
     c32* x = rows + f * P;
-    const c32* xr = x + lsw(j);
+    const c32* xr = x + row_index<LAY>(j);
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = xr[r * (NR + NR / 8)];
+    for (int r = 0; r < R; ++r) v[r] = xr[r * (LAY == 1 ? 70 : NR + NR / 8)];
     if (Ns > 1) {
       const int sh = LOGN - logNs - LOGR;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const c32 w = tw[SW ? lsw((r * k) << sh) : (r * k) << sh];
+        const c32 w = LAY == 1 ? tw[(Ns - 8) + (r - 1) * Ns + k] : tw[SW ? lsw((r * k) << sh) : (r * k) << sh];
         v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
       }
     }
     dftd<INV, float, R>(v);
     if (!last) {
       const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
-      c32* yy = x + lsw(d0);
+      c32* yy = x + row_index<LAY>(d0);
       if (Ns == 1) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
+        for (int r = 0; r < R; ++r) yy[LAY == 1 ? r : r + (r >> 3)] = v[r];
       } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) yy[r * (Ns + (Ns >> 3))] = v[r];
+        for (int r = 0; r < R; ++r) {
+          if constexpr (LAY == 1)
+            yy[Ns == 8 ? 8 * r + (r >> 2) : 70 * r] = v[r];
+          else
+            yy[r * (Ns + (Ns >> 3))] = v[r];
+        }
       }
     }
   }
@@ -167,15 +216,15 @@ constexpr int fft_max_nr() {
 // In-place inverse/forward FFT of the wavefront's rows; the last pass leaves
 // its butterfly outputs in registers (lane (f, j): row f, elements
 // j + r*N/R) and calls emit(f, j, v) instead of writing them back.
-template <int N, bool INV, int NS = 1, bool SW = false, typename Emit>
+template <int N, bool INV, int NS = 1, bool SW = false, int LAY = 0, typename Emit>
 __device__ __forceinline__ void wave_fft(c32* rows, int P, int nrw, const c32* __restrict__ tw, int lane, Emit&& emit) {
   constexpr int R = fft_radix<N, NS>();
   c32 v[R];
   if constexpr (NS * R < N) {
-    wave_pass<N, R, INV, SW>(rows, P, nrw, NS, tw, lane, v, false);
-    wave_fft<N, INV, NS * R, SW>(rows, P, nrw, tw, lane, emit);
+    wave_pass<N, R, INV, SW, LAY>(rows, P, nrw, NS, tw, lane, v, false);
+    wave_fft<N, INV, NS * R, SW, LAY>(rows, P, nrw, tw, lane, emit);
   } else {
-    wave_pass<N, R, INV, SW>(rows, P, nrw, NS, tw, lane, v, true);
+    wave_pass<N, R, INV, SW, LAY>(rows, P, nrw, NS, tw, lane, v, true);
     constexpr int NR = N / R;
     if (lane < nrw * NR) emit(lane / NR, lane % NR, v);
   }
