@@ -233,8 +233,9 @@ template <int N1, int N2, int NT, int ROWS, bool PAIRED = false>
 __global__ __launch_bounds__(N1 < 512 ? N1 : 512, k2_waves_per_eu(N1)) void k2_czt(K2Args a) {
   constexpr int T = N1 < 512 ? N1 : 512;
   constexpr int PP = N1 / T;
-  // N1 = 512: rows in layout 1 (rsw) with per-pass twiddle tables; other
-  // N1: rows padded one in eight (lsw), twiddle table swizzled alike
+  // N1 = 512: rows in layout 1 (q64, the first pass's output at lsw) with
+  // per-pass twiddle tables; other N1: rows padded one in eight (lsw),
+  // twiddle table swizzled alike
   constexpr int LAY = N1 == 512 ? 1 : 0;
   constexpr int P = N1 + N1 / 8;  // row pitch
   constexpr int HALF = N1 / 2;

@@ -27,22 +27,25 @@ constexpr int fft_radix() {
 // N + N/8 entries.
 __device__ __forceinline__ int lsw(int i) { return i + (i >> 3); }
 
-// Second row layout (LAY = 1), for 512-point rows of the radix 8*8*8 passes:
-// entry i at rsw(i) = i + i/32 + 4 (i/64), pitch 576.  Measured on gfx950,
-// 8-byte LDS accesses serve 32 lanes per cycle over 64 banks; under that
-// model the contiguous reads (j + 64 r) and runs, and the first-pass writes
-// (8 j + r), are conflict-free and the second-pass writes (64 (j/8) +
-// j%8 + 8 r) are two-way, where the one-in-eight padding of lsw costs two
-// extra cycles on every contiguous access.  The offsets stay immediate:
-// rsw(j + 64 r) = rsw(j) + 70 r, rsw(8 j + r) = rsw(8 j) + r and, for
-// d0 % 64 < 8, rsw(d0 + 8 r) = rsw(d0) + 8 r + r/4.  Twiddles come from
-// per-pass tables (lay1_load_twiddles), read at contiguous or broadcast
-// addresses.
-__device__ __forceinline__ int rsw(int i) { return i + (i >> 5) + ((i >> 6) << 2); }
+// Second row layout (LAY = 1), for 512-point rows of the radix 8*8*8
+// passes: rows hold entry i at q64(i) = i + 8 (i/64) (pitch 576), except the
+// output of the first pass, which is written (and read by the second pass)
+// at lsw(i).  Measured on gfx950 (SQ_LDS_BANK_CONFLICT), 8-byte LDS reads are
+// served 32 lanes per cycle over 64 banks and 8-byte writes 16 lanes per
+// cycle over 32 banks.  Under that model, q64 keeps the contiguous reads and
+// writes (j + 64 r, runs of lanes) and the second-pass writes (64 (j/8) + j%8
+// + 8 r) conflict-free; no padding of this kind does so for the first-pass
+// writes (8 j + r), which lsw does, at the cost of two-way conflicts on the
+// second pass's reads -- one conflicted access pattern per 512-point FFT
+// where lsw throughout has four.  Offsets stay immediate: q64(j + 64 r) =
+// q64(j) + 72 r = lsw-alike, q64(d0 + 8 r) = q64(d0) + 8 r for d0 % 64 < 8,
+// lsw(8 j + r) = lsw(8 j) + r.  Twiddles come from per-pass tables
+// (lay1_load_twiddles), read at contiguous or broadcast addresses.
+__device__ __forceinline__ int q64(int i) { return i + ((i >> 6) << 3); }
 
 template <int LAY>
 __device__ __forceinline__ int row_index(int i) {
-  return LAY == 1 ? rsw(i) : lsw(i);
+  return LAY == 1 ? q64(i) : lsw(i);
 }
 
 // Per-pass twiddle table of the LAY = 1 passes (N = 512): the pass at stride
@@ -89,10 +92,11 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
     // Opaque to the optimizer: keeps the R-1 twiddle addresses from being
     // hoisted out of the caller's rate loop, where they would be spilled.
     asm volatile("" : "+v"(k));
-    const c32* x = in + f * P + row_index<LAY>(j);
+    // layout 1: the second pass reads the first pass's lsw output
+    const c32* x = in + f * P + (LAY == 1 && Ns != 8 ? q64(j) : lsw(j));
     c32 v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = x[r * (LAY == 1 ? 70 : NR + NR / 8)];
+    for (int r = 0; r < R; ++r) v[r] = x[r * (NR + NR / 8)];
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -104,18 +108,19 @@ __device__ __forceinline__ void lds_pass_pp(const c32* in, c32* out, int P, int 
     const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
     c32* y = out + f * P;
     if (Ns == 1) {
-      // d0 = j*R is a multiple of 8 (R >= 8): lsw(d0 + r) = lsw(d0) + r + r/8,
-      // rsw(d0 + r) = rsw(d0) + r
-      c32* yy = y + row_index<LAY>(d0);
+      // d0 = j*R is a multiple of 8 (R >= 8): lsw(d0 + r) = lsw(d0) + r + r/8
+      // (both layouts write the first pass at lsw)
+      c32* yy = y + lsw(d0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) yy[LAY == 1 ? r : r + (r >> 3)] = v[r];
+      for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
     } else {
-      // Ns >= 8: the stride Ns keeps the swizzle offset linear
+      // Ns >= 8: the stride Ns keeps the swizzle offset linear (layout 1:
+      // d0 % 64 < 8 for Ns = 8, d0 < 64 for Ns = 64)
       c32* yy = y + row_index<LAY>(d0);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if constexpr (LAY == 1)
-          yy[Ns == 8 ? 8 * r + (r >> 2) : 70 * r] = v[r];  // d0 % 64 < 8 (Ns = 8) or d0 < 64 (Ns = 64)
+          yy[Ns == 8 ? 8 * r : 72 * r] = v[r];
         else
           yy[r * (Ns + (Ns >> 3))] = v[r];
       }
@@ -170,9 +175,9 @@ __device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, con
     int k = j & (Ns - 1);
     asm volatile("" : "+v"(k));  // keeps twiddle addresses out of the caller's loop
     c32* x = rows + f * P;
-    const c32* xr = x + row_index<LAY>(j);
+    const c32* xr = x + (LAY == 1 && Ns != 8 ? q64(j) : lsw(j));
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = xr[r * (LAY == 1 ? 70 : NR + NR / 8)];
+    for (int r = 0; r < R; ++r) v[r] = xr[r * (NR + NR / 8)];
     if (Ns > 1) {
       const int sh = LOGN - logNs - LOGR;
 #pragma unroll
@@ -184,15 +189,16 @@ __device__ __forceinline__ void wave_pass(c32* rows, int P, int nrw, int Ns, con
     dftd<INV, float, R>(v);
     if (!last) {
       const int d0 = ((j >> logNs) << (logNs + LOGR)) + k;
-      c32* yy = x + row_index<LAY>(d0);
       if (Ns == 1) {
+        c32* yy = x + lsw(d0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) yy[LAY == 1 ? r : r + (r >> 3)] = v[r];
+        for (int r = 0; r < R; ++r) yy[r + (r >> 3)] = v[r];
       } else {
+        c32* yy = x + row_index<LAY>(d0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           if constexpr (LAY == 1)
-            yy[Ns == 8 ? 8 * r + (r >> 2) : 70 * r] = v[r];
+            yy[Ns == 8 ? 8 * r : 72 * r] = v[r];
           else
             yy[r * (Ns + (Ns >> 3))] = v[r];
         }
