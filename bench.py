@@ -160,6 +160,30 @@ def valu_calibration() -> int:
     return 0
 
 
+LDS_PATTERNS = ("read_unpadded", "read_pad8", "read_pad16", "read_pad32", "read_q64", "read_stride8_pad8",
+                "write_unpadded", "write_pad8", "write_q64", "write_pass1_pad8", "write_pass1_q64",
+                "write_pass2_q64", "write_pass2_pad8")
+
+
+def lds_calibration() -> int:
+    """Cost of 8-byte LDS accesses in the row layouts and strides of the
+    fused kernels' 512-point FFT passes (zdt_calib_lds), in ns per wavefront
+    access and CU.  Prints one JSON line; under rocprofv3 --pmc
+    SQ_LDS_BANK_CONFLICT the kernels lds_pattern<P> give the conflict cycles
+    of pattern P."""
+    import torch  # noqa: F401  (the HIP runtime the library binds to)
+
+    from fdrt_hip import _lib
+    L = _lib.lib()
+    out = {}
+    for pat, name in enumerate(LDS_PATTERNS):
+        r = C.c_double()
+        _lib.check(L.zdt_calib_lds(pat, C.byref(r)))
+        out[f"{pat}_{name}"] = r.value
+    print(json.dumps({"lds_calibration_ns_per_wave_access_per_cu": out}))
+    return 0
+
+
 def cpu_baseline(gpu_cols: dict, seed: int = 1234) -> tuple:
     """One batch of the FP32 CPU restatement of the reference on one core
     (scipy.fft workers=1: a default single-threaded Julia session runs FFTW
@@ -341,6 +365,8 @@ def main(argv=None) -> int:
                     help="only run a 1 GiB device copy (to scale rocprof byte counters)")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="only measure FP32 VALU issue rates (instruction forms x independent chains)")
+    ap.add_argument("--lds-calibration", action="store_true",
+                    help="only measure 8-byte LDS access costs of the FFT row layouts and strides")
     ap.add_argument("--ragged", action="store_true", help="the ragged multi-channel plan workload instead")
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
@@ -357,6 +383,8 @@ def main(argv=None) -> int:
         return copy_calibration()
     if a.valu_calibration:
         return valu_calibration()
+    if a.lds_calibration:
+        return lds_calibration()
     if a.long:
         if a.gpus != 1:
             print("bench.py measures one GPU (N=1 only)", file=sys.stderr)

@@ -88,6 +88,7 @@ SIGNATURES = {
     "zdt_findprotohits": (C.c_int, [C.POINTER(_vp), C.c_int, _i64, _i64, _i64, C.c_double, _vp, _i64, _i64p,
                                     _vp]),
     "zdt_calib_valu": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "zdt_calib_lds": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "zdt_plan_tuning": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "zdt_tune_table": (C.c_int, [C.c_char_p]),
     "zdt_findprotohits_noalloc": (C.c_int, [C.POINTER(_vp), C.c_int, _i64, _i64, _i64, C.c_double, _vp, _i64,

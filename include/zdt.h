@@ -204,6 +204,14 @@ int zdt_profile_read(zdt_ws* ws, int idx, int* count, char* name, int name_len,
  * for the compute roofline of bench.py (bench.py --valu-calibration). */
 int zdt_calib_valu(int form, int chains, double* lane_ops_per_s);
 
+/* Cost of 8-byte LDS accesses of one pattern (0..12: reads 0-5, writes 6-12;
+ * the row layouts and strides of the fused kernels' 512-point FFT passes,
+ * listed in csrc/zdt_calib.hip), in nanoseconds per wavefront access and
+ * CU over a full-device launch.  No counterpart in the reference: a
+ * measurement behind the fused kernels' LDS layout (bench.py
+ * --lds-calibration). */
+int zdt_calib_lds(int pattern, double* ns_per_access);
+
 /* ---------------------------------------------------------- statistics */
 
 /* fdrstats(fdr) / fdrstats(fdrs)  -- src/fdrutils.jl:39-50.
