@@ -36,8 +36,10 @@ def F(gpu):
 
 
 @pytest.mark.parametrize("Nf", [1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20])
-@pytest.mark.parametrize("Nt", [16, 7, 1])
+@pytest.mark.parametrize("Nt", [16, 7, 3, 1])
 def test_fast_sizes(F, Nf, Nt):
+    """Every k2 time-step template (Nt padded to 16, 8, 4, 2) at every N1;
+    N1 = 512 (2^20 channels) runs the q64 row layout."""
     Nr = 33 if Nf >= 1 << 19 else 65
     s = synth.spectrogram(Nf, Nt, [(Nf * 0.31, 1.5, 25.0), (Nf * 0.77, -2.75, 40.0)], seed=Nf % 1000 + Nt)
     say(f"fast Nf={Nf} Nt={Nt} Nr={Nr}")
