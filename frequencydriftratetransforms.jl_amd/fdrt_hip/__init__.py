@@ -13,6 +13,7 @@ from .fdrutils import (create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_, 
                        findprotohits)
 from .fftfdr import (FftFdrWorkspace, fdshift, fdshift_, fdshiftsum, fdshiftsum_, fftfdr, fftfdr_,
                      fftfdr_workspace, fftfdr_workspace_)
+from .filterbank import FilterbankError, FilterbankFile, read_spectrogram, write_filterbank
 from .plan import RaggedPlan, set_tune_table
 from .stream import StreamSearch, pinned_spectrogram
 from .zdt import (ZDTWorkspace, convolve_, fdrsynchronize, input_, output_, postprocess_,
@@ -27,4 +28,5 @@ __all__ = [
     "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "ZDTError", "lib",
     "RaggedPlan", "set_tune_table", "FftFdrWorkspace", "fftfdr_workspace", "fftfdr_workspace_", "fftfdr", "fftfdr_",
     "fdshift", "fdshift_", "fdshiftsum", "fdshiftsum_", "StreamSearch", "pinned_spectrogram",
+    "FilterbankFile", "FilterbankError", "read_spectrogram", "write_filterbank",
 ]
