@@ -331,6 +331,7 @@ int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, doub
   const int64_t nt = hits_tiles(Nf, Nr);
   if (nt == 0) return ZDT_OK;
   const int vec = (ld == Nf && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
+  ProfScope ps(st, "hits_count", 4.0 * Nf * Nr + 8.0 * nt);
   hits_count<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev, counts, vec);
   ZDT_CHECK_LAUNCH("hits_count");
   return ZDT_OK;
@@ -338,6 +339,7 @@ int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, doub
 
 int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, int64_t* scratch, hipStream_t st) {
   const int64_t nb = hits_scan_scratch(n);
+  ProfScope ps(st, "hits_scan", 24.0 * n);
   if (nb > 0) {
     hits_scan_local<<<static_cast<unsigned>(nb), SCAN_THREADS, 0, st>>>(counts, n, scratch);
     ZDT_CHECK_LAUNCH("hits_scan_local");
@@ -360,6 +362,7 @@ int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld,
   // offsets of this matrix's tiles start at tile0 of the global scan; the end
   // of its last tile is the next matrix's first offset (or the total)
   const int vec = (ld == Nf && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
+  ProfScope ps(st, "hits_write", 4.0 * Nf * Nr + 8.0 * nt);
   hits_write<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev,
                                                                 offsets + tile0, ntiles_all - tile0, total_dev,
                                                                 base_index, pos_base, hits, cap, vec);
@@ -413,6 +416,7 @@ int launch_set_double(double* p, double v, hipStream_t st) {
 
 int launch_hits_threshold(const double* colmean, const double* colvar, int64_t Nr, double snr, double* thr,
                           hipStream_t st) {
+  ProfScope ps(st, "hits_threshold", 16.0 * Nr);
   hits_threshold<<<1, 64, 0, st>>>(colmean, colvar, Nr, snr, thr);
   ZDT_CHECK_LAUNCH("hits_threshold");
   return ZDT_OK;

@@ -171,6 +171,8 @@ struct zdt_ws {
   int precision = ZDT_PRECISION_FP32;
 };
 
+Profiler* zdt::ws_profiler(const zdt_ws* ws) { return ws ? ws->prof : nullptr; }
+
 namespace {
 
 // installs the workspace's profiler (if any) for the duration of an entry point

@@ -152,4 +152,14 @@ struct ProfScope {
   }
 };
 
+// the profiler of a workspace (null without ZDT_FLAG_PROFILE), and a scope
+// that installs a profiler for the calling thread (launch sites outside the
+// workspace entry points: the long-integration search's hit extraction)
+Profiler* ws_profiler(const zdt_ws* ws);
+struct ProfUse {
+  Profiler* prev;
+  explicit ProfUse(Profiler* p) : prev(g_prof) { g_prof = p; }
+  ~ProfUse() { g_prof = prev; }
+};
+
 }  // namespace zdt

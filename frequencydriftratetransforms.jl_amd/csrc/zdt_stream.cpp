@@ -129,7 +129,8 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
   if (Nf < 2 || Nt < 1 || Nr < 1 || Nrb < 0 || hbm_bytes < 0)
     return fail(ZDT_EINVAL, "stream: bad shape Nf=%lld Nt=%lld Nr=%lld Nrb=%lld", (long long)Nf, (long long)Nt,
                 (long long)Nr, (long long)Nrb);
-  if (flags & ~(ZDT_FLAG_UNNORMALIZED | ZDT_FLAG_GENERIC)) return fail(ZDT_EINVAL, "stream: unsupported flags");
+  if (flags & ~(ZDT_FLAG_UNNORMALIZED | ZDT_FLAG_GENERIC | ZDT_FLAG_PROFILE))
+    return fail(ZDT_EINVAL, "stream: unsupported flags");
   if (!std::isfinite(r0) || !std::isfinite(dr)) return fail(ZDT_EINVAL, "stream: r0 and dr must be finite");
   const int64_t dcap = int64_t(1) << 20;  // initial device hit staging per spectrogram (grows)
   int64_t budget = hbm_bytes;
@@ -222,6 +223,12 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
   return ZDT_OK;
 }
 
+int zdt_stream_workspace(const zdt_stream* s, zdt_ws** ws) {
+  if (s == nullptr || ws == nullptr) return fail(ZDT_EINVAL, "stream_workspace: NULL argument");
+  *ws = s->ws;
+  return ZDT_OK;
+}
+
 int zdt_stream_info(const zdt_stream* s, int64_t* Nrb, int64_t* nbatch, int* resident, int64_t* bytes) {
   if (s == nullptr) return fail(ZDT_EINVAL, "stream is NULL");
   if (Nrb) *Nrb = s->Nrb;
@@ -288,6 +295,7 @@ int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, 
       float* f = s->fdr[s->resident ? b : 0];
       ZDT_TRY(run_devstats(s->ws, r0b(b), f, colmean + b * Nrb, colvar + b * Nrb, scratch));
     }
+    ProfUse prof_(ws_profiler(s->ws));  // the hit extraction joins the workspace's kernel profile
     ZDT_TRY(launch_hits_threshold(colmean, colvar, s->Nr, snr, thr_dev, s->compute));
     // hits in column-major order of the hcat'd batches
     for (int64_t b = 0; b < nb; ++b) {

@@ -105,6 +105,7 @@ SIGNATURES = {
     "zdt_host_free": (C.c_int, [_vp]),
     "zdt_stream_create": (C.c_int, [_i64, _i64, C.c_float, C.c_float, _i64, _i64, _i64, C.c_int, C.c_int,
                                     C.POINTER(_vp)]),
+    "zdt_stream_workspace": (C.c_int, [_vp, C.POINTER(_vp)]),
     "zdt_stream_destroy": (C.c_int, [_vp]),
     "zdt_stream_info": (C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), _i64p]),
     "zdt_stream_search": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_double, _vp, _i64, _vp, _vp]),

@@ -35,11 +35,12 @@ class StreamSearch:
     hcat'd batches, i.e. rate index + 1) and the thresholds used.  When Nr is
     not a multiple of the batch length the last batch computes rates past
     r0 + (Nr-1)*dr, but only the Nr requested columns enter the statistics
-    and the hits.
+    and the hits.  ``profile=True`` times every kernel of the searches (HIP
+    events on the search's stream; ``kernel_times``, ``reset_profile``).
     """
 
     def __init__(self, Nf: int, Nt: int, rates, *, Nrb: int = 0, hbm_bytes: int = 0, generic: bool = False,
-                 unnormalized: bool = False, precision: str = "fp32"):
+                 unnormalized: bool = False, precision: str = "fp32", profile: bool = False):
         if isinstance(rates, tuple) and len(rates) == 3:
             r0, dr, Nr = float32_of(rates[0]), float32_of(rates[1]), int(rates[2])
         else:
@@ -47,6 +48,7 @@ class StreamSearch:
         self.Nf, self.Nt, self.Nr = int(Nf), int(Nt), int(Nr)
         self.r0, self.dr = np.float32(r0), np.float32(dr)
         flags = (_lib.ZDT_FLAG_GENERIC if generic else 0) | (_lib.ZDT_FLAG_UNNORMALIZED if unnormalized else 0)
+        flags |= _lib.ZDT_FLAG_PROFILE if profile else 0
         if precision not in _lib.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
         self.precision = precision
@@ -58,6 +60,36 @@ class StreamSearch:
         nrb, nb, res, by = C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()
         _lib.check(_lib.lib().zdt_stream_info(self._h, C.byref(nrb), C.byref(nb), C.byref(res), C.byref(by)))
         self.Nrb, self.nbatch, self.resident, self.nbytes = nrb.value, nb.value, bool(res.value), by.value
+
+    def _workspace(self) -> C.c_void_p:
+        ws = C.c_void_p()
+        _lib.check(_lib.lib().zdt_stream_workspace(self._h, C.byref(ws)))
+        return ws
+
+    @property
+    def path(self) -> str:
+        """The kernel path of the search's workspace (as ZDTWorkspace.path)."""
+        p = C.c_int()
+        _lib.check(_lib.lib().zdt_path(self._workspace(), C.byref(p), None))
+        return _lib.PATHS[p.value]
+
+    def reset_profile(self) -> None:
+        _lib.check(_lib.lib().zdt_profile_reset(self._workspace()))
+
+    def kernel_times(self) -> List[Tuple[str, float, int]]:
+        """(kernel, total ms, launches) over the searches since the last
+        reset_profile (profile=True), longest first."""
+        L = _lib.lib()
+        ws = self._workspace()
+        cnt = C.c_int()
+        _lib.check(L.zdt_profile_read(ws, -1, C.byref(cnt), None, 0, None, None, None))
+        out = []
+        for i in range(cnt.value):
+            name = C.create_string_buffer(64)
+            ms, nl, by = C.c_double(), C.c_int64(), C.c_double()
+            _lib.check(L.zdt_profile_read(ws, i, None, name, 64, C.byref(ms), C.byref(nl), C.byref(by)))
+            out.append((name.value.decode(), ms.value, nl.value))
+        return sorted(out, key=lambda x: -x[1])
 
     def close(self) -> None:
         if self._h is not None:

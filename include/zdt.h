@@ -341,13 +341,20 @@ int zdt_host_free(void* p);
  * resident within hbm_bytes, else the largest that fits with one FDR buffer
  * (each batch is then computed twice: once for the statistics, once for the
  * hits).  hbm_bytes = 0 means 90% of the free device memory.
- * flags: ZDT_FLAG_UNNORMALIZED, ZDT_FLAG_GENERIC.  precision: ZDT_PRECISION_*
- * of the search's workspace (as zdt_create). */
+ * flags: ZDT_FLAG_UNNORMALIZED, ZDT_FLAG_GENERIC, ZDT_FLAG_PROFILE (every
+ * kernel of a search, the hit extraction included, is timed on the search's
+ * workspace; see zdt_stream_workspace).  precision: ZDT_PRECISION_* of the
+ * search's workspace (as zdt_create). */
 int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr,
                       int64_t Nrb, int64_t hbm_bytes, int flags, int precision, zdt_stream** out);
 int zdt_stream_destroy(zdt_stream* s);
 int zdt_stream_info(const zdt_stream* s, int64_t* Nrb, int64_t* nbatch, int* resident,
                     int64_t* bytes);
+
+/* The search's workspace, owned by the stream: zdt_path, zdt_info and, with
+ * ZDT_FLAG_PROFILE, zdt_profile_reset / zdt_profile_read on it.  No
+ * counterpart in the reference. */
+int zdt_stream_workspace(const zdt_stream* s, zdt_ws** ws);
 
 /* Searches nspec HOST spectrograms (Nf x Nt Float32, column-major).  For
  * spectrogram i: thr[i] = snr*s + m over all its batches (may be NULL),

@@ -151,6 +151,33 @@ def test_stream_split_precision(F, Nf, Nt, precision):
         F.StreamSearch(Nf, Nt, (r0, dr, Nr), precision="bf16")
 
 
+def test_stream_profile(F):
+    """profile=True: every kernel of a search is timed on the search's
+    workspace, the hit extraction included; the same thresholds and hits as
+    an unprofiled search."""
+    specs = specs3()[:2]
+    st = F.StreamSearch(NF, NT, (R0, DR, NR), Nrb=10, profile=True)
+    assert st.path == "fused"
+    h0, t0 = st.search(specs, snr=8.0)
+    st.reset_profile()
+    h1, t1 = st.search(specs, snr=8.0)
+    kt = st.kernel_times()
+    st.close()
+    names = {k for k, _, _ in kt}
+    say(f"stream profile: {[(k, round(ms, 3), n) for k, ms, n in kt]}")
+    assert {"k1_input", "k2_czt", "k3_output", "hits_threshold", "hits_count", "hits_write"} <= names
+    launches = dict((k, n) for k, _, n in kt)
+    assert launches["k1_input"] == 2 and launches["k3_output"] == 2 * 4 and launches["hits_count"] == 2 * 4
+    assert all(ms > 0 for _, ms, _ in kt)
+    assert np.array_equal(t0, t1) and all(np.array_equal(a, b) for a, b in zip(h0, h1))
+    plain = F.StreamSearch(NF, NT, (R0, DR, NR), Nrb=10)
+    h2, t2 = plain.search(specs, snr=8.0)
+    plain.close()
+    assert np.array_equal(t1, t2) and all(np.array_equal(a, b) for a, b in zip(h1, h2))
+    with pytest.raises(F.ZDTError):
+        F.StreamSearch(NF, NT, (R0, DR, NR), Nrb=10).reset_profile()
+
+
 def long_spectrogram(seed):
     """One long-integration spectrogram: 2^22 channels x 64 steps,
     Exponential(1) noise + 64 tones, amplitudes log-uniform 5..100, rates
