@@ -74,8 +74,13 @@ def step_hbm(Nf: int, Nt: int, Nr: int, dt: float) -> dict:
         Nl += 1
     b = 4 * Nf * Nt + 16 * Nk * Nt + 8 * Nk * Nl + 16 * Nk * Nr + 4 * Nf * Nr
     gbs = b / dt / 1e9
+    # this path's own traffic: spectrogram in and G out (k1), G in and B out
+    # (k2), B in and the FDR out (k3), G and B being (Nf/2) x (Nt or Nr) complex
+    moved = 12 * Nf * Nt + 12 * Nf * Nr
     return {"basis": "B_alg (reference dataflow)", "Nl": Nl, "bytes_per_step": b, "achieved": gbs,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "moved": {"basis": "bytes the fused kernels move (k1 + k2 + k3)", "bytes_per_step": moved,
+                      "achieved": moved / dt / 1e9, "frac": moved / dt / 1e9 / HBM_PEAK_GBS}}
 
 
 def _smooth(n: int) -> bool:
