@@ -1,0 +1,369 @@
+// Proto-hit clustering: connected components of the proto-hit cells of an FDR.
+//
+// findprotohits (src/fdrutils.jl:131-169) returns every cell at or above the
+// threshold; its docstring calls them proto-hits because "no clustering of
+// neighboring proto-hits has been performed", and README.md:48-53 names that
+// clustering as the next step of a search.  Here two proto-hits are neighbours
+// when their channels differ by at most di and their (hcat'd) rate columns by
+// at most dj, and a hit is one connected component of that neighbour graph.
+//
+// Input: the proto-hits as zdt_findprotohits writes them, 0-based linear
+// indices h = i + Nf*J in strictly ascending order.  Sorted input turns every
+// neighbour query into a contiguous range: the same column's neighbours are
+// the next entries, and column J+d's are one binary search away.
+//
+// Union-find over proto-hit ordinals, after the ECL-CC scheme (Jaiganesh and
+// Burtscher, "A High-Performance Connected Components Implementation for
+// GPUs", HPDC 2018): a root is only ever hooked under a smaller root with one
+// compare-and-swap, and finds shorten the path they walk.  So parent[x] <= x
+// always, every walk descends, and each set's root is its smallest ordinal --
+// its first proto-hit in column-major order.  Clusters are numbered by a scan
+// over the roots, which orders them by first proto-hit, and their size,
+// bounding box and peak are reduced with atomics that commute (add, min, max),
+// so every output is independent of scheduling.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "zdt_internal.h"
+#include "zdt_kernels.h"
+
+namespace zdt {
+
+namespace {
+
+constexpr int CL_THREADS = 256;
+constexpr int CL_PER_THREAD = 16;
+constexpr int CL_TILE = CL_THREADS * CL_PER_THREAD;  // proto-hits per tile of the root count
+
+__device__ __forceinline__ int ld_parent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_parent(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x.  Each visited entry is pointed at its grandparent, an ancestor of
+// it in the same set and smaller, so concurrent finds and hooks keep
+// parent[y] <= y.  A root's entry is never written here (only hooks do that).
+__device__ int uf_find(int* parent, int x) {
+  int cur = ld_parent(parent + x);
+  if (cur == x) return x;
+  int prev = x;
+  for (;;) {
+    const int next = ld_parent(parent + cur);
+    if (next == cur) return cur;
+    st_parent(parent + prev, next);
+    prev = cur;
+    cur = next;
+  }
+}
+
+// Merge the sets of a and b: hook the larger root under the smaller.  A failed
+// compare-and-swap means another thread hooked that root first; both roots are
+// found again and the attempt repeats.  Every failure is some thread's
+// success, so the loop ends.
+__device__ void uf_union(int* parent, int a, int b) {
+  int ra = uf_find(parent, a), rb = uf_find(parent, b);
+  while (ra != rb) {
+    if (ra > rb) {
+      const int t = ra;
+      ra = rb;
+      rb = t;
+    }
+    const int old = atomicCAS(parent + rb, rb, ra);
+    if (old == rb) return;
+    ra = uf_find(parent, ra);
+    rb = uf_find(parent, old);
+  }
+}
+
+struct LinkArgs {
+  const int64_t* hits;
+  int64_t n;
+  int64_t Nf;
+  int64_t ncol;  // columns of the hcat'd FDR: nb * Nr
+  int di, dj;
+  int* parent;
+  int* err;
+};
+
+__global__ __launch_bounds__(CL_THREADS) void cl_init(int* __restrict__ parent, int64_t n) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
+  if (k < n) parent[k] = static_cast<int>(k);
+}
+
+// One thread per proto-hit k: union with every neighbour after it in the sorted
+// order (the neighbours before it do the same towards k).  A proto-hit out of
+// range or out of order sets *err and links nothing; every read stays in
+// hits[0, n) either way.
+__global__ __launch_bounds__(CL_THREADS) void cl_link(LinkArgs a) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t h = a.hits[k];
+  if (h < 0 || h >= a.Nf * a.ncol || (k > 0 && a.hits[k - 1] >= h)) {
+    atomicOr(a.err, 1);
+    return;
+  }
+  const int64_t J = h / a.Nf, i = h - J * a.Nf;
+  // the same column: channels i+1 .. i+di are the next entries
+  const int64_t colend = (J + 1) * a.Nf;
+  for (int64_t m = k + 1; m < a.n; ++m) {
+    const int64_t g = a.hits[m];
+    if (g > h + a.di || g >= colend) break;
+    uf_union(a.parent, static_cast<int>(k), static_cast<int>(m));
+  }
+  // columns J+1 .. J+dj: channels max(0, i-di) .. min(Nf-1, i+di), one
+  // contiguous run of the sorted list starting at a lower bound
+  const int64_t ilo = i > a.di ? i - a.di : 0;
+  const int64_t ihi = i + a.di < a.Nf - 1 ? i + a.di : a.Nf - 1;
+  int64_t from = k + 1;
+  for (int d = 1; d <= a.dj && J + d < a.ncol; ++d) {
+    const int64_t base = (J + d) * a.Nf;
+    const int64_t lo = base + ilo, hi = base + ihi;
+    int64_t L = from, R = a.n;
+    while (L < R) {
+      const int64_t mid = (L + R) >> 1;
+      if (a.hits[mid] < lo) L = mid + 1;
+      else R = mid;
+    }
+    int64_t m = L;
+    for (; m < a.n && a.hits[m] <= hi; ++m) uf_union(a.parent, static_cast<int>(k), static_cast<int>(m));
+    from = m;  // the next column's run starts after this one
+  }
+}
+
+// After linking: every entry points straight at its root.
+__global__ __launch_bounds__(CL_THREADS) void cl_flatten(int* parent, int64_t n) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
+  if (k >= n) return;
+  int r = ld_parent(parent + k);
+  for (int next = ld_parent(parent + r); next != r; next = ld_parent(parent + r)) r = next;
+  st_parent(parent + k, r);
+}
+
+__device__ __forceinline__ int wave_incl_scan_i(int v, int lane) {
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// Roots (parent[k] == k) per tile of CL_TILE proto-hits.
+__global__ __launch_bounds__(CL_THREADS) void cl_count_roots(const int* __restrict__ parent, int64_t n,
+                                                            int64_t* __restrict__ counts) {
+  __shared__ int sh[CL_THREADS / 64];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * CL_TILE + threadIdx.x * CL_PER_THREAD;
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < CL_PER_THREAD; ++q) {
+    const int64_t k = base + q;
+    s += (k < n && parent[k] == static_cast<int>(k)) ? 1 : 0;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int incl = wave_incl_scan_i(s, lane);
+  if (lane == 63) sh[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < CL_THREADS / 64; ++w) t += sh[w];
+    counts[blockIdx.x] = t;
+  }
+}
+
+struct NumberArgs {
+  const int64_t* hits;
+  int64_t n;
+  int64_t Nf;
+  const int* parent;
+  const int64_t* offs;  // exclusive scan of the per-tile root counts
+  int* cid;             // cluster ordinal, written at each root's entry
+  int64_t cap;
+  unsigned long long* key;  // per cluster: peak key, cleared here
+  int64_t* size;
+  int64_t* box;  // per cluster: channel min, channel max, column min, column max
+};
+
+// Number the roots in order and clear the accumulators of the clusters that
+// fit in cap.  A root is its cluster's first proto-hit, so the column minimum
+// is the root's column.
+__global__ __launch_bounds__(CL_THREADS) void cl_number_roots(NumberArgs a) {
+  __shared__ int sh[CL_THREADS / 64 + 1];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * CL_TILE + threadIdx.x * CL_PER_THREAD;
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < CL_PER_THREAD; ++q) {
+    const int64_t k = base + q;
+    s += (k < a.n && a.parent[k] == static_cast<int>(k)) ? 1 : 0;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int incl = wave_incl_scan_i(s, lane);
+  if (lane == 63) sh[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < CL_THREADS / 64; ++w) {
+      const int v = sh[w];
+      sh[w] = t;
+      t += v;
+    }
+  }
+  __syncthreads();
+  int64_t c = a.offs[blockIdx.x] + sh[wv] + incl - s;
+  for (int q = 0; q < CL_PER_THREAD; ++q) {
+    const int64_t k = base + q;
+    if (k >= a.n || a.parent[k] != static_cast<int>(k)) continue;
+    a.cid[k] = static_cast<int>(c);
+    if (c < a.cap) {
+      a.key[c] = 0ull;
+      a.size[c] = 0;
+      a.box[4 * c + 0] = INT64_MAX;
+      a.box[4 * c + 1] = 0;
+      a.box[4 * c + 2] = a.hits[k] / a.Nf;
+      a.box[4 * c + 3] = 0;
+    }
+    ++c;
+  }
+}
+
+struct AccumArgs {
+  const int64_t* hits;
+  int64_t n;
+  int64_t Nf, Nr, ld;
+  int64_t ncol;
+  const float* const* fdrs;  // device table of the nb matrices
+  const int* parent;
+  const int* cid;
+  int64_t cap;
+  unsigned long long* key;
+  int64_t* size;
+  int64_t* box;
+  int64_t* label;  // per proto-hit cluster ordinal, or null
+};
+
+// Order-preserving map of a float onto an unsigned integer.
+__device__ __forceinline__ unsigned ordered_bits(float v) {
+  const unsigned u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unordered_bits(unsigned o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+__global__ __launch_bounds__(CL_THREADS) void cl_accumulate(AccumArgs a) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t h = a.hits[k];
+  if (h < 0 || h >= a.Nf * a.ncol) return;  // flagged by cl_link
+  const int c = a.cid[a.parent[k]];
+  if (a.label) a.label[k] = c;
+  if (c >= a.cap) return;
+  const int64_t J = h / a.Nf, i = h - J * a.Nf;
+  const int64_t b = J / a.Nr, j = J - b * a.Nr;
+  const float v = a.fdrs[b][i + a.ld * j];
+  // peak: the largest value, ties to the smaller ordinal (the earlier proto-hit)
+  const unsigned long long key =
+      (static_cast<unsigned long long>(ordered_bits(v)) << 32) | (0xffffffffull - static_cast<unsigned long long>(k));
+  atomicMax(a.key + c, key);
+  atomicAdd(reinterpret_cast<unsigned long long*>(a.size + c), 1ull);
+  atomicMin(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 0), static_cast<unsigned long long>(i));
+  atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 1), static_cast<unsigned long long>(i));
+  atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 3), static_cast<unsigned long long>(J));
+}
+
+__global__ __launch_bounds__(CL_THREADS) void cl_finish(const int64_t* __restrict__ hits,
+                                                       const unsigned long long* __restrict__ key, int64_t ncl,
+                                                       int64_t* __restrict__ peak, float* __restrict__ value) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
+  if (c >= ncl) return;
+  const unsigned long long kk = key[c];
+  const int64_t k = static_cast<int64_t>(0xffffffffull - (kk & 0xffffffffull));
+  peak[c] = hits[k];
+  value[c] = unordered_bits(static_cast<unsigned>(kk >> 32));
+}
+
+unsigned blocks_for(int64_t n, int64_t per) { return static_cast<unsigned>((n + per - 1) / per); }
+
+}  // namespace
+
+int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* fdrs, int nb, int64_t Nf,
+                           int64_t Nr, int64_t ld, int di, int dj, int64_t* peak, float* value, int64_t* size,
+                           int64_t* box, int64_t cap, int64_t* nclusters, int64_t* label, hipStream_t st) {
+  if (n == 0) {
+    *nclusters = 0;
+    return ZDT_OK;
+  }
+  const int64_t nt = (n + CL_TILE - 1) / CL_TILE;
+  const int64_t nscan = hits_scan_scratch(nt);
+  // one scratch block: parent[n], cid[n] (int), then the int64 words
+  // counts[nt], scan scratch, total, the error flag, and the matrix table
+  auto up = [](int64_t bytes) { return (bytes + 255) & ~int64_t(255); };
+  const int64_t o_cid = up(4 * n), o_w = o_cid + up(4 * n);
+  const int64_t o_total = o_w + 8 * (nt + nscan), o_tab = o_total + 16;
+  const int64_t bytes = o_tab + 8 * static_cast<int64_t>(nb);
+  char* buf = nullptr;
+  ZDT_TRY_HIP(hipMallocAsync(reinterpret_cast<void**>(&buf), static_cast<size_t>(bytes), st));
+  int* parent = reinterpret_cast<int*>(buf);
+  int* cid = reinterpret_cast<int*>(buf + o_cid);
+  int64_t* counts = reinterpret_cast<int64_t*>(buf + o_w);
+  int64_t* total_dev = reinterpret_cast<int64_t*>(buf + o_total);  // [0] root count, [1] error flag
+  const float** tab = reinterpret_cast<const float**>(buf + o_tab);
+  unsigned long long* key = nullptr;
+  int64_t flags[2] = {0, 0};
+  int rc = ZDT_OK;
+  auto run = [&]() -> int {
+    ZDT_TRY_HIP(hipMemsetAsync(total_dev, 0, 16, st));
+    ZDT_TRY_HIP(hipMemcpyAsync(tab, fdrs, sizeof(float*) * nb, hipMemcpyHostToDevice, st));
+    int* err = reinterpret_cast<int*>(total_dev + 1);
+    {
+      ProfScope ps(st, "cluster_link", 20.0 * n);
+      cl_init<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(parent, n);
+      ZDT_CHECK_LAUNCH("cl_init");
+      LinkArgs a{hits, n, Nf, nb * Nr, di, dj, parent, err};
+      cl_link<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
+      ZDT_CHECK_LAUNCH("cl_link");
+      cl_flatten<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(parent, n);
+      ZDT_CHECK_LAUNCH("cl_flatten");
+    }
+    {
+      ProfScope ps(st, "cluster_roots", 4.0 * n);
+      cl_count_roots<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(parent, n, counts);
+      ZDT_CHECK_LAUNCH("cl_count_roots");
+    }
+    ZDT_TRY(launch_hits_scan(counts, nt, total_dev, counts + nt, st));
+    ZDT_TRY_HIP(hipMemcpyAsync(flags, total_dev, 16, hipMemcpyDeviceToHost, st));
+    ZDT_TRY_HIP(hipStreamSynchronize(st));
+    if (static_cast<int>(flags[1]) != 0)
+      return fail(ZDT_EINVAL, "clusterprotohits: proto-hits must be strictly ascending linear indices below Nf*nb*Nr");
+    const int64_t ncl = flags[0] < cap ? flags[0] : cap;
+    if (ncl > 0) ZDT_TRY_HIP(hipMallocAsync(reinterpret_cast<void**>(&key), 8 * static_cast<size_t>(ncl), st));
+    {
+      ProfScope ps(st, "cluster_number", 12.0 * n);
+      NumberArgs a{hits, n, Nf, parent, counts, cid, ncl, key, size, box};
+      cl_number_roots<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(a);
+      ZDT_CHECK_LAUNCH("cl_number_roots");
+    }
+    if (ncl > 0 || label != nullptr) {
+      ProfScope ps(st, "cluster_reduce", 28.0 * n);
+      AccumArgs a{hits, n, Nf, Nr, ld, nb * Nr, tab, parent, cid, ncl, key, size, box, label};
+      cl_accumulate<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
+      ZDT_CHECK_LAUNCH("cl_accumulate");
+    }
+    if (ncl > 0) {
+      cl_finish<<<blocks_for(ncl, CL_THREADS), CL_THREADS, 0, st>>>(hits, key, ncl, peak, value);
+      ZDT_CHECK_LAUNCH("cl_finish");
+    }
+    return ZDT_OK;
+  };
+  rc = run();
+  if (key) (void)hipFreeAsync(key, st);
+  (void)hipFreeAsync(buf, st);
+  if (rc != ZDT_OK) return rc;
+  *nclusters = flags[0];
+  if (flags[0] > cap)
+    return fail(ZDT_ETRUNC, "clusterprotohits: %lld hits exceed capacity %lld", (long long)flags[0], (long long)cap);
+  return ZDT_OK;
+}
+
+}  // namespace zdt

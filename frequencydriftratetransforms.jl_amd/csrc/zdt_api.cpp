@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -899,6 +900,20 @@ int zdt_findprotohits(const float* const* fdrs, int nb, int64_t Nf, int64_t Nr, 
   if (total > cap) return fail(ZDT_ETRUNC, "findprotohits: %lld hits exceed capacity %lld", (long long)total,
                                (long long)cap);
   return ZDT_OK;
+}
+
+int zdt_clusterprotohits(const int64_t* hits, int64_t n, const float* const* fdrs, int nb, int64_t Nf,
+                         int64_t Nr, int64_t ld, int di, int dj, int64_t* peak, float* value, int64_t* size,
+                         int64_t* box, int64_t cap, int64_t* nclusters, int64_t* label, void* stream) {
+  ZDT_TRY(check_fdrs(reinterpret_cast<const void* const*>(fdrs), nb, Nf, Nr, ld));
+  if (n < 0 || (n > 0 && hits == nullptr)) return fail(ZDT_EINVAL, "clusterprotohits: bad proto-hit list");
+  if (n > INT32_MAX) return fail(ZDT_ENOTSUP, "clusterprotohits: %lld proto-hits exceed 2^31-1", (long long)n);
+  if (di < 0 || dj < 0) return fail(ZDT_EINVAL, "clusterprotohits: negative neighbourhood (%d, %d)", di, dj);
+  if (nclusters == nullptr) return fail(ZDT_EINVAL, "clusterprotohits: nclusters is NULL");
+  if (cap < 0 || (cap > 0 && (peak == nullptr || value == nullptr || size == nullptr || box == nullptr)))
+    return fail(ZDT_EINVAL, "clusterprotohits: bad output buffers");
+  return cluster_protohits_impl(hits, n, fdrs, nb, Nf, Nr, ld, di, dj, peak, value, size, box, cap, nclusters,
+                                label, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -70,4 +70,14 @@ int launch_set_double(double* p, double v, hipStream_t st);
 int launch_hits_threshold(const double* colmean, const double* colvar, int64_t Nr, double snr, double* thr,
                           hipStream_t st);
 
+// Connected components of proto-hits (hit_cluster.hip): hits are n strictly
+// ascending linear indices into the hcat of nb Nf x Nr matrices (column stride
+// ld); fdrs is the HOST array of the nb device pointers.  Writes the first cap
+// clusters' peak index, peak value, size and box (4 per cluster), and, when
+// label is not null, every proto-hit's cluster ordinal; *nclusters = count.
+// Synchronizes st once.  ZDT_ETRUNC when the count exceeds cap.
+int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* fdrs, int nb, int64_t Nf,
+                           int64_t Nr, int64_t ld, int di, int dj, int64_t* peak, float* value, int64_t* size,
+                           int64_t* box, int64_t cap, int64_t* nclusters, int64_t* label, hipStream_t st);
+
 }  // namespace zdt

@@ -9,8 +9,8 @@ extension of FrequencyDriftRateTransforms.
 import torch  # noqa: F401  (loads the HIP runtime libzdt_hip.so binds to)
 
 from ._lib import ZDTError, lib
-from .fdrutils import (create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_, fdrstats,
-                       findprotohits)
+from .fdrutils import (Hits, clusterprotohits, create_fdr, fdrdenormalize, fdrnormalize, fdrnormalize_,
+                       fdrstats, findhits, findprotohits)
 from .fftfdr import (FftFdrWorkspace, fdshift, fdshift_, fdshiftsum, fdshiftsum_, fftfdr, fftfdr_,
                      fftfdr_workspace, fftfdr_workspace_)
 from .filterbank import FilterbankError, FilterbankFile, read_spectrogram, write_filterbank
@@ -25,7 +25,7 @@ __all__ = [
     "ZDTWorkspace", "input_", "output_", "preprocess_", "convolve_", "postprocess_",
     "zdtfdr", "zdtfdr_", "calcNl", "growNr", "estimate_memory", "driftrates", "batchrates",
     "RateRange", "workspace_bytes", "create_fdr", "fdrstats", "fdrsynchronize",
-    "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "ZDTError", "lib",
+    "fdrnormalize_", "fdrnormalize", "fdrdenormalize", "findprotohits", "clusterprotohits", "findhits", "Hits", "ZDTError", "lib",
     "RaggedPlan", "set_tune_table", "FftFdrWorkspace", "fftfdr_workspace", "fftfdr_workspace_", "fftfdr", "fftfdr_",
     "fdshift", "fdshift_", "fdshiftsum", "fdshiftsum_", "StreamSearch", "pinned_spectrogram",
     "FilterbankFile", "FilterbankError", "read_spectrogram", "write_filterbank",

@@ -252,6 +252,32 @@ int zdt_findprotohits_noalloc(const float* const* fdrs, int nb, int64_t Nf, int6
                               int64_t ld, double threshold, int64_t* hits, int64_t cap,
                               int64_t* nhits, void* stream);
 
+/* Clustering of neighbouring proto-hits into hits.  The reference stops at
+ * proto-hits: findprotohits' docstring (src/fdrutils.jl:141-144) and
+ * README.md:48-53 leave "clustering of neighboring proto-hits" to the next
+ * stage of a search, outside the package.  Two proto-hits are neighbours when
+ * their channels differ by at most di and their hcat'd rate columns by at most
+ * dj (di = dj = 1: the 8 surrounding cells); a hit is one connected component.
+ *
+ * hits (device): n proto-hits as zdt_findprotohits writes them, strictly
+ * ascending 0-based linear indices i + Nf*J into the hcat of the nb Nf x Nr
+ * matrices fdrs (HOST array of device pointers, column stride ld), which
+ * supply the values.  Hits are numbered in the order of their first
+ * proto-hit.  For the first cap hits (device arrays of cap entries):
+ *   peak[c]   linear index of the largest value (ties: the earliest proto-hit)
+ *   value[c]  that value
+ *   size[c]   number of proto-hits
+ *   box[4c..4c+3]  first channel, last channel, first column, last column
+ * label (device, n entries, or NULL): each proto-hit's hit number.
+ * *nclusters receives the number of hits; ZDT_ETRUNC when it exceeds cap
+ * (the first cap hits and every label are still written).  ZDT_EINVAL for an
+ * index out of range or out of order, ZDT_ENOTSUP for n >= 2^31.
+ * Synchronizes the stream once (to size the outputs). */
+int zdt_clusterprotohits(const int64_t* hits, int64_t n, const float* const* fdrs, int nb,
+                         int64_t Nf, int64_t Nr, int64_t ld, int di, int dj, int64_t* peak,
+                         float* value, int64_t* size, int64_t* box, int64_t cap,
+                         int64_t* nclusters, int64_t* label, void* stream);
+
 /* --------------------------------------------------------- ragged plans
  * Many coarse channels of uneven Nf and Nt, every one through input!, one
  * zdtfdr! batch, fdrdenormalize and findprotohits, replayed as ONE hipGraph
