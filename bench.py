@@ -371,6 +371,68 @@ def long_integration(a) -> int:
     return 0
 
 
+def cluster_search(a) -> int:
+    """Proto-hit clustering on the FDR of the 2^20 x 16 x 129 workload (the
+    synthetic spectrogram through the fused path).  For three thresholds,
+    SNR 8, SNR 3 and the mean (about half of the cells, one component spanning
+    the matrix), the device proto-hits are made once, then
+    zdt_clusterprotohits (8-neighbourhood) is timed over --steps calls; each
+    call synchronizes once to size its outputs.  Prints one JSON line."""
+    import torch
+
+    import fdrt_hip as F
+    from fdrt_hip import _lib
+    from fdrt_hip import fdrutils as U
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    spec = torch.from_numpy(synthetic_spectrogram(NF, NT)).to(dev).t()
+    ws = F.ZDTWorkspace(spec, R0, DR, NR)
+    fdr = F.zdtfdr(ws)
+    L = _lib.lib()
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rows = []
+    for snr in (8.0, 3.0, 0.0):
+        thr = F.fdrdenormalize(snr, fdr)
+        buf, n, Nf, Nr, ld, arr = U._protohits_device([fdr], thr, 1 << 20)
+        nc = C.c_int64()
+        label = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        rc = L.zdt_clusterprotohits(buf.data_ptr(), n, arr, 1, Nf, Nr, ld, 1, 1, None, None, None, None, 0,
+                                    C.byref(nc), label.data_ptr(), st)
+        if rc not in (_lib.ZDT_OK, _lib.ZDT_ETRUNC):
+            _lib.check(rc)
+        cap = max(int(nc.value), 1)
+        peak = torch.empty(cap, dtype=torch.int64, device=dev)
+        value = torch.empty(cap, dtype=torch.float32, device=dev)
+        size = torch.empty(cap, dtype=torch.int64, device=dev)
+        box = torch.empty(4 * cap, dtype=torch.int64, device=dev)
+
+        def call():
+            _lib.check(L.zdt_clusterprotohits(buf.data_ptr(), n, arr, 1, Nf, Nr, ld, 1, 1, peak.data_ptr(),
+                                              value.data_ptr(), size.data_ptr(), box.data_ptr(), cap,
+                                              C.byref(nc), label.data_ptr(), st))
+
+        for _ in range(a.warmup):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            call()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        rows.append({"snr": snr, "threshold": thr, "protohits": n, "hits": int(nc.value), "ms_per_call": dt * 1e3,
+                     "protohits_per_s": n / dt})
+        del buf, label, peak, value, size, box
+    print(json.dumps({
+        "metric": "proto-hits clustered per second (zdt_clusterprotohits, threshold = mean)",
+        "value": rows[-1]["protohits_per_s"], "unit": "proto-hits/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "higher_is_better": True, "dtype": "int32/int64 (union-find), f32 values",
+        "data": "synthetic (seed 1234)",
+        "config": {"workload": "FDR of Nf=2^20, Nt=16, rates -4:1/16:4 (Nr=129), 8-neighbourhood"},
+        "thresholds": rows,
+    }))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,17 +454,23 @@ def main(argv=None) -> int:
     ap.add_argument("--spectrograms", type=int, default=6)
     ap.add_argument("--precision", default="fp32", choices=("fp32", "split-bf16", "split-fp16"),
                     help="--long: the column DFTs' arithmetic")
+    ap.add_argument("--cluster", action="store_true", help="proto-hit clustering on the 2^20 x 129 FDR instead")
     a = ap.parse_args(argv)
     if a.steps is None:
-        a.steps = 20 if a.ragged else 200
+        a.steps = 20 if (a.ragged or a.cluster) else 200
     if a.warmup is None:
-        a.warmup = 5 if a.ragged else 100
+        a.warmup = 5 if (a.ragged or a.cluster) else 100
     if a.copy_calibration:
         return copy_calibration()
     if a.valu_calibration:
         return valu_calibration()
     if a.lds_calibration:
         return lds_calibration()
+    if a.cluster:
+        if a.gpus != 1:
+            print("bench.py measures one GPU (N=1 only)", file=sys.stderr)
+            return 2
+        return cluster_search(a)
     if a.long:
         if a.gpus != 1:
             print("bench.py measures one GPU (N=1 only)", file=sys.stderr)

@@ -23,6 +23,7 @@
 // so every output is independent of scheduling.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "zdt_internal.h"
@@ -33,7 +34,10 @@ namespace zdt {
 namespace {
 
 constexpr int CL_THREADS = 256;
-constexpr int CL_PER_THREAD = 16;
+// one proto-hit per thread in every kernel: the root walks and the numbering
+// are chains of dependent loads, so parallelism, not work per thread, sets
+// their time
+constexpr int CL_PER_THREAD = 1;
 constexpr int CL_TILE = CL_THREADS * CL_PER_THREAD;  // proto-hits per tile of the root count
 
 __device__ __forceinline__ int ld_parent(const int* p) {
@@ -78,31 +82,68 @@ __device__ void uf_union(int* parent, int a, int b) {
   }
 }
 
+// The nb matrices: one pointer by value (the usual single matrix), or a
+// device table of nb pointers.
+struct Mats {
+  const float* one;
+  const float* const* tab;
+  __device__ const float* get(int64_t b) const { return tab ? tab[b] : one; }
+};
+
+struct InitArgs {
+  const int64_t* hits;
+  int64_t n;
+  int64_t Nf;
+  int64_t ncol;       // columns of the hcat'd FDR: nb * Nr
+  int* parent;
+  int64_t* colstart;  // ncol + 1 entries: first proto-hit of each column (null when dj = 0)
+  int64_t* flags;     // [0] hit count (written by the scan), [1] input error
+};
+
+// The column of a proto-hit index, clamped to [0, ncol] so that malformed
+// input cannot address outside colstart.
+__device__ __forceinline__ int64_t col_of(int64_t h, int64_t Nf, int64_t ncol) {
+  if (h < 0) return 0;
+  const int64_t c = h / Nf;
+  return c < ncol ? c : ncol;
+}
+
+// parent[k] = k; colstart[c] = first k in column >= c (every entry written
+// exactly once for sorted input; unwritten ones are clamped where read).
+__global__ __launch_bounds__(CL_THREADS) void cl_init(InitArgs a) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
+  if (k >= a.n) return;
+  a.parent[k] = static_cast<int>(k);
+  if (k == 0) a.flags[1] = 0;
+  if (a.colstart == nullptr) return;
+  const int64_t c = col_of(a.hits[k], a.Nf, a.ncol);
+  const int64_t cp = k == 0 ? -1 : col_of(a.hits[k - 1], a.Nf, a.ncol);
+  for (int64_t q = cp + 1; q <= c; ++q) a.colstart[q] = k;
+  if (k == a.n - 1)
+    for (int64_t q = c + 1; q <= a.ncol; ++q) a.colstart[q] = a.n;
+}
+
 struct LinkArgs {
   const int64_t* hits;
   int64_t n;
   int64_t Nf;
-  int64_t ncol;  // columns of the hcat'd FDR: nb * Nr
+  int64_t ncol;
   int di, dj;
   int* parent;
-  int* err;
+  const int64_t* colstart;
+  int64_t* flags;
 };
-
-__global__ __launch_bounds__(CL_THREADS) void cl_init(int* __restrict__ parent, int64_t n) {
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
-  if (k < n) parent[k] = static_cast<int>(k);
-}
 
 // One thread per proto-hit k: union with every neighbour after it in the sorted
 // order (the neighbours before it do the same towards k).  A proto-hit out of
-// range or out of order sets *err and links nothing; every read stays in
-// hits[0, n) either way.
+// range or out of order sets the error flag and links nothing; every read
+// stays in hits[0, n) either way.
 __global__ __launch_bounds__(CL_THREADS) void cl_link(LinkArgs a) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
   if (k >= a.n) return;
   const int64_t h = a.hits[k];
   if (h < 0 || h >= a.Nf * a.ncol || (k > 0 && a.hits[k - 1] >= h)) {
-    atomicOr(a.err, 1);
+    atomicOr(reinterpret_cast<unsigned long long*>(a.flags + 1), 1ull);
     return;
   }
   const int64_t J = h / a.Nf, i = h - J * a.Nf;
@@ -114,32 +155,48 @@ __global__ __launch_bounds__(CL_THREADS) void cl_link(LinkArgs a) {
     uf_union(a.parent, static_cast<int>(k), static_cast<int>(m));
   }
   // columns J+1 .. J+dj: channels max(0, i-di) .. min(Nf-1, i+di), one
-  // contiguous run of the sorted list starting at a lower bound
+  // contiguous run inside the column's own range of the sorted list
   const int64_t ilo = i > a.di ? i - a.di : 0;
   const int64_t ihi = i + a.di < a.Nf - 1 ? i + a.di : a.Nf - 1;
-  int64_t from = k + 1;
   for (int d = 1; d <= a.dj && J + d < a.ncol; ++d) {
     const int64_t base = (J + d) * a.Nf;
     const int64_t lo = base + ilo, hi = base + ihi;
-    int64_t L = from, R = a.n;
+    int64_t L = a.colstart[J + d], R = a.colstart[J + d + 1];
+    L = L < k + 1 ? k + 1 : (L > a.n ? a.n : L);
+    R = R < L ? L : (R > a.n ? a.n : R);
+    int64_t e = R;
     while (L < R) {
       const int64_t mid = (L + R) >> 1;
       if (a.hits[mid] < lo) L = mid + 1;
       else R = mid;
     }
-    int64_t m = L;
-    for (; m < a.n && a.hits[m] <= hi; ++m) uf_union(a.parent, static_cast<int>(k), static_cast<int>(m));
-    from = m;  // the next column's run starts after this one
+    for (int64_t m = L; m < e && a.hits[m] <= hi; ++m) uf_union(a.parent, static_cast<int>(k), static_cast<int>(m));
   }
 }
 
-// After linking: every entry points straight at its root.
-__global__ __launch_bounds__(CL_THREADS) void cl_flatten(int* parent, int64_t n) {
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
-  if (k >= n) return;
-  int r = ld_parent(parent + k);
-  for (int next = ld_parent(parent + r); next != r; next = ld_parent(parent + r)) r = next;
-  st_parent(parent + k, r);
+// After linking: every entry is pointed straight at its root, and the roots
+// (parent[k] == k, which flattening leaves unchanged) of each tile of CL_TILE
+// proto-hits are counted.
+__global__ __launch_bounds__(CL_THREADS) void cl_flatten_count(int* parent, int64_t n, int64_t* __restrict__ counts) {
+  __shared__ int sh[CL_THREADS / 64];
+  int s = 0;
+#pragma unroll 4
+  for (int q = 0; q < CL_PER_THREAD; ++q) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_TILE + q * CL_THREADS + threadIdx.x;
+    if (k >= n) break;
+    int r = ld_parent(parent + k);
+    for (int next = ld_parent(parent + r); next != r; next = ld_parent(parent + r)) r = next;
+    st_parent(parent + k, r);
+    s += r == static_cast<int>(k) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < CL_THREADS / 64; ++w) t += sh[w];
+    counts[blockIdx.x] = t;
+  }
 }
 
 __device__ __forceinline__ int wave_incl_scan_i(int v, int lane) {
@@ -150,28 +207,6 @@ __device__ __forceinline__ int wave_incl_scan_i(int v, int lane) {
   return v;
 }
 
-// Roots (parent[k] == k) per tile of CL_TILE proto-hits.
-__global__ __launch_bounds__(CL_THREADS) void cl_count_roots(const int* __restrict__ parent, int64_t n,
-                                                            int64_t* __restrict__ counts) {
-  __shared__ int sh[CL_THREADS / 64];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * CL_TILE + threadIdx.x * CL_PER_THREAD;
-  int s = 0;
-#pragma unroll
-  for (int q = 0; q < CL_PER_THREAD; ++q) {
-    const int64_t k = base + q;
-    s += (k < n && parent[k] == static_cast<int>(k)) ? 1 : 0;
-  }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int incl = wave_incl_scan_i(s, lane);
-  if (lane == 63) sh[wv] = incl;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int w = 0; w < CL_THREADS / 64; ++w) t += sh[w];
-    counts[blockIdx.x] = t;
-  }
-}
-
 struct NumberArgs {
   const int64_t* hits;
   int64_t n;
@@ -180,7 +215,7 @@ struct NumberArgs {
   const int64_t* offs;  // exclusive scan of the per-tile root counts
   int* cid;             // cluster ordinal, written at each root's entry
   int64_t cap;
-  unsigned long long* key;  // per cluster: peak key, cleared here
+  int64_t* peak;        // per cluster: the peak key while reducing, cleared here
   int64_t* size;
   int64_t* box;  // per cluster: channel min, channel max, column min, column max
 };
@@ -216,7 +251,7 @@ __global__ __launch_bounds__(CL_THREADS) void cl_number_roots(NumberArgs a) {
     if (k >= a.n || a.parent[k] != static_cast<int>(k)) continue;
     a.cid[k] = static_cast<int>(c);
     if (c < a.cap) {
-      a.key[c] = 0ull;
+      a.peak[c] = 0;
       a.size[c] = 0;
       a.box[4 * c + 0] = INT64_MAX;
       a.box[4 * c + 1] = 0;
@@ -232,11 +267,11 @@ struct AccumArgs {
   int64_t n;
   int64_t Nf, Nr, ld;
   int64_t ncol;
-  const float* const* fdrs;  // device table of the nb matrices
+  Mats mats;
   const int* parent;
   const int* cid;
   int64_t cap;
-  unsigned long long* key;
+  int64_t* peak;
   int64_t* size;
   int64_t* box;
   int64_t* label;  // per proto-hit cluster ordinal, or null
@@ -251,33 +286,85 @@ __device__ __forceinline__ float unordered_bits(unsigned o) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
+// One thread per proto-hit.  Proto-hits of one hit are often neighbours in the
+// sorted order, and at dense thresholds one hit can hold most of them, so per
+// proto-hit atomics would queue on one hit's five words.  Each wavefront first
+// folds the lanes of up to three of its hits (taken in lane order) with
+// cross-lane reductions, and one lane per such hit applies the result; the
+// lanes left do their own atomics.  max/min/add commute, so the result is the
+// same either way.  The peak is reduced as a key (value bits, then the
+// reversed ordinal) in the peak array itself until cl_finish decodes it.
 __global__ __launch_bounds__(CL_THREADS) void cl_accumulate(AccumArgs a) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
-  if (k >= a.n) return;
-  const int64_t h = a.hits[k];
-  if (h < 0 || h >= a.Nf * a.ncol) return;  // flagged by cl_link
-  const int c = a.cid[a.parent[k]];
-  if (a.label) a.label[k] = c;
-  if (c >= a.cap) return;
-  const int64_t J = h / a.Nf, i = h - J * a.Nf;
-  const int64_t b = J / a.Nr, j = J - b * a.Nr;
-  const float v = a.fdrs[b][i + a.ld * j];
-  // peak: the largest value, ties to the smaller ordinal (the earlier proto-hit)
-  const unsigned long long key =
-      (static_cast<unsigned long long>(ordered_bits(v)) << 32) | (0xffffffffull - static_cast<unsigned long long>(k));
-  atomicMax(a.key + c, key);
-  atomicAdd(reinterpret_cast<unsigned long long*>(a.size + c), 1ull);
-  atomicMin(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 0), static_cast<unsigned long long>(i));
-  atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 1), static_cast<unsigned long long>(i));
-  atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 3), static_cast<unsigned long long>(J));
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(a.peak);
+  bool live = false;
+  int c = -1, i = 0, J = 0;
+  unsigned long long key = 0ull;
+  if (k < a.n) {
+    const int64_t h = a.hits[k];
+    if (h >= 0 && h < a.Nf * a.ncol) {  // flagged by cl_link otherwise
+      c = a.cid[a.parent[k]];
+      if (a.label) a.label[k] = c;
+      if (c < a.cap) {
+        const int64_t JJ = h / a.Nf, ii = h - JJ * a.Nf;
+        const int64_t b = JJ / a.Nr, j = JJ - b * a.Nr;
+        const float v = a.mats.get(b)[ii + a.ld * j];
+        // peak: the largest value, ties to the smaller ordinal (the earlier proto-hit)
+        key = (static_cast<unsigned long long>(ordered_bits(v)) << 32) |
+              (0xffffffffull - static_cast<unsigned long long>(k));
+        i = static_cast<int>(ii);
+        J = static_cast<int>(JJ);
+        live = true;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(live);
+  for (int it = 0; it < 3 && todo != 0ull; ++it) {
+    const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+    const int cl = __shfl(c, leader, 64);
+    const unsigned long long same = __ballot(live && c == cl);
+    const int cnt = __popcll(same);
+    if (cnt > 1) {
+      const bool in = (same >> lane) & 1ull;
+      unsigned long long kr = in ? key : 0ull;
+      int imn = in ? i : INT32_MAX, imx = in ? i : 0, jmx = in ? J : 0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long ko = __shfl_xor(kr, off, 64);
+        kr = ko > kr ? ko : kr;
+        imn = min(imn, __shfl_xor(imn, off, 64));
+        imx = max(imx, __shfl_xor(imx, off, 64));
+        jmx = max(jmx, __shfl_xor(jmx, off, 64));
+      }
+      if (lane == leader) {
+        atomicMax(keys + cl, kr);
+        atomicAdd(reinterpret_cast<unsigned long long*>(a.size + cl), static_cast<unsigned long long>(cnt));
+        atomicMin(reinterpret_cast<unsigned long long*>(a.box + 4 * cl + 0), static_cast<unsigned long long>(imn));
+        atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * cl + 1), static_cast<unsigned long long>(imx));
+        atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * cl + 3), static_cast<unsigned long long>(jmx));
+      }
+      if (in) live = false;
+    }
+    todo &= ~same;
+  }
+  if (live) {
+    atomicMax(keys + c, key);
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.size + c), 1ull);
+    atomicMin(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 0), static_cast<unsigned long long>(i));
+    atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 1), static_cast<unsigned long long>(i));
+    atomicMax(reinterpret_cast<unsigned long long*>(a.box + 4 * c + 3), static_cast<unsigned long long>(J));
+  }
 }
 
+// Decode the peak keys of the first min(count, cap) hits in place.
 __global__ __launch_bounds__(CL_THREADS) void cl_finish(const int64_t* __restrict__ hits,
-                                                       const unsigned long long* __restrict__ key, int64_t ncl,
+                                                       const int64_t* __restrict__ flags, int64_t cap,
                                                        int64_t* __restrict__ peak, float* __restrict__ value) {
   const int64_t c = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
+  const int64_t ncl = flags[0] < cap ? flags[0] : cap;
   if (c >= ncl) return;
-  const unsigned long long kk = key[c];
+  const unsigned long long kk = static_cast<unsigned long long>(peak[c]);
   const int64_t k = static_cast<int64_t>(0xffffffffull - (kk & 0xffffffffull));
   peak[c] = hits[k];
   value[c] = unordered_bits(static_cast<unsigned>(kk >> 32));
@@ -294,72 +381,70 @@ int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* f
     *nclusters = 0;
     return ZDT_OK;
   }
+  const int64_t ncol = static_cast<int64_t>(nb) * Nr;
   const int64_t nt = (n + CL_TILE - 1) / CL_TILE;
   const int64_t nscan = hits_scan_scratch(nt);
-  // one scratch block: parent[n], cid[n] (int), then the int64 words
-  // counts[nt], scan scratch, total, the error flag, and the matrix table
+  // one scratch block, no other allocation and one synchronization (at the
+  // end, for the count and the error flag):
+  //   parent[n], cid[n] (int); counts[nt] + scan scratch, flags[2],
+  //   colstart[ncol + 1] (dj > 0), the matrix table (nb > 1) (int64 words)
   auto up = [](int64_t bytes) { return (bytes + 255) & ~int64_t(255); };
   const int64_t o_cid = up(4 * n), o_w = o_cid + up(4 * n);
-  const int64_t o_total = o_w + 8 * (nt + nscan), o_tab = o_total + 16;
-  const int64_t bytes = o_tab + 8 * static_cast<int64_t>(nb);
+  const int64_t o_flags = o_w + 8 * (nt + nscan), o_col = o_flags + 16;
+  const int64_t o_tab = o_col + (dj > 0 ? 8 * (ncol + 1) : 0);
+  const int64_t bytes = o_tab + (nb > 1 ? 8 * static_cast<int64_t>(nb) : 0);
   char* buf = nullptr;
   ZDT_TRY_HIP(hipMallocAsync(reinterpret_cast<void**>(&buf), static_cast<size_t>(bytes), st));
   int* parent = reinterpret_cast<int*>(buf);
   int* cid = reinterpret_cast<int*>(buf + o_cid);
   int64_t* counts = reinterpret_cast<int64_t*>(buf + o_w);
-  int64_t* total_dev = reinterpret_cast<int64_t*>(buf + o_total);  // [0] root count, [1] error flag
-  const float** tab = reinterpret_cast<const float**>(buf + o_tab);
-  unsigned long long* key = nullptr;
+  int64_t* flags_dev = reinterpret_cast<int64_t*>(buf + o_flags);
+  int64_t* colstart = dj > 0 ? reinterpret_cast<int64_t*>(buf + o_col) : nullptr;
+  Mats mats{fdrs[0], nb > 1 ? reinterpret_cast<const float* const*>(buf + o_tab) : nullptr};
   int64_t flags[2] = {0, 0};
-  int rc = ZDT_OK;
   auto run = [&]() -> int {
-    ZDT_TRY_HIP(hipMemsetAsync(total_dev, 0, 16, st));
-    ZDT_TRY_HIP(hipMemcpyAsync(tab, fdrs, sizeof(float*) * nb, hipMemcpyHostToDevice, st));
-    int* err = reinterpret_cast<int*>(total_dev + 1);
+    if (nb > 1)
+      ZDT_TRY_HIP(hipMemcpyAsync(buf + o_tab, fdrs, sizeof(float*) * nb, hipMemcpyHostToDevice, st));
     {
       ProfScope ps(st, "cluster_link", 20.0 * n);
-      cl_init<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(parent, n);
+      InitArgs ia{hits, n, Nf, ncol, parent, colstart, flags_dev};
+      cl_init<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(ia);
       ZDT_CHECK_LAUNCH("cl_init");
-      LinkArgs a{hits, n, Nf, nb * Nr, di, dj, parent, err};
+      LinkArgs a{hits, n, Nf, ncol, di, dj, parent, colstart, flags_dev};
       cl_link<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
       ZDT_CHECK_LAUNCH("cl_link");
-      cl_flatten<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(parent, n);
-      ZDT_CHECK_LAUNCH("cl_flatten");
     }
     {
-      ProfScope ps(st, "cluster_roots", 4.0 * n);
-      cl_count_roots<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(parent, n, counts);
-      ZDT_CHECK_LAUNCH("cl_count_roots");
+      ProfScope ps(st, "cluster_roots", 8.0 * n);
+      cl_flatten_count<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(parent, n, counts);
+      ZDT_CHECK_LAUNCH("cl_flatten_count");
     }
-    ZDT_TRY(launch_hits_scan(counts, nt, total_dev, counts + nt, st));
-    ZDT_TRY_HIP(hipMemcpyAsync(flags, total_dev, 16, hipMemcpyDeviceToHost, st));
-    ZDT_TRY_HIP(hipStreamSynchronize(st));
-    if (static_cast<int>(flags[1]) != 0)
-      return fail(ZDT_EINVAL, "clusterprotohits: proto-hits must be strictly ascending linear indices below Nf*nb*Nr");
-    const int64_t ncl = flags[0] < cap ? flags[0] : cap;
-    if (ncl > 0) ZDT_TRY_HIP(hipMallocAsync(reinterpret_cast<void**>(&key), 8 * static_cast<size_t>(ncl), st));
+    ZDT_TRY(launch_hits_scan(counts, nt, flags_dev, counts + nt, st));
     {
       ProfScope ps(st, "cluster_number", 12.0 * n);
-      NumberArgs a{hits, n, Nf, parent, counts, cid, ncl, key, size, box};
+      NumberArgs a{hits, n, Nf, parent, counts, cid, cap, peak, size, box};
       cl_number_roots<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(a);
       ZDT_CHECK_LAUNCH("cl_number_roots");
     }
-    if (ncl > 0 || label != nullptr) {
+    if (cap > 0 || label != nullptr) {
       ProfScope ps(st, "cluster_reduce", 28.0 * n);
-      AccumArgs a{hits, n, Nf, Nr, ld, nb * Nr, tab, parent, cid, ncl, key, size, box, label};
+      AccumArgs a{hits, n, Nf, Nr, ld, ncol, mats, parent, cid, cap, peak, size, box, label};
       cl_accumulate<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
       ZDT_CHECK_LAUNCH("cl_accumulate");
     }
-    if (ncl > 0) {
-      cl_finish<<<blocks_for(ncl, CL_THREADS), CL_THREADS, 0, st>>>(hits, key, ncl, peak, value);
+    if (cap > 0) {
+      cl_finish<<<blocks_for(cap < n ? cap : n, CL_THREADS), CL_THREADS, 0, st>>>(hits, flags_dev, cap, peak, value);
       ZDT_CHECK_LAUNCH("cl_finish");
     }
+    ZDT_TRY_HIP(hipMemcpyAsync(flags, flags_dev, 16, hipMemcpyDeviceToHost, st));
+    ZDT_TRY_HIP(hipStreamSynchronize(st));
     return ZDT_OK;
   };
-  rc = run();
-  if (key) (void)hipFreeAsync(key, st);
+  const int rc = run();
   (void)hipFreeAsync(buf, st);
   if (rc != ZDT_OK) return rc;
+  if (flags[1] != 0)
+    return fail(ZDT_EINVAL, "clusterprotohits: proto-hits must be strictly ascending linear indices below Nf*nb*Nr");
   *nclusters = flags[0];
   if (flags[0] > cap)
     return fail(ZDT_ETRUNC, "clusterprotohits: %lld hits exceed capacity %lld", (long long)flags[0], (long long)cap);

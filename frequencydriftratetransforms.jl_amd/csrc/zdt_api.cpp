@@ -908,6 +908,8 @@ int zdt_clusterprotohits(const int64_t* hits, int64_t n, const float* const* fdr
   ZDT_TRY(check_fdrs(reinterpret_cast<const void* const*>(fdrs), nb, Nf, Nr, ld));
   if (n < 0 || (n > 0 && hits == nullptr)) return fail(ZDT_EINVAL, "clusterprotohits: bad proto-hit list");
   if (n > INT32_MAX) return fail(ZDT_ENOTSUP, "clusterprotohits: %lld proto-hits exceed 2^31-1", (long long)n);
+  if (Nf > INT32_MAX || static_cast<int64_t>(nb) * Nr > INT32_MAX)
+    return fail(ZDT_ENOTSUP, "clusterprotohits: channels or columns exceed 2^31-1");
   if (di < 0 || dj < 0) return fail(ZDT_EINVAL, "clusterprotohits: negative neighbourhood (%d, %d)", di, dj);
   if (nclusters == nullptr) return fail(ZDT_EINVAL, "clusterprotohits: nclusters is NULL");
   if (cap < 0 || (cap > 0 && (peak == nullptr || value == nullptr || size == nullptr || box == nullptr)))
