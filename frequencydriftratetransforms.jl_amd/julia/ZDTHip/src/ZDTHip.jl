@@ -16,7 +16,7 @@ live here.  Every library call returns a status; a non-zero status throws
 """
 module ZDTHip
 
-export HipMatrix, HipZDTWorkspace, zdtpath, zdtprecision,
+export HipMatrix, HipZDTWorkspace, zdtpath, zdtprecision, clusterprotohits, findhits,
        RaggedPlan, bind!, launch!, planhits, plantuning, tune_table!, StreamSearch, search
 
 const LIBZDT = get(ENV, "ZDT_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "libzdt_hip.so"))
@@ -248,9 +248,9 @@ function _normalize!(fdrs, m, s)
     fdrs
 end
 
-# every element >= threshold, column-major over the hcat'd matrices, as
-# 1-based CartesianIndex{2} (the device returns 0-based i + Nf*(b*Nr + j))
-function _hits(fdrs, threshold)
+# every element >= threshold, column-major over the hcat'd matrices, as 0-based
+# linear indices i + Nf*(b*Nr + j) left on the device: (H, n)
+function _hits_device(fdrs, threshold)
     Nf, Nr = size(first(fdrs))
     n = Ref{Int64}(0)
     ptrs = _ptrs(fdrs)
@@ -262,8 +262,86 @@ function _hits(fdrs, threshold)
     check(ccall((:zdt_findprotohits, LIBZDT), Cint,
                 (Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cdouble, Ptr{Int64}, Int64, Ref{Int64}, Ptr{Cvoid}),
                 ptrs, length(fdrs), Nf, Nr, Nf, threshold, H.ptr, n[], n, C_NULL))
-    lin = n[] == 0 ? Int64[] : vec(Array(H))[1:n[]]
-    [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
+    H, n[]
+end
+
+_cartesian(lin, Nf) = [CartesianIndex(Int(l % Nf) + 1, Int(l ÷ Nf) + 1) for l in lin]
+
+# the same as 1-based CartesianIndex{2}
+function _hits(fdrs, threshold)
+    H, n = _hits_device(fdrs, threshold)
+    _cartesian(n == 0 ? Int64[] : vec(Array(H))[1:n], size(first(fdrs), 1))
+end
+
+# ----------------------------------------------------------------- hits from proto-hits
+
+"""
+    clusterprotohits(fdr[s], hijs; di=1, dj=1) -> (; peak, value, size, box, labels)
+    findhits(fdr[s], threshold; snr=false, di=1, dj=1) -> (hijs, hits)
+
+Group proto-hits into hits (include/zdt.h, `zdt_clusterprotohits`).  Two
+proto-hits are neighbours when their channels differ by at most `di` and their
+(hcat'd) rate columns by at most `dj`; a hit is a connected group.  Hits are
+numbered by their first proto-hit in column-major order:
+
+- `peak[h]`: the `CartesianIndex` of hit h's largest value `value[h]` (ties to
+  the earlier proto-hit);
+- `size[h]`: its number of proto-hits;
+- `box[h]`: (first channel, last channel, first column, last column);
+- `labels[p]`: the hit of proto-hit p.
+
+`findhits` is `findprotohits` followed by the clustering, with the proto-hits
+kept on the device in between.  No counterpart in the package: it stops at
+proto-hits and leaves this step to the next stage of a search
+(src/fdrutils.jl:141-144, README.md:48-53).
+"""
+function clusterprotohits(fdrs::HipFdrs, hijs::AbstractVector{<:CartesianIndex{2}}; di::Integer=1, dj::Integer=1)
+    Nf = size(first(fdrs), 1)
+    lin = Int64[(h[1] - 1) + Nf * (h[2] - 1) for h in hijs]
+    H = HipMatrix(reshape(isempty(lin) ? Int64[0] : lin, :, 1))
+    _cluster(fdrs, H, length(lin), di, dj)
+end
+clusterprotohits(fdr::HipMatrix{Float32}, hijs; kw...) = clusterprotohits((fdr,), hijs; kw...)
+
+function findhits(fdrs::HipFdrs, threshold; snr::Bool=false, di::Integer=1, dj::Integer=1)
+    if snr  # the package's fdrdenormalize: snr * s + m (src/fdrutils.jl:104-119)
+        m, s = _stats(fdrs)
+        threshold = threshold * s + m
+    end
+    H, n = _hits_device(fdrs, threshold)
+    hijs = _cartesian(n == 0 ? Int64[] : vec(Array(H))[1:n], size(first(fdrs), 1))
+    hijs, _cluster(fdrs, H, n, di, dj)
+end
+findhits(fdr::HipMatrix{Float32}, threshold; kw...) = findhits((fdr,), threshold; kw...)
+
+function _cluster(fdrs, H::HipMatrix{Int64}, n::Integer, di::Integer, dj::Integer)
+    Nf, Nr = size(first(fdrs))
+    ptrs = _ptrs(fdrs)
+    nc = Ref{Int64}(0)
+    rc = ccall((:zdt_clusterprotohits, LIBZDT), Cint,
+               (Ptr{Int64}, Int64, Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cint, Cint,
+                Ptr{Int64}, Ptr{Float32}, Ptr{Int64}, Ptr{Int64}, Int64, Ref{Int64}, Ptr{Int64}, Ptr{Cvoid}),
+               H.ptr, n, ptrs, length(fdrs), Nf, Nr, Nf, di, dj,
+               C_NULL, C_NULL, C_NULL, C_NULL, 0, nc, C_NULL, C_NULL)
+    rc in (ZDT_OK, ZDT_ETRUNC) || check(rc)
+    c = max(nc[], 1)
+    P = HipMatrix{Int64}(undef, c, 1)
+    V = HipMatrix{Float32}(undef, c, 1)
+    S = HipMatrix{Int64}(undef, c, 1)
+    B = HipMatrix{Int64}(undef, 4, c)
+    L = HipMatrix{Int64}(undef, max(n, 1), 1)
+    check(ccall((:zdt_clusterprotohits, LIBZDT), Cint,
+                (Ptr{Int64}, Int64, Ptr{Ptr{Float32}}, Cint, Int64, Int64, Int64, Cint, Cint,
+                 Ptr{Int64}, Ptr{Float32}, Ptr{Int64}, Ptr{Int64}, Int64, Ref{Int64}, Ptr{Int64}, Ptr{Cvoid}),
+                H.ptr, n, ptrs, length(fdrs), Nf, Nr, Nf, di, dj,
+                P.ptr, V.ptr, S.ptr, B.ptr, nc[], nc, L.ptr, C_NULL))
+    h = nc[]
+    box = Array(B)
+    (; peak = _cartesian(vec(Array(P))[1:h], Nf),
+       value = vec(Array(V))[1:h],
+       size = Int.(vec(Array(S))[1:h]),
+       box = [Tuple(Int.(box[:, q]) .+ 1) for q in 1:h],
+       labels = n == 0 ? Int[] : Int.(vec(Array(L))[1:n]) .+ 1)
 end
 
 # ----------------------------------------------------------------- ragged plans
