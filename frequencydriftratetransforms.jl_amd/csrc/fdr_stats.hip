@@ -15,8 +15,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 
+#include "zdt_fast_dev.h"
 #include "zdt_kernels.h"
 
 namespace zdt {
@@ -144,6 +146,7 @@ int launch_normalize(float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double m, d
 constexpr int HIT_THREADS = 256;
 constexpr int HIT_PER_THREAD = 16;
 constexpr int64_t HIT_TILE = HIT_THREADS * HIT_PER_THREAD;
+static_assert(HIT_TILE == FDR_TILE && HIT_TILE % FDR_RUN == 0, "k3's run maxima tile the proto-hit tiles");
 
 int64_t hits_tiles(int64_t Nf, int64_t Nr) { return (Nf * Nr + HIT_TILE - 1) / HIT_TILE; }
 
@@ -152,12 +155,10 @@ __device__ __forceinline__ float load_lin(const float* fdr, int64_t Nf, int64_t 
   return fdr[(g % Nf) + (g / Nf) * ld];
 }
 
-// vec: the matrix is contiguous (ld == Nf) and 16-byte aligned
-__global__ __launch_bounds__(HIT_THREADS) void hits_count(const float* __restrict__ fdr, int64_t Nf, int64_t n,
-                                                          int64_t ld, double thr_h, const double* thr_d,
-                                                          int64_t* __restrict__ counts, int vec) {
-  const double thr = thr_d ? *thr_d : thr_h;
-  const int64_t tile = blockIdx.x;
+// The proto-hits of one tile, counted by the whole block; the total is valid
+// in thread 0.  vec: the matrix is contiguous (ld == Nf) and 16-byte aligned.
+__device__ int count_tile(const float* __restrict__ fdr, int64_t Nf, int64_t n, int64_t ld, double thr,
+                          int64_t tile, int vec, int* sh) {
   const int64_t base = tile * HIT_TILE;
   int c = 0;
   if (vec && base + HIT_TILE <= n) {
@@ -178,13 +179,88 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_count(const float* __restric
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
   constexpr int NW = HIT_THREADS / 64;
-  __shared__ int sh[NW];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
+  int t = 0;
+  if (threadIdx.x == 0)
     for (int w = 0; w < NW; ++w) t += sh[w];
-    counts[tile] = t;
+  __syncthreads();  // sh is reused by the caller's next tile
+  return t;
+}
+
+__global__ __launch_bounds__(HIT_THREADS) void hits_count(const float* __restrict__ fdr, int64_t Nf, int64_t n,
+                                                          int64_t ld, double thr_h, const double* thr_d,
+                                                          int64_t* __restrict__ counts, int vec) {
+  const double thr = thr_d ? *thr_d : thr_h;
+  __shared__ int sh[HIT_THREADS / 64];
+  const int t = count_tile(fdr, Nf, n, ld, thr, blockIdx.x, vec, sh);
+  if (threadIdx.x == 0) counts[blockIdx.x] = t;
+}
+
+// Tiles per block of the sparse proto-hit kernels: a block looks at the
+// bounds of HIT_GROUP tiles and works only on those that can hold a hit.
+constexpr int HIT_GROUP = 16;
+
+// hits_count with the FDR's run maxima (smax, written by k3, layout
+// [column][N1/16][N2] with N1 = Nf/(2 N2)): a tile whose runs all lie under the
+// threshold has no proto-hit and is not read.  Tile tc of a column holds the
+// rows n2 of [tc R, tc R + R), R = 2048/N1, of every run column nb.  A block
+// takes HIT_GROUP consecutive tiles of one column: 16 R rows of N1/16 run
+// columns, 2048 maxima, 8 per thread (8 consecutive rows of one nb), folded
+// per tile in LDS.
+__global__ __launch_bounds__(HIT_THREADS) void hits_count_sparse(const float* __restrict__ fdr, int64_t Nf, int64_t n,
+                                                                 int64_t ld, double thr_h, const double* thr_d,
+                                                                 const float* __restrict__ smax, int64_t ntiles,
+                                                                 int64_t* __restrict__ counts, int vec, int N2,
+                                                                 int rpt) {
+  static_assert(HIT_GROUP * (HIT_TILE / FDR_RUN) == HIT_THREADS * 8, "8 run maxima per thread");
+  const double thr = thr_d ? *thr_d : thr_h;
+  __shared__ int sh[HIT_THREADS / 64];
+  __shared__ unsigned tm[HIT_GROUP];
+  __shared__ unsigned need;
+  if (threadIdx.x < HIT_GROUP) tm[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) need = 0u;
+  __syncthreads();
+  const int64_t tpc = Nf / HIT_TILE;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * HIT_GROUP;  // tpc is a multiple of HIT_GROUP
+  const int64_t jc = t0 / tpc, tc0 = t0 - jc * tpc;
+  const int R = HIT_GROUP * rpt;  // rows of the group (a multiple of 16)
+  const int v0 = threadIdx.x * 8;
+  const int nb = v0 / R, r0 = v0 - nb * R;
+  if (t0 < ntiles) {
+    const float4* s = reinterpret_cast<const float4*>(smax + (jc * (Nf / (2 * static_cast<int64_t>(N2)) / 16) + nb) *
+                                                                 static_cast<int64_t>(N2) + tc0 * rpt + r0);
+    const float4 u = s[0], w = s[1];
+    const float v[8] = {u.x, u.y, u.z, u.w, w.x, w.y, w.z, w.w};
+    // fold runs of the same tile, one LDS max per tile met
+    int tl = r0 / rpt;
+    float m = v[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+      const int ti = (r0 + i) / rpt;
+      if (ti != tl) {
+        atomicMax(&tm[tl], fdr_ordered(m));
+        tl = ti;
+        m = v[i];
+      } else {
+        m = fmaxf(m, v[i]);
+      }
+    }
+    atomicMax(&tm[tl], fdr_ordered(m));
+  }
+  __syncthreads();
+  if (threadIdx.x < HIT_GROUP) {
+    const int64_t tile = t0 + threadIdx.x;
+    if (tile < ntiles) {
+      if (static_cast<double>(fdr_unordered(tm[threadIdx.x])) < thr) counts[tile] = 0;
+      else atomicOr(&need, 1u << threadIdx.x);
+    }
+  }
+  __syncthreads();
+  for (unsigned mk = need; mk != 0u; mk &= mk - 1u) {  // uniform
+    const int64_t tl = t0 + (__ffs(mk) - 1);
+    const int c = count_tile(fdr, Nf, n, ld, thr, tl, vec, sh);
+    if (threadIdx.x == 0) counts[tl] = c;
   }
 }
 
@@ -274,18 +350,11 @@ __global__ __launch_bounds__(1024) void hits_scan(int64_t* __restrict__ counts, 
   if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restrict__ fdr, int64_t Nf, int64_t n,
-                                                          int64_t ld, double thr_h, const double* thr_d,
-                                                          const int64_t* __restrict__ offsets, int64_t ntiles,
-                                                          const int64_t* __restrict__ total, int64_t base_index,
-                                                          const int64_t* __restrict__ pos_base,
-                                                          int64_t* __restrict__ hits, int64_t cap, int vec) {
-  const double thr = thr_d ? *thr_d : thr_h;
-  const int64_t tile = blockIdx.x;
-  const int64_t pb = pos_base ? *pos_base : 0;  // hits already written by earlier matrices
-  const int64_t start = offsets[tile] + pb;
-  const int64_t end = ((tile + 1 < ntiles) ? offsets[tile + 1] : *total) + pb;
-  if (end == start || start >= cap) return;  // uniform: no hits here (or out of room)
+// The proto-hits of one tile written in order at hits[start...], by the whole
+// block (which must not have returned early).
+__device__ void write_tile(const float* __restrict__ fdr, int64_t Nf, int64_t n, int64_t ld, double thr, int64_t tile,
+                           int64_t start, int64_t base_index, int64_t* __restrict__ hits, int64_t cap, int vec,
+                           int* sh) {
   // thread handles HIT_PER_THREAD consecutive elements, so a block scan of
   // the per-thread counts gives each hit its ordered slot
   const int64_t base = tile * HIT_TILE + static_cast<int64_t>(threadIdx.x) * HIT_PER_THREAD;
@@ -307,7 +376,6 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restric
       if (g < n && static_cast<double>(load_lin(fdr, Nf, ld, g)) >= thr) mask |= 1u << i;
     }
   }
-  __shared__ int sh[HIT_THREADS];
   const int c = __popc(mask);
   sh[threadIdx.x] = c;
   __syncthreads();
@@ -318,6 +386,7 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restric
     __syncthreads();
   }
   int64_t pos = start + sh[threadIdx.x] - c;
+  __syncthreads();  // sh is reused by the next tile
   while (mask) {
     const int i = __ffs(mask) - 1;
     mask &= mask - 1;
@@ -326,11 +395,60 @@ __global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restric
   }
 }
 
+// HIT_GROUP tiles per block; a tile whose scanned count is zero (or whose
+// hits lie past cap) is not read.
+__global__ __launch_bounds__(HIT_THREADS) void hits_write(const float* __restrict__ fdr, int64_t Nf, int64_t n,
+                                                          int64_t ld, double thr_h, const double* thr_d,
+                                                          const int64_t* __restrict__ offsets, int64_t ntiles,
+                                                          const int64_t* __restrict__ total, int64_t base_index,
+                                                          const int64_t* __restrict__ pos_base,
+                                                          int64_t* __restrict__ hits, int64_t cap, int vec,
+                                                          int64_t nt_here) {
+  const double thr = thr_d ? *thr_d : thr_h;
+  __shared__ int sh[HIT_THREADS];
+  __shared__ unsigned need;
+  __shared__ int64_t starts[HIT_GROUP];
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * HIT_GROUP;
+  if (threadIdx.x < 64) {
+    const int64_t tile = t0 + threadIdx.x;
+    bool work = false;
+    if (threadIdx.x < HIT_GROUP && tile < nt_here) {
+      const int64_t pb = pos_base ? *pos_base : 0;  // hits already written by earlier matrices
+      const int64_t start = offsets[tile] + pb;
+      const int64_t end = ((tile + 1 < ntiles) ? offsets[tile + 1] : *total) + pb;
+      work = end > start && start < cap;
+      starts[threadIdx.x] = start;
+    }
+    const unsigned long long b = __ballot(work);
+    if (threadIdx.x == 0) need = static_cast<unsigned>(b);
+  }
+  __syncthreads();
+  for (unsigned m = need; m != 0u; m &= m - 1u) {  // uniform
+    const int q = __ffs(m) - 1;
+    write_tile(fdr, Nf, n, ld, thr, t0 + q, starts[q], base_index, hits, cap, vec, sh);
+  }
+}
+
 int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, const double* thr_dev,
-                      int64_t* counts, hipStream_t st) {
+                      int64_t* counts, hipStream_t st, const float* smax, int smax_n2) {
   const int64_t nt = hits_tiles(Nf, Nr);
   if (nt == 0) return ZDT_OK;
   const int vec = (ld == Nf && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
+  if (smax != nullptr) {
+    // the layout the fused output writes: N1 = Nf/(2 N2) a power of two in
+    // [16, 2048] (whole rows per tile), tiles per column a multiple of the group
+    const int64_t N1 = smax_n2 > 0 ? Nf / (2 * static_cast<int64_t>(smax_n2)) : 0;
+    if (ld != Nf || Nf % (HIT_TILE * HIT_GROUP) != 0 || N1 < 16 || N1 > 2048 || (N1 & (N1 - 1)) != 0 ||
+        2 * N1 * smax_n2 != Nf || (reinterpret_cast<uintptr_t>(smax) & 15) != 0)
+      return fail(ZDT_EINVAL, "hits_count: run maxima of an unsupported shape (Nf=%lld, N2=%d)", (long long)Nf,
+                  smax_n2);
+    // bytes: the run maxima, and the counts
+    ProfScope ps(st, "hits_count_sparse", 4.0 * (HIT_TILE / FDR_RUN) * nt + 8.0 * nt);
+    hits_count_sparse<<<static_cast<unsigned>((nt + HIT_GROUP - 1) / HIT_GROUP), HIT_THREADS, 0, st>>>(
+        fdr, Nf, Nf * Nr, ld, thr, thr_dev, smax, nt, counts, vec, smax_n2, static_cast<int>(2048 / N1));
+    ZDT_CHECK_LAUNCH("hits_count_sparse");
+    return ZDT_OK;
+  }
   ProfScope ps(st, "hits_count", 4.0 * Nf * Nr + 8.0 * nt);
   hits_count<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev, counts, vec);
   ZDT_CHECK_LAUNCH("hits_count");
@@ -363,9 +481,9 @@ int launch_hits_write_impl(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld,
   // of its last tile is the next matrix's first offset (or the total)
   const int vec = (ld == Nf && (reinterpret_cast<uintptr_t>(fdr) & 15) == 0) ? 1 : 0;
   ProfScope ps(st, "hits_write", 4.0 * Nf * Nr + 8.0 * nt);
-  hits_write<<<static_cast<unsigned>(nt), HIT_THREADS, 0, st>>>(fdr, Nf, Nf * Nr, ld, thr, thr_dev,
-                                                                offsets + tile0, ntiles_all - tile0, total_dev,
-                                                                base_index, pos_base, hits, cap, vec);
+  hits_write<<<static_cast<unsigned>((nt + HIT_GROUP - 1) / HIT_GROUP), HIT_THREADS, 0, st>>>(
+      fdr, Nf, Nf * Nr, ld, thr, thr_dev, offsets + tile0, ntiles_all - tile0, total_dev, base_index, pos_base, hits,
+      cap, vec, nt);
   ZDT_CHECK_LAUNCH("hits_write");
   return ZDT_OK;
 }

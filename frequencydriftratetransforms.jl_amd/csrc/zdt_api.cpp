@@ -454,14 +454,22 @@ bool ws_is_fused(const zdt_ws* ws) { return ws != nullptr && ws->fast != nullptr
 
 int64_t devstats_scratch_doubles(int64_t Nf, int64_t Nr) { return 2 * static_cast<int64_t>(colstats_nchunk(Nf)) * Nr; }
 
-int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* colvar, double* part) {
+int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* colvar, double* part,
+                 float* smax) {
   ZDT_TRY(check_ws(ws));
   ProfInstall prof_(ws);
   if (dest == nullptr || colmean == nullptr || colvar == nullptr)
     return fail(ZDT_EINVAL, "zdtfdr! with statistics: dest and statistics buffers are required");
   ZDT_TRY(set_device(ws));
   if (!ws->have_input) return fail(ZDT_ESTATE, "zdtfdr!: no input (call input! or pass a spectrogram)");
+  // the run maxima ride on this one run's k3
+  struct RunMax {
+    FastPlan* p;
+    RunMax(FastPlan* q, float* s) : p(q) { fast_set_smax(p, s); }
+    ~RunMax() { fast_set_smax(p, nullptr); }
+  };
   if (use_fast(ws)) {
+    RunMax rm_(ws->fast, smax);
     ZDT_TRY(fast_run(ws->fast, r0, ZDT_WINDOW_RECT, nullptr, dest, out_scale(ws), ws->stream, colmean, colvar));
     ws->ys_fast = false;
     ws->ys_fout = false;
@@ -469,6 +477,7 @@ int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* col
     return ZDT_OK;
   }
   if (ws->fout != nullptr) {
+    RunMax rm_(ws->fout, smax);
     ZDT_TRY(fast_out_run(ws->fout, ws->Ft, ws->Nk, true, static_cast<int>(ws->Nt), r0, ws->dr, ZDT_WINDOW_RECT,
                          nullptr, dest, out_scale(ws), ws->stream, colmean, colvar));
     ws->ys_fast = false;
@@ -476,6 +485,7 @@ int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* col
     ws->ys_src = nullptr;
     return ZDT_OK;
   }
+  if (smax != nullptr) return fail(ZDT_EINVAL, "zdtfdr! with run maxima: the generic path has none");
   if (part == nullptr) return fail(ZDT_EINVAL, "zdtfdr! with statistics: the generic path needs scratch");
   ZDT_TRY(run_impl(ws, r0, ZDT_WINDOW_RECT, nullptr, dest));
   return launch_colstats2(dest, ws->Nf, ws->Nr, ws->Nf, part, colmean, colvar, ws->stream);
@@ -484,6 +494,11 @@ int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* col
 int ws_path(const zdt_ws* ws) {
   if (ws == nullptr) return 0;
   return ws->fast ? 1 : (ws->fout ? 2 : 0);
+}
+
+int ws_fused_n2(const zdt_ws* ws) {
+  if (ws == nullptr) return 0;
+  return ws->fast ? fast_plan_n2(ws->fast) : (ws->fout ? fast_plan_n2(ws->fout) : 0);
 }
 
 }  // namespace zdt

@@ -26,6 +26,11 @@ int fast_run(FastPlan* p, float r0, int wkind, const float* window, float* dest,
              double* colmean = nullptr, double* colvar = nullptr);
 int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* colmean = nullptr,
                 double* colvar = nullptr);
+// the next outputs also write the FDR's run maxima into smax (NULL: no
+// longer), Nr * Nf/FDR_RUN floats (K3Args::smax); p may be NULL
+void fast_set_smax(FastPlan* p, float* smax);
+// the plan's strided FFT length N2 (the run maxima's row count), 0 for NULL
+int fast_plan_n2(const FastPlan* p);
 // doubles of k3's statistics partials (held by every fused plan)
 int64_t fast_stats_doubles(int64_t Nf, int64_t Nr);
 

@@ -39,6 +39,7 @@
 #include "fft_generic.h"
 #include "zdt_fast_dev.h"
 #include "zdt_fast_plan.h"
+#include "zdt_kernels.h"
 
 namespace zdt {
 
@@ -90,7 +91,9 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k1_inp
 
 // ---------------------------------------------------------------- k3: output pass 2
 
-template <int R1>
+// TMAX: also write the FDR's run maxima to a.smax (a separate instance, so
+// that the plain one keeps its registers)
+template <int R1, bool TMAX>
 __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_output(K3Args a) {
   using C = CF<R1>;
   __shared__ c32 ex[C::LDS];
@@ -172,10 +175,26 @@ __global__ __launch_bounds__(CF<R1>::THREADS, 512 / CF<R1>::THREADS) void k3_out
         typedef float f2v __attribute__((ext_vector_type(2)));
         f2v* q = reinterpret_cast<f2v*>(out + (mp << a.logN1) + o);
         const int64_t step = int64_t(32) << a.logN1;
+        // run maxima (TMAX): the 16 lanes of a row store the run of outputs
+        // n1 = 16 nb + c at n2 = mp + 32 rp; its maximum goes to
+        // smax[j][nb][n2].  The row maximum of step rp is kept by lane
+        // c = rp mod 16, which stores it after the loop, so that the block's
+        // stores fill one contiguous run of N2 floats.
+        float keep0 = 0.0f, keep1 = 0.0f;
 #pragma unroll
         for (int rp = 0; rp < R1; ++rp) {
           const f2v v = {y[rp].x, y[rp].y};
           __builtin_nontemporal_store(v, q + rp * step);
+          if constexpr (TMAX) {
+            const float m = row16_max(fmaxf(y[rp].x, y[rp].y));
+            if (rp < 16) keep0 = c == rp ? m : keep0;
+            else keep1 = c == rp - 16 ? m : keep1;
+          }
+        }
+        if constexpr (TMAX) {
+          float* sm = a.smax + (static_cast<int64_t>(j) * (N1 >> 4) + nb) * C::N2;
+          sm[mp + 32u * c] = keep0;
+          if (R1 > 16) sm[mp + 32u * (c + 16)] = keep1;
         }
       }
     }
@@ -923,6 +942,12 @@ static int k3_waves(const FastPlan* p) { return (p->N2 == 1024 ? CF<32>::THREADS
 // (8 wavefronts) and N2 = 512 (4 wavefronts)
 int64_t fast_stats_doubles(int64_t Nf, int64_t Nr) { return Nr * (Nf / 2 / 2048) * 3; }
 
+void fast_set_smax(FastPlan* p, float* smax) {
+  if (p != nullptr) p->smax = smax;
+}
+
+int fast_plan_n2(const FastPlan* p) { return p != nullptr ? p->N2 : 0; }
+
 int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* colmean, double* colvar) {
   (void)scale;  // folded into k2
   if (!p->have_B) return fail(ZDT_ESTATE, "output!: no post-processed spectra");
@@ -938,16 +963,19 @@ int fast_output(FastPlan* p, float* dest, double scale, hipStream_t st, double* 
   a.ntiles = static_cast<int>(p->Nr * (p->N1 / 16));
   a.part = colmean != nullptr ? p->part : nullptr;
   a.paired = p->paired ? 1 : 0;
+  a.smax = p->smax;
   {
     ProfScope ps(st, "k3_output", 8.0 * p->M * p->Nr + 4.0 * p->Nf * p->Nr);
     if (p->precision != ZDT_PRECISION_FP32) {
       ZDT_TRY(launch_k3_mfma(p, a, st));
     } else if (p->N2 == 1024) {
       const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
-      k3_output<32><<<grid, CF<32>::THREADS, 0, st>>>(a);
+      if (a.smax) k3_output<32, true><<<grid, CF<32>::THREADS, 0, st>>>(a);
+      else k3_output<32, false><<<grid, CF<32>::THREADS, 0, st>>>(a);
     } else {
       const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks));
-      k3_output<16><<<grid, CF<16>::THREADS, 0, st>>>(a);
+      if (a.smax) k3_output<16, true><<<grid, CF<16>::THREADS, 0, st>>>(a);
+      else k3_output<16, false><<<grid, CF<16>::THREADS, 0, st>>>(a);
     }
     ZDT_CHECK_LAUNCH("k3_output");
   }

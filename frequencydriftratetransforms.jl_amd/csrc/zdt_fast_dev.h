@@ -8,6 +8,7 @@
 
 #include "fft_butterflies.h"
 #include "zdt_internal.h"
+#include "zdt_kernels.h"
 
 namespace zdt {
 
@@ -308,7 +309,41 @@ struct K3Args {
   int ntiles;    // Nr * N1/16
   double* part;  // NULL, or per-wavefront FDR statistics [Nr][N1/16][waves][3] (see k3_stats)
   int paired;    // k2 wrote the rates in mirrored pairs from the middle outwards
+  // NULL, or the largest value of every run of FDR_RUN consecutive cells of
+  // the FDR (Nr * Nf/FDR_RUN floats): the run maxima that let the proto-hit
+  // kernels skip tiles under a threshold.  A run is the 16 complex outputs
+  // n = n1 + N1 n2, n1 = 16 nb .. 16 nb + 15, of one column (cells 2n, 2n+1),
+  // which 16 adjacent lanes of k3 store in one step; its maximum is
+  // smax[j][nb][n2] ([Nr][N1/16][N2]).
+  float* smax;
 };
+
+// Order-preserving map of a float onto an unsigned integer (0 orders below
+// every float), and its inverse.
+__device__ __forceinline__ unsigned fdr_ordered(float v) {
+  const unsigned u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fdr_unordered(unsigned o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// max(v, v of the lane that DPP control CTRL selects within the row of 16)
+template <int CTRL>
+__device__ __forceinline__ float dpp_max(float v) {
+  return fmaxf(v, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false)));
+}
+
+// The maximum of v over each row of 16 lanes, in every lane of the row.
+// Every lane of the wavefront must take part.  Row-local DPP moves (no LDS):
+// lanes i^1, i^2 (quad permutations), then the mirrors within the half-row
+// (quads 0 and 1) and the row (halves 0 and 1).
+__device__ __forceinline__ float row16_max(float v) {
+  v = dpp_max<0xb1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_max<0x4e>(v);   // quad_perm [2,3,0,1]
+  v = dpp_max<0x141>(v);  // row_half_mirror
+  return dpp_max<0x140>(v);  // row_mirror
+}
 
 // k3's i-th rate: the reverse of k2's write order, so the spectra written
 // last are read first (Nr-1, Nr-2, ... or, for paired rates, Nr-1, 0, Nr-2, 1, ...)

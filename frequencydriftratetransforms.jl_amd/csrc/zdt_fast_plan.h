@@ -33,6 +33,9 @@ struct FastPlan {
   bool have_B = false;
   int64_t resident_blocks = 512;  // persistent k1/k3 grid: 2 blocks per CU
   int precision = 0;              // ZDT_PRECISION_* of the DFT stages
+  // set by a caller around one run: k3 then also writes the FDR's run
+  // maxima there (Nr * Nf/FDR_RUN floats, see K3Args::smax)
+  float* smax = nullptr;
 };
 
 int ilog2i(int64_t n);

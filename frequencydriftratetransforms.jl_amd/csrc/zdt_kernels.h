@@ -6,6 +6,13 @@
 
 namespace zdt {
 
+// Cells per tile of the proto-hit kernels: tiles are FDR_TILE consecutive
+// cells of the column-major matrix (fdr_stats.hip).  The fused output kernel
+// can also write the maximum of every run of FDR_RUN cells (K3Args::smax),
+// FDR_TILE/FDR_RUN consecutive floats per tile.
+constexpr int64_t FDR_TILE = 4096;
+constexpr int64_t FDR_RUN = 32;
+
 struct CztArgs {
   const c32* F;
   int64_t ldF;  // Nk x Nt
@@ -50,8 +57,11 @@ int launch_normalize(float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double m, d
 // proto-hits: count per tile, exclusive scan, ordered write (see fdr_stats.hip)
 int64_t hits_tiles(int64_t Nf, int64_t Nr);
 // thr_dev, when not NULL, holds the threshold on the device (thr is ignored)
+// smax: NULL, or the matrix's run maxima as the fused output writes them
+// (K3Args::smax, [Nr][N1/16][smax_n2] with N1 = Nf/(2 smax_n2); needs ld == Nf):
+// tiles whose runs are all under the threshold are not read
 int launch_hits_count(const float* fdr, int64_t Nf, int64_t Nr, int64_t ld, double thr, const double* thr_dev,
-                      int64_t* counts, hipStream_t st);
+                      int64_t* counts, hipStream_t st, const float* smax = nullptr, int smax_n2 = 0);
 // scratch: hits_scan_scratch(n) int64 words
 int64_t hits_scan_scratch(int64_t n);
 int launch_hits_scan(int64_t* counts, int64_t n, int64_t* total, int64_t* scratch, hipStream_t st);

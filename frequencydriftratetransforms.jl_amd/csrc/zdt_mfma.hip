@@ -371,6 +371,7 @@ __global__ __launch_bounds__(512, 1) void k3_output_mfma(K3Args a) {
       f32x16 yr, yi;
       dft32_tile<PREC>(y, A, sc, yr, yi);
       typedef float f2v __attribute__((ext_vector_type(2)));
+      float keep = 0.0f;  // run maximum of step v = c (lanes of a row share mp and h)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const float fr = yr[v] * isc, fi = yi[v] * isc;
@@ -384,7 +385,13 @@ __global__ __launch_bounds__(512, 1) void k3_output_mfma(K3Args a) {
         // the FDR is not read again (see k3_output)
         const f2v val = {fr, fi};
         __builtin_nontemporal_store(val, reinterpret_cast<f2v*>(out + ((mp + 32u * rp) << a.logN1) + n1));
+        if (a.smax != nullptr) {  // uniform
+          const float m = row16_max(fmaxf(fr, fi));
+          keep = c == v ? m : keep;
+        }
       }
+      if (a.smax != nullptr)  // the run n1 = 16 nb + c at n2 = mp + 32 rp: smax[j][nb][n2]
+        a.smax[(static_cast<int64_t>(j) * (int64_t(1) << (a.logN1 - 4)) + nb) * N2 + mp + 32u * mf_drow(c, h)] = keep;
     }
     if (a.part != nullptr) {
       k3_stats(a.part + ((static_cast<int64_t>(j) * nblk + nb) * 8 + w) * 3, s1, s2, ct);

@@ -43,6 +43,7 @@ struct zdt_stream {
   hipEvent_t copied[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
   float* spec[2] = {nullptr, nullptr};
   std::vector<float*> fdr;             // resident batch FDRs (or one reused buffer)
+  std::vector<float*> smax;            // their run maxima (fused paths; empty otherwise)
   int64_t* dhits[2] = {nullptr, nullptr};  // device hit staging, one per spectrogram in flight
   int64_t dcap = 0;
   int64_t* counts = nullptr;  // one batch's tile counts, scan scratch, batch total, running total
@@ -69,7 +70,8 @@ int64_t stream_bytes(int64_t Nf, int64_t Nt, int64_t Nr, int64_t Nrb, bool resid
   const int64_t ws = zdt_workspace_bytes(Nf, Nt, Nrb, nullptr, 0, flags);
   if (ws < 0) return -1;
   const int64_t nb = (Nr + Nrb - 1) / Nrb;
-  const int64_t fdrs = (resident ? nb : 1) * Nf * Nrb * 4;
+  // the FDR buffers and their run maxima (one float per FDR_RUN cells)
+  const int64_t fdrs = (resident ? nb : 1) * (Nf * Nrb * 4 + (Nf / FDR_RUN) * Nrb * 4);
   const int64_t nt = hits_tiles(Nf, Nrb);
   return ws + fdrs + 2 * Nf * Nt * 4 + 2 * dcap * 8 + (nt + hits_scan_scratch(nt) + 2) * 8 +
          (2 * nb * Nrb + 1 + devstats_scratch_doubles(Nf, Nrb)) * 8;
@@ -102,6 +104,8 @@ int zdt_stream_destroy(zdt_stream* s) {
   if (s->ws) zdt_destroy(s->ws);
   for (float* f : s->fdr)
     if (f) (void)hipFree(f);
+  for (float* m : s->smax)
+    if (m) (void)hipFree(m);
   for (int i = 0; i < 2; ++i) {
     if (s->spec[i]) (void)hipFree(s->spec[i]);
     if (s->copied[i]) (void)hipEventDestroy(s->copied[i]);
@@ -200,6 +204,16 @@ int zdt_stream_create(int64_t Nf, int64_t Nt, float r0, float dr, int64_t Nr, in
     rc = salloc(s, reinterpret_cast<void**>(&f), static_cast<size_t>(Nf * s->Nrb) * 4);
     if (rc == ZDT_OK) s->fdr.push_back(f);
   }
+  // run maxima let the hit count skip tiles under the threshold; the fused
+  // paths' output kernel writes them (their Nf are multiples of the tile)
+  if (rc == ZDT_OK && ws_path(s->ws) != 0 && Nf % (16 * FDR_TILE) == 0 && ws_fused_n2(s->ws) > 0 &&
+      Nf / (2 * ws_fused_n2(s->ws)) <= 2048) {
+    for (int64_t b = 0; b < nbuf && rc == ZDT_OK; ++b) {
+      float* m = nullptr;
+      rc = salloc(s, reinterpret_cast<void**>(&m), static_cast<size_t>((Nf / FDR_RUN) * s->Nrb) * 4);
+      if (rc == ZDT_OK) s->smax.push_back(m);
+    }
+  }
   for (int i = 0; i < 2 && rc == ZDT_OK; ++i)
     rc = salloc(s, reinterpret_cast<void**>(&s->dhits[i]), static_cast<size_t>(dcap) * 8);
   const int64_t nt = hits_tiles(Nf, s->Nrb);
@@ -291,19 +305,22 @@ int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, 
     ZDT_TRY_HIP(hipEventRecord(s->used[slot], s->compute));
     // statistics of every batch, side by side: the first Nr entries are the
     // requested rates' columns
+    auto smax_of = [&](int64_t b) { return s->smax.empty() ? nullptr : s->smax[s->resident ? b : 0]; };
     for (int64_t b = 0; b < nb; ++b) {
       float* f = s->fdr[s->resident ? b : 0];
-      ZDT_TRY(run_devstats(s->ws, r0b(b), f, colmean + b * Nrb, colvar + b * Nrb, scratch));
+      ZDT_TRY(run_devstats(s->ws, r0b(b), f, colmean + b * Nrb, colvar + b * Nrb, scratch, smax_of(b)));
     }
     ProfUse prof_(ws_profiler(s->ws));  // the hit extraction joins the workspace's kernel profile
     ZDT_TRY(launch_hits_threshold(colmean, colvar, s->Nr, snr, thr_dev, s->compute));
     // hits in column-major order of the hcat'd batches
     for (int64_t b = 0; b < nb; ++b) {
       float* f = s->fdr[s->resident ? b : 0];
-      if (!s->resident) ZDT_TRY(zdt_run(s->ws, r0b(b), ZDT_WINDOW_RECT, nullptr, f));
+      // recomputed batches rewrite their (identical) statistics, with the run maxima
+      if (!s->resident)
+        ZDT_TRY(run_devstats(s->ws, r0b(b), f, colmean + b * Nrb, colvar + b * Nrb, scratch, smax_of(b)));
       const int64_t nv = valid(b);
       const int64_t nt = hits_tiles(Nf, nv);
-      ZDT_TRY(launch_hits_count(f, Nf, nv, Nf, 0.0, thr_dev, counts, s->compute));
+      ZDT_TRY(launch_hits_count(f, Nf, nv, Nf, 0.0, thr_dev, counts, s->compute, smax_of(b), ws_fused_n2(s->ws)));
       ZDT_TRY(launch_hits_scan(counts, nt, batch_total, scan_scratch, s->compute));
       if (cap > 0)
         ZDT_TRY(launch_hits_write_impl(f, Nf, nv, Nf, 0.0, thr_dev, counts, nt, 0, batch_total, b * Nrb * Nf,

@@ -19,13 +19,18 @@ int create_ws_tuned(const float* spec, int64_t Nf, int64_t Nt, float r0, float d
 bool ws_is_fused(const zdt_ws* ws);
 // 1: fused kernels; 2: per-bin chirp-Z with the fused output; 0: generic
 int ws_path(const zdt_ws* ws);
+// the fused paths' strided FFT length N2 (the run maxima's row count), 0 otherwise
+int ws_fused_n2(const zdt_ws* ws);
 
 // zdtfdr! (rectangular window) into dest plus the FDR's column mean and
 // corrected variance on the device (colmean, colvar: Nr doubles each), all
 // enqueued on the workspace stream.  The fused paths compute the statistics
 // inside the output kernel; the generic path runs colstats over dest with
-// part as scratch (devstats_scratch_doubles).
-int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* colvar, double* part);
+// part as scratch (devstats_scratch_doubles).  smax: NULL, or (fused paths,
+// ws_path != 0) Nr * Nf/FDR_RUN floats that receive the FDR's run maxima
+// for launch_hits_count.
+int run_devstats(zdt_ws* ws, float r0, float* dest, double* colmean, double* colvar, double* part,
+                 float* smax = nullptr);
 int64_t devstats_scratch_doubles(int64_t Nf, int64_t Nr);
 
 }  // namespace zdt
