@@ -43,6 +43,7 @@ struct Shape {
   double* stats = nullptr;  // partial sums, colmean, colvar, threshold
   int64_t* counts = nullptr;  // tile counts / offsets
   float* fdr = nullptr;       // FDR for channels bound without a dest
+  float* smax = nullptr;      // the FDR's run maxima (fused paths): the hit count reads only tiles that can hold one
 };
 
 }  // namespace
@@ -270,7 +271,7 @@ int enqueue_channel(Shape& s, int c, const float* spec, float* dest, double thr,
     double* colmean = s.stats + part_n;
     double* colvar = colmean + s.Nr;
     double* t = thr_out ? thr_out + c : colvar + s.Nr;
-    ZDT_TRY(run_devstats(s.ws, s.r0, dest, colmean, colvar, s.stats));
+    ZDT_TRY(run_devstats(s.ws, s.r0, dest, colmean, colvar, s.stats, s.smax));
     ZDT_TRY(launch_hits_threshold(colmean, colvar, s.Nr, thr, t, s.st));
     thr_dev = t;
   } else {
@@ -278,7 +279,9 @@ int enqueue_channel(Shape& s, int c, const float* spec, float* dest, double thr,
     if (thr_out) ZDT_TRY(launch_set_double(thr_out + c, thr_h, s.st));
   }
   const int64_t nt = hits_tiles(s.Nf, s.Nr);
-  ZDT_TRY(launch_hits_count(dest, s.Nf, s.Nr, ld, thr_h, thr_dev, s.counts, s.st));
+  // the run maxima exist when the statistics pass wrote them
+  const float* smax = snr ? s.smax : nullptr;
+  ZDT_TRY(launch_hits_count(dest, s.Nf, s.Nr, ld, thr_h, thr_dev, s.counts, s.st, smax, ws_fused_n2(s.ws)));
   ZDT_TRY(launch_hits_scan(s.counts, nt, nhits + c, s.counts + nt, s.st));
   if (cap > 0)
     ZDT_TRY(launch_hits_write_impl(dest, s.Nf, s.Nr, ld, thr_h, thr_dev, s.counts, nt, 0, nhits + c, 0,
@@ -303,6 +306,7 @@ int zdt_plan_destroy(zdt_plan* p) {
     if (s.ws) zdt_destroy(s.ws);
     if (s.stats) (void)hipFree(s.stats);
     if (s.counts) (void)hipFree(s.counts);
+    if (s.smax) (void)hipFree(s.smax);
     if (s.fdr) (void)hipFree(s.fdr);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.st) (void)hipStreamDestroy(s.st);
@@ -379,6 +383,9 @@ int zdt_plan_ragged(int nch, const int64_t* Nf, const int64_t* Nt, const float* 
     const int64_t nt = hits_tiles(s.Nf, s.Nr);
     if (rc == ZDT_OK)
       rc = hip_alloc(reinterpret_cast<void**>(&s.counts), (nt + hits_scan_scratch(nt)) * sizeof(int64_t));
+    const int n2 = ws_fused_n2(s.ws);
+    if (rc == ZDT_OK && n2 > 0 && s.Nf % (16 * FDR_TILE) == 0 && s.Nf / (2 * n2) <= 2048)
+      rc = hip_alloc(reinterpret_cast<void**>(&s.smax), (s.Nf / FDR_RUN) * s.Nr * sizeof(float));
   }
   if (rc != ZDT_OK) {
     std::string msg = zdt_last_error();
