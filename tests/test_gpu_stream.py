@@ -64,6 +64,22 @@ def check_hits(hits, ref64, thr, cols=None, tol=2e-6):
     return len(r)
 
 
+def batch_hits(F, spec, r0, dr, Nr, Nrb, thr, precision="fp32"):
+    """The proto-hits a batched search must return, by a full read of every
+    batch FDR: each batch is a workspace of Nrb rates from r0 + b*Nrb*dr (the
+    search's own batch, so the same FDR), its requested columns are read by
+    the plain findprotohits, and the columns are offset by b*Nrb."""
+    out = []
+    for b in range((Nr + Nrb - 1) // Nrb):
+        ws = F.ZDTWorkspace(spec, r0 + b * Nrb * dr, dr, Nrb, precision=precision)
+        fdr = F.zdtfdr(ws)
+        h = F.findprotohits(fdr[:, :min(Nrb, Nr - b * Nrb)], thr)
+        h[:, 1] += b * Nrb
+        out.append(h)
+        ws.close()
+    return np.concatenate(out)
+
+
 def oracle_thr(ref64, snr):
     m, s = H.fdrstats(ref64)
     return H.fdrdenormalize(snr, m, s)
@@ -125,6 +141,9 @@ def test_stream_fused_output_and_generic_paths(F, Nf, Nt):
         ref = Z.fdr64(s, r0, dr, Nr)
         assert abs(thr[i] - oracle_thr(ref, 7.0)) <= 1e-5 * abs(thr[i])
         check_hits(hits[i], ref, thr[i])
+        # exactly the hits of a full read of the same batch FDRs (the fused
+        # paths count only the tiles their run maxima allow)
+        assert np.array_equal(hits[i], batch_hits(F, s, r0, dr, Nr, Nrb, thr[i]))
     st.close()
 
 
@@ -146,6 +165,7 @@ def test_stream_split_precision(F, Nf, Nt, precision):
         ref = Z.fdr64(s, r0, dr, Nr)
         assert abs(thr[i] - oracle_thr(ref, 7.0)) <= 1e-4 * abs(thr[i])
         assert check_hits(hits[i], ref, thr[i], tol=1e-4) > 0
+        assert np.array_equal(hits[i], batch_hits(F, s, r0, dr, Nr, Nrb, thr[i], precision))
     st.close()
     with pytest.raises(ValueError):
         F.StreamSearch(Nf, Nt, (r0, dr, Nr), precision="bf16")
@@ -239,6 +259,12 @@ def test_long_integration_shape_vs_fp64(F):
     assert ws.path == "fused-output"
     fdr = F.zdtfdr(ws)
     out = fdr[:, cols].cpu().numpy()
+    # the resident search computed this same FDR (one batch of all 1025
+    # rates); its hit count skipped tiles by k3's run maxima, the plain
+    # findprotohits reads every cell: the hit lists are identical
+    full = F.findprotohits(fdr, t1[0])
+    say(f"long: {len(full)} proto-hits by a full read, {len(h1[0])} by the search")
+    assert np.array_equal(full, h1[0])
     del fdr
     ws.close()
     torch.cuda.empty_cache()
