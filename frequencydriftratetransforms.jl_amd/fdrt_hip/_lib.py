@@ -91,6 +91,7 @@ SIGNATURES = {
     "zdt_calib_lds": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "zdt_plan_tuning": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "zdt_tune_table": (C.c_int, [C.c_char_p]),
+    "zdt_bshuf_decompress": (C.c_int, [_vp, C.c_size_t, _vp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]),
     "zdt_clusterprotohits": (C.c_int, [_vp, _i64, C.POINTER(_vp), C.c_int, _i64, _i64, _i64, C.c_int, C.c_int,
                                        _vp, _vp, _vp, _vp, _i64, _i64p, _vp, _vp]),
     "zdt_findprotohits_noalloc": (C.c_int, [C.POINTER(_vp), C.c_int, _i64, _i64, _i64, C.c_double, _vp, _i64,

@@ -15,8 +15,11 @@ Nf x Nt buffer -- a numpy array, or a (pinned) torch tensor such as
 GPU asynchronously.  Compressed datasets go through HDF5's own filter
 pipeline: deflate and shuffle are built into libhdf5, and bitshuffle (filter
 32008, what rawspec writes) is used when its plugin is on
-``HDF5_PLUGIN_PATH``; a dataset whose filters libhdf5 cannot run is refused
-before any read, naming the filter.
+``HDF5_PLUGIN_PATH``.  Without the plugin, a dataset whose only filter is
+bitshuffle (plain or with LZ4, the Breakthrough Listen files' compression) is
+read chunk by chunk with HDF5's direct chunk reads and decoded by the
+library's own decoder (``zdt_bshuf_decompress``).  A dataset with any other
+filter libhdf5 cannot run is refused before any read, naming the filter.
 
 Indices are Python's: channel and time ranges are 0-based, half-open, and the
 IF index is 0-based (the reference's ``data[freq_range, 1, :]`` is
@@ -107,6 +110,14 @@ class _Lib:
             "H5Pget_filter2": (C.c_int, [_hid, C.c_uint, C.POINTER(C.c_uint), C.POINTER(C.c_size_t),
                                          C.POINTER(C.c_uint), C.c_size_t, C.c_char_p, C.POINTER(C.c_uint)]),
             "H5Zfilter_avail": (C.c_int, [C.c_int]),
+            "H5Pget_chunk": (C.c_int, [_hid, C.c_int, C.POINTER(_hsize)]),
+            "H5Pget_filter_by_id2": (_herr, [_hid, C.c_int, C.POINTER(C.c_uint), C.POINTER(C.c_size_t),
+                                             C.POINTER(C.c_uint), C.c_size_t, C.c_char_p, C.POINTER(C.c_uint)]),
+            "H5Dget_type": (_hid, [_hid]),
+            "H5Tget_order": (C.c_int, [_hid]),
+            "H5Dget_chunk_storage_size": (_herr, [_hid, C.POINTER(_hsize), C.POINTER(_hsize)]),
+            "H5Dread_chunk": (_herr, [_hid, _hid, C.POINTER(_hsize), C.POINTER(C.c_uint32), C.c_void_p]),
+            "H5Dwrite_chunk": (_herr, [_hid, _hid, C.c_uint32, C.POINTER(_hsize), C.c_size_t, C.c_void_p]),
             "H5Aexists": (C.c_int, [_hid, C.c_char_p]),
             "H5Aopen": (_hid, [_hid, C.c_char_p, _hid]),
             "H5Acreate2": (_hid, [_hid, C.c_char_p, _hid, _hid, _hid, _hid]),
@@ -253,6 +264,7 @@ class FilterbankFile:
             self.ntime, self.nifs, self.nchans = dims
             self.shape = (self.nchans, self.nifs, self.ntime)
             self.filters = self._filters()
+            self._bshuf = self._bitshuffle_params()
             self.header = {}
             for key in HEADER_KEYS:
                 for obj in (self._d, self._f):
@@ -278,9 +290,44 @@ class FilterbankFile:
             h.L.H5Pclose(pl)
         return out
 
+    def _bitshuffle_params(self):
+        """(element size, block size, compression, chunk dims) when the chunks
+        are to be decoded here: bitshuffle is the only filter, libhdf5 has no
+        plugin for it, the data are little-endian Float32 and the compression
+        is none (0) or LZ4 (2).  None otherwise."""
+        h = self._h
+        if [f for f, _ in self.filters] != [32008] or h.L.H5Zfilter_avail(32008) > 0:
+            return None
+        pl = h.ok(h.L.H5Dget_create_plist(self._d), "H5Dget_create_plist")
+        ty = h.ok(h.L.H5Dget_type(self._d), "H5Dget_type")
+        try:
+            flags, nelm, cfg = C.c_uint(), C.c_size_t(8), C.c_uint()
+            cd = (C.c_uint * 8)()
+            h.ok(h.L.H5Pget_filter_by_id2(pl, 32008, C.byref(flags), C.byref(nelm), cd, 0, None, C.byref(cfg)),
+                 "H5Pget_filter_by_id2")
+            cd = list(cd)[:nelm.value]
+            dims = (_hsize * 3)()
+            if h.L.H5Pget_chunk(pl, 3, dims) != 3:
+                return None
+            if h.L.H5Tget_class(ty) != _H5T_FLOAT or h.L.H5Tget_size(ty) != 4 or h.L.H5Tget_order(ty) != 0:
+                return None
+        finally:
+            h.L.H5Tclose(ty)
+            h.L.H5Pclose(pl)
+        elem = cd[2] if len(cd) > 2 else 4
+        block = cd[3] if len(cd) > 3 else 0
+        comp = cd[4] if len(cd) > 4 else 0
+        if elem != 4 or comp not in (0, 2):
+            return None
+        return elem, block, comp, tuple(int(x) for x in dims)
+
     def check_filters(self) -> None:
         """Raises FilterbankError when libhdf5 cannot run one of the dataset's
-        filters (bitshuffle without its plugin, for one)."""
+        filters and this module cannot decode it (bitshuffle combined with
+        other filters, or with a compression other than LZ4, without its
+        plugin, for one)."""
+        if self._bshuf is not None:
+            return
         for fid, name in self.filters:
             if self._h.L.H5Zfilter_avail(fid) <= 0:
                 label = _FILTER_NAMES.get(fid, name)
@@ -308,6 +355,10 @@ class FilterbankFile:
         if out is None:
             out = np.empty((nc, nt), np.float32, order="F")
         ptr = _column_major_ptr(out, nc, nt)
+        if self._bshuf is not None:
+            dst = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_float)), shape=(nt, nc)).T  # (nc, nt) view
+            self._read_chunks(c0, nc, ifnum, t0, nt, dst)
+            return out
         fs = h.ok(h.L.H5Dget_space(self._d), "H5Dget_space")
         ms = -1
         try:
@@ -323,6 +374,39 @@ class FilterbankFile:
                 h.L.H5Sclose(ms)
             h.L.H5Sclose(fs)
         return out
+
+    def _read_chunks(self, c0: int, nc: int, ifnum: int, t0: int, nt: int, dst: np.ndarray) -> None:
+        """The hyperslab by direct chunk reads, bitshuffle chunks decoded by
+        zdt_bshuf_decompress; dst: the (nc, nt) destination view."""
+        from . import _lib as Z
+        h = self._h
+        elem, block, comp, (ct, ci, cc) = self._bshuf
+        L = Z.lib()
+        raw = np.empty(ct * ci * cc, np.float32)
+        ic = ifnum // ci
+        for tc in range(t0 // ct, (t0 + nt - 1) // ct + 1):
+            for ccx in range(c0 // cc, (c0 + nc - 1) // cc + 1):
+                off = (_hsize * 3)(tc * ct, ic * ci, ccx * cc)
+                size = _hsize(0)
+                h.ok(h.L.H5Dget_chunk_storage_size(self._d, off, C.byref(size)), "H5Dget_chunk_storage_size")
+                ta, tb = max(t0, tc * ct), min(t0 + nt, (tc + 1) * ct)
+                ca, cb = max(c0, ccx * cc), min(c0 + nc, (ccx + 1) * cc)
+                if size.value == 0:  # never written: the fill value
+                    dst[ca - c0:cb - c0, ta - t0:tb - t0] = 0.0
+                    continue
+                buf = np.empty(size.value, np.uint8)
+                mask = C.c_uint32(0)
+                h.ok(h.L.H5Dread_chunk(self._d, 0, off, C.byref(mask), buf.ctypes.data_as(C.c_void_p)), "H5Dread_chunk")
+                if mask.value & 1:  # the filter was skipped for this chunk: stored as is
+                    if size.value != raw.nbytes:
+                        raise FilterbankError(f"{self.path}: unfiltered chunk of {size.value} bytes, {raw.nbytes} expected")
+                    raw[:] = buf.view(np.float32)
+                else:
+                    Z.check(L.zdt_bshuf_decompress(buf.ctypes.data_as(C.c_void_p), size.value,
+                                                   raw.ctypes.data_as(C.c_void_p), raw.nbytes, elem, block, comp))
+                chunk = raw.reshape(ct, ci, cc)
+                dst[ca - c0:cb - c0, ta - t0:tb - t0] = \
+                    chunk[ta - tc * ct:tb - tc * ct, ifnum - ic * ci, ca - ccx * cc:cb - ccx * cc].T
 
     def close(self) -> None:
         h = self._h

@@ -363,6 +363,20 @@ typedef struct zdt_stream zdt_stream;
 int zdt_host_alloc(size_t bytes, void** p);
 int zdt_host_free(void* p);
 
+/* Decodes one bitshuffle-filtered HDF5 chunk (filter 32008) on the host: the
+ * compression of the Breakthrough Listen filterbank files that the
+ * reference's heavy test reads through the H5Zbitshuffle plugin
+ * (test/heavytests.jl:6-14).  No counterpart in the reference itself; the
+ * filterbank ingest uses it with HDF5's direct chunk reads when libhdf5 has
+ * no bitshuffle plugin.  in: the stored chunk (in_len bytes); out: the
+ * chunk's out_len uncompressed bytes; elem_size: the filter's cd_values[2];
+ * compression: cd_values[4], 0 (bit transposition only, in blocks of
+ * block_size elements, 0 = the library's default) or 2 (LZ4: the block size
+ * comes from the chunk's header).  ZDT_EINVAL for a malformed chunk (every
+ * read and write is bounds-checked), ZDT_ENOTSUP for other compressions. */
+int zdt_bshuf_decompress(const void* in, size_t in_len, void* out, size_t out_len,
+                         size_t elem_size, size_t block_size, int compression);
+
 /* Nrb = 0 chooses the batch length: the largest that keeps every batch FDR
  * resident within hbm_bytes, else the largest that fits with one FDR buffer
  * (each batch is then computed twice: once for the statistics, once for the

@@ -66,13 +66,18 @@ def test_read_into_buffers(tmp_path):
 
 def test_errors(tmp_path):
     a = spec(64, 4)
-    # a filter libhdf5 cannot run here (bitshuffle, registered as optional so
-    # the write stores the chunks unfiltered): refused before any read
-    FB.write_filterbank(tmp_path / "d.h5", a, extra_filter=(32008, True))
+    # a filter libhdf5 cannot run here (blosc, registered as optional so the
+    # write stores the chunks unfiltered): refused before any read
+    FB.write_filterbank(tmp_path / "d.h5", a, extra_filter=(32001, True))
     with FB.FilterbankFile(tmp_path / "d.h5") as f:
-        assert f.filters and f.filters[0][0] == 32008
-        with pytest.raises(FB.FilterbankError, match="32008"):
+        assert f.filters and f.filters[0][0] == 32001
+        with pytest.raises(FB.FilterbankError, match="32001"):
             f.read()
+    # bitshuffle alone is read by direct chunk reads without its plugin; these
+    # chunks were stored unfiltered (the filter was skipped on write)
+    FB.write_filterbank(tmp_path / "g.h5", a, extra_filter=(32008, True))
+    with FB.FilterbankFile(tmp_path / "g.h5") as f:
+        assert f.filters[0][0] == 32008 and np.array_equal(f.read(), a)
     FB.write_filterbank(tmp_path / "e.h5", a)
     with FB.FilterbankFile(tmp_path / "e.h5") as f:
         for kw in ({"channels": range(60, 70)}, {"channels": range(5, 5)}, {"times": range(0, 5)},

@@ -9,7 +9,10 @@ the one the FP64 oracle and the integer-shift intfdr agree on (55, 10)
 (0-based; test_heavy_analogue_oracle pins it on the CPU).  The GPU test takes
 its input the reference's way: the 150 channels are written into a wider
 filterbank HDF5 file (Voyager-like header, channels 2000..2149 of 4096) and
-read back as data[freq_range, 1, :]."""
+read back as data[freq_range, 1, :], with the file's chunks compressed either
+by HDF5's deflate + shuffle or by bitshuffle + LZ4 as the Breakthrough Listen
+files are (decoded without the plugin the reference loads,
+test/heavytests.jl:6)."""
 import sys
 
 import numpy as np
@@ -39,33 +42,39 @@ def test_heavy_analogue_oracle():
     assert set(np.argmax(Z.fdshift64(s, -2.43), axis=0).tolist()) <= {54, 55}
 
 
-def filterbank_file(path):
-    """The analogue spectrogram inside a 4096-channel filterbank file
-    (bitshuffle is not available here; deflate + shuffle chunks instead)."""
+def filterbank_file(path, compression="deflate"):
+    """The analogue spectrogram inside a 4096-channel filterbank file, its
+    chunks deflate + shuffle or bitshuffle + LZ4 compressed."""
     from fdrt_hip import filterbank as FB
     rng = np.random.default_rng(59046)
     wide = rng.exponential(1.0, size=(4096, NT)).astype(np.float32)
     wide[2000:2000 + NF] = spectrogram()
-    FB.write_filterbank(path, wide, {"fch1": 8421.386717353016, "foff": -2.7939677238464355e-06,
-                                     "tsamp": 18.253611008, "source_name": "VOYAGER-1"},
-                        chunks=(NT, 1, 1024), deflate=4, shuffle=True)
+    if compression == "bitshuffle":
+        from test_bitshuffle import write_bitshuffled
+        write_bitshuffled(path, wide, (NT, 1, 1024), comp=2)
+    else:
+        FB.write_filterbank(path, wide, {"fch1": 8421.386717353016, "foff": -2.7939677238464355e-06,
+                                         "tsamp": 18.253611008, "source_name": "VOYAGER-1"},
+                            chunks=(NT, 1, 1024), deflate=4, shuffle=True)
     return range(2000, 2000 + NF)
 
 
-def test_heavy_analogue_ingest(tmp_path):
+@pytest.mark.parametrize("compression", ["deflate", "bitshuffle"])
+def test_heavy_analogue_ingest(tmp_path, compression):
     from fdrt_hip import filterbank as FB
-    freq_range = filterbank_file(tmp_path / "voyager_like.h5")
+    freq_range = filterbank_file(tmp_path / "voyager_like.h5", compression)
     s = FB.read_spectrogram(tmp_path / "voyager_like.h5", channels=freq_range)
     assert s.shape == (NF, NT) and np.array_equal(s, spectrogram())
     assert argmax2(Z.intfdr(s, RATES)) == PEAK
 
 
 @pytest.mark.gpu
-def test_heavy_analogue_gpu(gpu, tmp_path):
+@pytest.mark.parametrize("compression", ["deflate", "bitshuffle"])
+def test_heavy_analogue_gpu(gpu, tmp_path, compression):
     import fdrt_hip as F
-    print("heavy-test analogue: 150 x 16, rates 0:-0.25:-5, via a filterbank HDF5 file", flush=True)
+    print(f"heavy-test analogue: 150 x 16, rates 0:-0.25:-5, via a {compression} filterbank HDF5 file", flush=True)
     sys.stdout.flush()
-    freq_range = filterbank_file(tmp_path / "voyager_like.h5")
+    freq_range = filterbank_file(tmp_path / "voyager_like.h5", compression)
     s = F.read_spectrogram(tmp_path / "voyager_like.h5", channels=freq_range)
     assert np.array_equal(s, spectrogram())
     ws = F.fftfdr_workspace(s)
