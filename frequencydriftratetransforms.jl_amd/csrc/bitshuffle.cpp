@@ -24,8 +24,11 @@
 // literal length and match length - 4 in 4 bits each, 255-continued; literals;
 // a 2-byte little-endian match offset; the last sequence holds only
 // literals).  Every read and write is bounds-checked.
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "zdt_internal.h"
@@ -140,33 +143,70 @@ extern "C" int zdt_bshuf_decompress(const void* in, size_t in_len, void* out, si
     if (in_len != out_len) return fail(ZDT_EINVAL, "bitshuffle: uncompressed chunk of the wrong size");
   }
   if (block_size % 8 != 0) return fail(ZDT_EINVAL, "bitshuffle: block size %zu not a multiple of 8", block_size);
+  // locate every block first (LZ4 blocks are size-prefixed), then decode
+  // them on several threads: blocks are independent
+  struct Blk {
+    const uint8_t* src;
+    size_t src_len;  // compressed bytes (LZ4), or the block's bytes
+    size_t out_off, nb;
+  };
+  std::vector<Blk> blks;
   const size_t n = out_len / elem_size;
-  std::vector<uint8_t> tmp(compression == 2 ? block_size * elem_size : 0);
   size_t done = 0;  // elements
   while (n - done >= 8) {
     const size_t nb = (n - done >= block_size) ? block_size : (n - done) / 8 * 8;
     const size_t bytes = nb * elem_size;
-    const uint8_t* planes = ip;
     if (compression == 2) {
       if (left < 4) return fail(ZDT_EINVAL, "bitshuffle: truncated block header");
       const uint32_t cs = be32(ip);
       ip += 4;
       left -= 4;
       if (cs > left) return fail(ZDT_EINVAL, "bitshuffle: block of %u bytes past the chunk end", cs);
-      if (lz4_block(ip, cs, tmp.data(), bytes) != static_cast<int64_t>(bytes))
-        return fail(ZDT_EINVAL, "bitshuffle: malformed LZ4 block at element %zu", done);
-      planes = tmp.data();
+      blks.push_back({ip, cs, done * elem_size, nb});
       ip += cs;
       left -= cs;
     } else {
+      if (bytes > left) return fail(ZDT_EINVAL, "bitshuffle: truncated chunk");
+      blks.push_back({ip, bytes, done * elem_size, nb});
       ip += bytes;
       left -= bytes;
     }
-    untranspose(planes, op + done * elem_size, nb, elem_size);
     done += nb;
   }
   const size_t rest = (n - done) * elem_size;  // fewer than 8 elements, stored raw
   if (left != rest) return fail(ZDT_EINVAL, "bitshuffle: %zu trailing bytes, %zu expected", left, rest);
+  std::atomic<int64_t> bad{-1};  // element offset of a malformed block (any one of them)
+  auto work = [&](size_t b0, size_t b1) {
+    std::vector<uint8_t> tmp(compression == 2 ? block_size * elem_size : 0);
+    for (size_t q = b0; q < b1; ++q) {
+      const Blk& k = blks[q];
+      const uint8_t* planes = k.src;
+      if (compression == 2) {
+        const size_t bytes = k.nb * elem_size;
+        if (lz4_block(k.src, k.src_len, tmp.data(), bytes) != static_cast<int64_t>(bytes)) {
+          int64_t expect = -1;
+          bad.compare_exchange_strong(expect, static_cast<int64_t>(k.out_off / elem_size));
+          return;
+        }
+        planes = tmp.data();
+      }
+      untranspose(planes, op + k.out_off, k.nb, elem_size);
+    }
+  };
+  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+  const size_t nth = std::min<size_t>({16, hw, (blks.size() + 7) / 8});
+  if (nth <= 1) {
+    work(0, blks.size());
+  } else {
+    std::vector<std::thread> pool;
+    const size_t per = (blks.size() + nth - 1) / nth;
+    for (size_t i = 0; i < nth; ++i) {
+      const size_t b0 = i * per, b1 = std::min(blks.size(), b0 + per);
+      if (b0 < b1) pool.emplace_back(work, b0, b1);
+    }
+    for (auto& th : pool) th.join();
+  }
+  if (bad.load() >= 0) return fail(ZDT_EINVAL, "bitshuffle: malformed LZ4 block at element %lld", (long long)bad.load());
   std::memcpy(op + done * elem_size, ip, rest);
   return ZDT_OK;
 }
