@@ -178,7 +178,7 @@ void tune_store(const TuneKey& key, const TuneVal& v) {
   if (g_tune_path.empty()) return;
   FILE* f = std::fopen(g_tune_path.c_str(), "a");
   if (f == nullptr) return;  // a table that cannot be written only costs the next plan a retune
-  std::fprintf(f, "zdt-tune-v1 %s %d %lld %lld %lld %d %d %d %d %.4f\n", key.arch.c_str(), key.cus, (long long)key.Nf,
+  std::fprintf(f, "zdt-tune-v1 %s %d %lld %lld %lld %d %d %d %d %.6g\n", key.arch.c_str(), key.cus, (long long)key.Nf,
                (long long)key.Nt, (long long)key.Nr, key.generic, v.path, v.n2, v.maxr, v.ms);
   std::fclose(f);
 }
