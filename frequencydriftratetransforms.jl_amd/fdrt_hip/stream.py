@@ -133,3 +133,39 @@ class StreamSearch:
             out.append(np.stack([lin % self.Nf + 1, lin // self.Nf + 1], axis=1).astype(np.int64)
                        if nh[i] else np.zeros((0, 2), np.int64))
         return out, thr
+
+    def search_files(self, paths: Sequence, snr: float = 10.0, cap: int = 1 << 20, channels=None, ifnum: int = 0,
+                     times=None, group: int = 2, dataset: str = "data") -> Tuple[List[np.ndarray], np.ndarray]:
+        """search() over filterbank HDF5 files (fdrt_hip.filterbank): each
+        file's data[channels, ifnum, times] spectrogram (Nf x Nt) is read
+        straight into pinned host memory on a reader thread, ``group`` files
+        at a time, while the GPU searches the group before; two groups of
+        pinned buffers alternate.  The reads (HDF5, bitshuffle decoding
+        included) release the GIL, so they overlap the search.  Returns
+        (hits, thresholds) as search() does, one entry per file, in order."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        from .filterbank import read_spectrogram
+        paths = list(paths)
+        if not paths:
+            return [], np.zeros(0)
+        group = max(1, int(group))
+        groups = [paths[i:i + group] for i in range(0, len(paths), group)]
+        bufs = [[pinned_spectrogram(self.Nf, self.Nt) for _ in range(len(groups[0]))] for _ in range(min(2, len(groups)))]
+
+        def fill(k, names):
+            for buf, path in zip(bufs[k], names):
+                read_spectrogram(path, channels=channels, ifnum=ifnum, times=times, out=buf, dataset=dataset)
+            return len(names)
+
+        hits, thrs = [], []
+        with ThreadPoolExecutor(max_workers=1) as reader:
+            pending = reader.submit(fill, 0, groups[0])
+            for g in range(len(groups)):
+                m = pending.result()
+                if g + 1 < len(groups):  # the next group's reads run during this search
+                    pending = reader.submit(fill, (g + 1) % 2, groups[g + 1])
+                h, t = self.search(bufs[g % 2][:m], snr=snr, cap=cap)
+                hits += h
+                thrs.append(t)
+        return hits, np.concatenate(thrs)

@@ -171,6 +171,32 @@ def test_stream_split_precision(F, Nf, Nt, precision):
         F.StreamSearch(Nf, Nt, (r0, dr, Nr), precision="bf16")
 
 
+def test_stream_search_files(F, tmp_path):
+    """search_files: the same hits and thresholds as search() over the
+    arrays, with the spectrograms read on the reader thread, two files at a
+    time, from filterbank files (deflate, and bitshuffle/LZ4 decoded without
+    the plugin) as a channel window of a wider file."""
+    from fdrt_hip import filterbank as FB
+    from test_bitshuffle import write_bitshuffled
+    specs = specs3()
+    paths = []
+    for i, s in enumerate(specs):
+        wide = np.random.default_rng(i).exponential(1.0, size=(NF + 100, NT)).astype(np.float32)
+        wide[50:50 + NF] = s
+        path = tmp_path / f"spec{i}.h5"
+        if i == 0:
+            FB.write_filterbank(path, wide, chunks=(NT, 1, 8192), deflate=1)
+        else:
+            write_bitshuffled(path, wide, (NT, 1, 8192))
+        paths.append(path)
+    st = F.StreamSearch(NF, NT, (R0, DR, NR), Nrb=10)
+    h0, t0 = st.search(specs, snr=8.0)
+    h1, t1 = st.search_files(paths, snr=8.0, channels=range(50, 50 + NF), group=2)
+    st.close()
+    assert len(h1) == 3 and np.array_equal(t0, t1)
+    assert all(np.array_equal(a, b) for a, b in zip(h0, h1))
+
+
 def test_stream_profile(F):
     """profile=True: every kernel of a search is timed on the search's
     workspace, the hit extraction included; the same thresholds and hits as
