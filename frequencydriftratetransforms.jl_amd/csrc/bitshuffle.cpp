@@ -115,8 +115,8 @@ void untranspose(const uint8_t* in, uint8_t* out, size_t n, size_t E) {
 
 using namespace zdt;
 
-extern "C" int zdt_bshuf_decompress(const void* in, size_t in_len, void* out, size_t out_len, size_t elem_size,
-                                    size_t block_size, int compression) {
+static int bshuf_decompress(const void* in, size_t in_len, void* out, size_t out_len, size_t elem_size,
+                            size_t block_size, int compression) {
   if ((in == nullptr && in_len > 0) || (out == nullptr && out_len > 0) || elem_size == 0)
     return fail(ZDT_EINVAL, "bitshuffle: bad buffers or element size");
   if (compression != 0 && compression != 2)
@@ -176,8 +176,15 @@ extern "C" int zdt_bshuf_decompress(const void* in, size_t in_len, void* out, si
   const size_t rest = (n - done) * elem_size;  // fewer than 8 elements, stored raw
   if (left != rest) return fail(ZDT_EINVAL, "bitshuffle: %zu trailing bytes, %zu expected", left, rest);
   std::atomic<int64_t> bad{-1};  // element offset of a malformed block (any one of them)
+  std::atomic<bool> oom{false};
   auto work = [&](size_t b0, size_t b1) {
-    std::vector<uint8_t> tmp(compression == 2 ? block_size * elem_size : 0);
+    std::vector<uint8_t> tmp;
+    try {
+      tmp.resize(compression == 2 ? block_size * elem_size : 0);
+    } catch (...) {  // nothing may leave a thread
+      oom.store(true);
+      return;
+    }
     for (size_t q = b0; q < b1; ++q) {
       const Blk& k = blks[q];
       const uint8_t* planes = k.src;
@@ -195,18 +202,36 @@ extern "C" int zdt_bshuf_decompress(const void* in, size_t in_len, void* out, si
   };
   const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
   const size_t nth = std::min<size_t>({16, hw, (blks.size() + 7) / 8});
-  if (nth <= 1) {
-    work(0, blks.size());
-  } else {
+  bool serial = nth <= 1;
+  if (!serial) {
     std::vector<std::thread> pool;
     const size_t per = (blks.size() + nth - 1) / nth;
-    for (size_t i = 0; i < nth; ++i) {
-      const size_t b0 = i * per, b1 = std::min(blks.size(), b0 + per);
-      if (b0 < b1) pool.emplace_back(work, b0, b1);
+    try {
+      for (size_t i = 0; i < nth; ++i) {
+        const size_t b0 = i * per, b1 = std::min(blks.size(), b0 + per);
+        if (b0 < b1) pool.emplace_back(work, b0, b1);
+      }
+    } catch (...) {  // no thread to be had: the started ones finish, then one thread does it all
+      serial = true;
     }
     for (auto& th : pool) th.join();
   }
+  if (serial) {
+    bad.store(-1);
+    oom.store(false);
+    work(0, blks.size());
+  }
+  if (oom.load()) return fail(ZDT_ENOMEM, "bitshuffle: out of host memory");
   if (bad.load() >= 0) return fail(ZDT_EINVAL, "bitshuffle: malformed LZ4 block at element %lld", (long long)bad.load());
   std::memcpy(op + done * elem_size, ip, rest);
   return ZDT_OK;
+}
+
+extern "C" int zdt_bshuf_decompress(const void* in, size_t in_len, void* out, size_t out_len, size_t elem_size,
+                                    size_t block_size, int compression) {
+  try {
+    return bshuf_decompress(in, in_len, out, out_len, elem_size, block_size, compression);
+  } catch (...) {  // the block list or a decode buffer could not be allocated
+    return fail(ZDT_ENOMEM, "bitshuffle: out of host memory");
+  }
 }
