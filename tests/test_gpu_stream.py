@@ -126,10 +126,11 @@ def test_stream_auto_batch_and_truncation(F):
     st.close()
 
 
-@pytest.mark.parametrize("Nf,Nt", [(1 << 17, 20), (24576, 9)])
+@pytest.mark.parametrize("Nf,Nt", [(1 << 17, 20), (1 << 21, 20), (24576, 9)])
 def test_stream_fused_output_and_generic_paths(F, Nf, Nt):
-    """The statistics come from the output kernel (fused output, Nt > 16) or
-    from colstats over each batch (generic path, Nf not a power of two)."""
+    """The statistics come from the output kernel (fused output, Nt > 16;
+    at 2^21 its column DFTs have N1 = 1024, two rows per hit tile) or from
+    colstats over each batch (generic path, Nf not a power of two)."""
     r0, dr, Nr, Nrb = -2.0, 0.125, 33, 11
     specs = [synth.spectrogram(Nf, Nt, [(Nf * 0.3, 1.5 - i, 30.0), (Nf * 0.7, -1.0, 25.0)], seed=400 + i)
              for i in range(2)]
