@@ -8,7 +8,6 @@
 
 #include "fft_butterflies.h"
 #include "zdt_internal.h"
-#include "zdt_kernels.h"
 
 namespace zdt {
 
