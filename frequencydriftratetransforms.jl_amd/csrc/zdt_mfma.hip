@@ -269,7 +269,9 @@ __global__ __launch_bounds__(512, 1) void k1_input_mfma(K1Args a) {
 
 // ---------------------------------------------------------------- k3 (output!)
 
-template <int PREC>
+// TMAX: also write the FDR's run maxima (K3Args::smax; a separate instance,
+// so that the plain one keeps its registers)
+template <int PREC, bool TMAX>
 __global__ __launch_bounds__(512, 1) void k3_output_mfma(K3Args a) {
   constexpr int N2 = 1024;
   __shared__ c32 ex[32 * MF_PITCH];
@@ -371,7 +373,7 @@ __global__ __launch_bounds__(512, 1) void k3_output_mfma(K3Args a) {
       f32x16 yr, yi;
       dft32_tile<PREC>(y, A, sc, yr, yi);
       typedef float f2v __attribute__((ext_vector_type(2)));
-      float keep = 0.0f;  // run maximum of step v = c (lanes of a row share mp and h)
+      float keep = 0.0f;  // TMAX: run maximum of step v = c (lanes of a row share mp and h)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const float fr = yr[v] * isc, fi = yi[v] * isc;
@@ -385,12 +387,12 @@ __global__ __launch_bounds__(512, 1) void k3_output_mfma(K3Args a) {
         // the FDR is not read again (see k3_output)
         const f2v val = {fr, fi};
         __builtin_nontemporal_store(val, reinterpret_cast<f2v*>(out + ((mp + 32u * rp) << a.logN1) + n1));
-        if (a.smax != nullptr) {  // uniform
+        if constexpr (TMAX) {
           const float m = row16_max(fmaxf(fr, fi));
           keep = c == v ? m : keep;
         }
       }
-      if (a.smax != nullptr)  // the run n1 = 16 nb + c at n2 = mp + 32 rp: smax[j][nb][n2]
+      if constexpr (TMAX)  // the run n1 = 16 nb + c at n2 = mp + 32 rp: smax[j][nb][n2]
         a.smax[(static_cast<int64_t>(j) * (int64_t(1) << (a.logN1 - 4)) + nb) * N2 + mp + 32u * mf_drow(c, h)] = keep;
     }
     if (a.part != nullptr) {
@@ -420,10 +422,14 @@ int launch_k1_mfma(const FastPlan* p, const K1Args& a, hipStream_t st) {
 int launch_k3_mfma(const FastPlan* p, const K3Args& a, hipStream_t st) {
   if (!mfma_supported(p)) return fail(ZDT_ENOTSUP, "split-precision column FFTs need N2 = 1024");
   const unsigned grid = static_cast<unsigned>(std::min<int64_t>(a.ntiles, p->resident_blocks / 2));
-  if (p->precision == ZDT_PRECISION_SPLIT_BF16)
-    k3_output_mfma<ZDT_PRECISION_SPLIT_BF16><<<grid, 512, 0, st>>>(a);
-  else
-    k3_output_mfma<ZDT_PRECISION_SPLIT_FP16><<<grid, 512, 0, st>>>(a);
+  const bool bf16 = p->precision == ZDT_PRECISION_SPLIT_BF16;
+  if (a.smax != nullptr) {
+    if (bf16) k3_output_mfma<ZDT_PRECISION_SPLIT_BF16, true><<<grid, 512, 0, st>>>(a);
+    else k3_output_mfma<ZDT_PRECISION_SPLIT_FP16, true><<<grid, 512, 0, st>>>(a);
+  } else {
+    if (bf16) k3_output_mfma<ZDT_PRECISION_SPLIT_BF16, false><<<grid, 512, 0, st>>>(a);
+    else k3_output_mfma<ZDT_PRECISION_SPLIT_FP16, false><<<grid, 512, 0, st>>>(a);
+  }
   ZDT_CHECK_LAUNCH("k3_output_mfma");
   return ZDT_OK;
 }
