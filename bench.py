@@ -8,11 +8,15 @@ One step = zdtfdr!(dest, ws, spectrogram) = input! + one ZDT batch writing the
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 100] [--no-cpu-baseline]
     python bench.py --ragged [--channels 64] [--steps 20] [--warmup 5]
-    python bench.py --long [--spectrograms 4]
+    python bench.py --long [--spectrograms 6] [--precision fp32|split-bf16|split-fp16]
+    python bench.py --cluster [--steps 20] [--warmup 5]
+    python bench.py --copy-calibration | --valu-calibration | --lds-calibration
 
 The second form times the ragged multi-channel plan (one hipGraph launch
 over 64 channels of uneven Nf and Nt), the third the long-integration
-search streamed from pinned host memory; each prints its own JSON line.
+search streamed from pinned host memory, the fourth the proto-hit
+clustering on the workload's FDR, the last the device measurements behind
+the rooflines; each prints its own JSON line.
 
 Prints ONE JSON line.  value = FDR cells/s = Nf*Nr / (seconds per step).
 The default 100 warm-up steps (~45 ms) let the shader clock settle: the
