@@ -147,11 +147,13 @@ __global__ __launch_bounds__(CL_THREADS) void cl_link(LinkArgs a) {
     return;
   }
   const int64_t J = h / a.Nf, i = h - J * a.Nf;
-  // the same column: channels i+1 .. i+di are the next entries
+  // the same column: channels i+1 .. i+di are the next entries (an entry not
+  // above h is out of order, flagged by its own thread, and ends the walk, so
+  // that unsorted input costs at most di steps here too)
   const int64_t colend = (J + 1) * a.Nf;
   for (int64_t m = k + 1; m < a.n; ++m) {
     const int64_t g = a.hits[m];
-    if (g > h + a.di || g >= colend) break;
+    if (g <= h || g > h + a.di || g >= colend) break;
     uf_union(a.parent, static_cast<int>(k), static_cast<int>(m));
   }
   // columns J+1 .. J+dj: channels max(0, i-di) .. min(Nf-1, i+di), one
@@ -357,8 +359,11 @@ __global__ __launch_bounds__(CL_THREADS) void cl_accumulate(AccumArgs a) {
   }
 }
 
-// Decode the peak keys of the first min(count, cap) hits in place.
-__global__ __launch_bounds__(CL_THREADS) void cl_finish(const int64_t* __restrict__ hits,
+// Decode the peak keys of the first min(count, cap) hits in place.  A cluster
+// whose only member is a proto-hit cl_link flagged (out of range) received no
+// key: its key is still 0, which decodes to an ordinal past n; it gets peak -1
+// and a NaN value (the call reports the input error).
+__global__ __launch_bounds__(CL_THREADS) void cl_finish(const int64_t* __restrict__ hits, int64_t n,
                                                        const int64_t* __restrict__ flags, int64_t cap,
                                                        int64_t* __restrict__ peak, float* __restrict__ value) {
   const int64_t c = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
@@ -366,6 +371,11 @@ __global__ __launch_bounds__(CL_THREADS) void cl_finish(const int64_t* __restric
   if (c >= ncl) return;
   const unsigned long long kk = static_cast<unsigned long long>(peak[c]);
   const int64_t k = static_cast<int64_t>(0xffffffffull - (kk & 0xffffffffull));
+  if (k >= n) {
+    peak[c] = -1;
+    value[c] = __uint_as_float(0x7fc00000u);
+    return;
+  }
   peak[c] = hits[k];
   value[c] = unordered_bits(static_cast<unsigned>(kk >> 32));
 }
@@ -433,7 +443,8 @@ int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* f
       ZDT_CHECK_LAUNCH("cl_accumulate");
     }
     if (cap > 0) {
-      cl_finish<<<blocks_for(cap < n ? cap : n, CL_THREADS), CL_THREADS, 0, st>>>(hits, flags_dev, cap, peak, value);
+      cl_finish<<<blocks_for(cap < n ? cap : n, CL_THREADS), CL_THREADS, 0, st>>>(hits, n, flags_dev, cap, peak,
+                                                                                  value);
       ZDT_CHECK_LAUNCH("cl_finish");
     }
     ZDT_TRY_HIP(hipMemcpyAsync(flags, flags_dev, 16, hipMemcpyDeviceToHost, st));

@@ -146,6 +146,15 @@ def test_abi_errors(F):
     bad = lin.clone()
     bad[-1] = 4096 * 33
     assert call(bad.data_ptr(), len(bad), cap) == _lib.ZDT_EINVAL
+    # out of range first: that proto-hit is cluster 0, within cap, with no
+    # value to take a peak from; its slot reads peak -1
+    bad[-1] = lin[-1]
+    bad[0] = -1
+    assert call(bad.data_ptr(), len(bad), cap) == _lib.ZDT_EINVAL
+    assert peak[0].item() == -1 and size[0].item() == 0
+    # past the end and out of order
+    bad[0] = 4096 * 33 + 5
+    assert call(bad.data_ptr(), len(bad), cap) == _lib.ZDT_EINVAL and peak[0].item() == -1
     assert call(lin.data_ptr(), len(lin), cap, di=-1) == _lib.ZDT_EINVAL
     torch.cuda.synchronize()
 
