@@ -270,7 +270,7 @@ struct AccumArgs {
   int64_t Nf, Nr, ld;
   int64_t ncol;
   Mats mats;
-  const int* parent;
+  const int* parent;  // flattened roots; null: cid is per proto-hit (the one-workgroup path)
   const int* cid;
   int64_t cap;
   int64_t* peak;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(CL_THREADS) void cl_accumulate(AccumArgs a) {
   if (k < a.n) {
     const int64_t h = a.hits[k];
     if (h >= 0 && h < a.Nf * a.ncol) {  // flagged by cl_link otherwise
-      c = a.cid[a.parent[k]];
+      c = a.parent ? a.cid[a.parent[k]] : a.cid[k];
       if (a.label) a.label[k] = c;
       if (c < a.cap) {
         const int64_t JJ = h / a.Nf, ii = h - JJ * a.Nf;
@@ -359,6 +359,192 @@ __global__ __launch_bounds__(CL_THREADS) void cl_accumulate(AccumArgs a) {
   }
 }
 
+// ---- one workgroup for few proto-hits
+//
+// Up to CL_SMALL_MAX proto-hits (and Nf * ncol < 2^31) are clustered by one
+// workgroup with the proto-hits and the parents in LDS: the same union-find,
+// neighbour rule and numbering as cl_init .. cl_number_roots, in one launch,
+// with every root walk and search on LDS instead of device-scope memory.
+// The neighbours in column J+d are found by one binary search over the rest
+// of the sorted list (no column table).  It writes cid[k] = the hit ordinal
+// of every proto-hit (cl_accumulate then reads it without the parent array),
+// clears the accumulators of the hits below cap and writes the count, so that
+// cl_accumulate and cl_finish complete it as they complete the general path.
+// Input that cl_link would flag sets the error flag and gives every proto-hit
+// the ordinal INT_MAX and a count of 0, so that the kernels after it touch
+// nothing within cap.
+constexpr int CL_SMALL_THREADS = 1024;
+constexpr int CL_SMALL_MAX = 16384;
+
+__device__ __forceinline__ int lds_ld(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_st(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ int lds_find(int* parent, int x) {
+  int cur = lds_ld(parent + x);
+  if (cur == x) return x;
+  int prev = x;
+  for (;;) {
+    const int next = lds_ld(parent + cur);
+    if (next == cur) return cur;
+    lds_st(parent + prev, next);
+    prev = cur;
+    cur = next;
+  }
+}
+__device__ void lds_union(int* parent, int a, int b) {
+  int ra = lds_find(parent, a), rb = lds_find(parent, b);
+  while (ra != rb) {
+    if (ra > rb) {
+      const int t = ra;
+      ra = rb;
+      rb = t;
+    }
+    int expect = rb;
+    if (__hip_atomic_compare_exchange_strong(parent + rb, &expect, ra, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      return;
+    ra = lds_find(parent, ra);
+    rb = lds_find(parent, expect);
+  }
+}
+
+struct SmallArgs {
+  const int64_t* hits;
+  int n;
+  int64_t Nf;
+  int64_t ncol;
+  int di, dj;
+  int* cid;  // out: hit ordinal of each proto-hit
+  int64_t* flags;
+  int64_t cap;
+  int64_t* peak;
+  int64_t* size;
+  int64_t* box;
+};
+
+__global__ __launch_bounds__(CL_SMALL_THREADS) void cl_small(SmallArgs a) {
+  __shared__ int hv[CL_SMALL_MAX];   // the proto-hits, as int32 linear indices
+  __shared__ int par[CL_SMALL_MAX];  // the forest; after numbering, -(ordinal + 1) at the roots
+  __shared__ int bad;
+  __shared__ int wsum[CL_SMALL_THREADS / 64 + 1];
+  const int n = a.n, tid = threadIdx.x;
+  const int64_t lim = a.Nf * a.ncol;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  {
+    // every load of this thread in flight at once (at most 16 per thread)
+    constexpr int PT = CL_SMALL_MAX / CL_SMALL_THREADS;
+    int64_t hh[PT];
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const int k = tid + q * CL_SMALL_THREADS;
+      hh[q] = k < n ? a.hits[k] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const int k = tid + q * CL_SMALL_THREADS;
+      if (k < n) {
+        const int64_t h = hh[q];
+        if (h < 0 || h >= lim) lds_st(&bad, 1);
+        hv[k] = static_cast<int>(h < 0 ? 0 : (h >= lim ? lim - 1 : h));
+        par[k] = k;
+      }
+    }
+  }
+  __syncthreads();
+  // order, on the LDS copy: exact once every entry is in range
+  if (!bad)
+    for (int k = tid + 1; k < n; k += CL_SMALL_THREADS)
+      if (hv[k - 1] >= hv[k]) lds_st(&bad, 1);
+  __syncthreads();
+  if (bad) {  // uniform
+    for (int k = tid; k < n; k += CL_SMALL_THREADS) a.cid[k] = INT_MAX;
+    if (tid == 0) {
+      a.flags[0] = 0;
+      a.flags[1] = 1;
+    }
+    return;
+  }
+  const int Nf = static_cast<int>(a.Nf), ncol = static_cast<int>(a.ncol);
+  for (int k = tid; k < n; k += CL_SMALL_THREADS) {
+    const int h = hv[k];
+    const int J = h / Nf, i = h - J * Nf;
+    // the same column: the next entries up to channel i + di
+    const int colend = (J + 1) * Nf;  // <= Nf * ncol < 2^31
+    for (int m = k + 1; m < n; ++m) {
+      const int g = hv[m];
+      if (g - h > a.di || g >= colend) break;
+      lds_union(par, k, m);
+    }
+    // columns J+1 .. J+dj: channels max(0, i-di) .. min(Nf-1, i+di)
+    const int ilo = i > a.di ? i - a.di : 0;
+    const int ihi = i < Nf - 1 - a.di ? i + a.di : Nf - 1;
+    for (int d = 1; d <= a.dj && J + d < ncol; ++d) {
+      const int lo = (J + d) * Nf + ilo, hi = (J + d) * Nf + ihi;
+      int L = k + 1, R = n;
+      while (L < R) {
+        const int mid = (L + R) >> 1;
+        if (hv[mid] < lo) L = mid + 1;
+        else R = mid;
+      }
+      for (int m = L; m < n && hv[m] <= hi; ++m) lds_union(par, k, m);
+    }
+  }
+  __syncthreads();
+  // roots for every entry
+  for (int k = tid; k < n; k += CL_SMALL_THREADS) {
+    int r = lds_ld(par + k);
+    for (int next = lds_ld(par + r); next != r; next = lds_ld(par + r)) r = next;
+    lds_st(par + k, r);
+  }
+  __syncthreads();
+  // number the roots in order: thread t takes the entries [t*per, (t+1)*per)
+  const int per = (n + CL_SMALL_THREADS - 1) / CL_SMALL_THREADS;
+  const int k0 = tid * per, k1 = min(n, k0 + per);
+  int s = 0;
+  for (int k = k0; k < k1; ++k) s += par[k] == k ? 1 : 0;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int incl = wave_incl_scan_i(s, lane);
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    int t = 0;
+    for (int w = 0; w < CL_SMALL_THREADS / 64; ++w) {
+      const int v = wsum[w];
+      wsum[w] = t;
+      t += v;
+    }
+    wsum[CL_SMALL_THREADS / 64] = t;
+  }
+  __syncthreads();
+  int c = wsum[wv] + incl - s;
+  for (int k = k0; k < k1; ++k) {
+    if (par[k] != k) continue;
+    if (c < a.cap) {
+      a.peak[c] = 0;
+      a.size[c] = 0;
+      a.box[4 * c + 0] = INT64_MAX;
+      a.box[4 * c + 1] = 0;
+      a.box[4 * c + 2] = hv[k] / Nf;
+      a.box[4 * c + 3] = 0;
+    }
+    par[k] = -(c + 1);  // read only after the barrier below
+    ++c;
+  }
+  if (tid == 0) {
+    a.flags[0] = wsum[CL_SMALL_THREADS / 64];
+    a.flags[1] = 0;
+  }
+  __syncthreads();
+  for (int k = tid; k < n; k += CL_SMALL_THREADS) {
+    const int p = par[k];
+    a.cid[k] = p < 0 ? -p - 1 : -par[p] - 1;
+  }
+}
+
 // Decode the peak keys of the first min(count, cap) hits in place.  A cluster
 // whose only member is a proto-hit cl_link flagged (out of range) received no
 // key: its key is still 0, which decodes to an ordinal past n; it gets peak -1
@@ -392,16 +578,20 @@ int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* f
     return ZDT_OK;
   }
   const int64_t ncol = static_cast<int64_t>(nb) * Nr;
-  const int64_t nt = (n + CL_TILE - 1) / CL_TILE;
-  const int64_t nscan = hits_scan_scratch(nt);
+  // few proto-hits: one workgroup in LDS (cl_small) instead of cl_init ..
+  // cl_number_roots and the scan
+  const bool small = n <= CL_SMALL_MAX && Nf * ncol < (int64_t(1) << 31);
+  const int64_t nt = small ? 0 : (n + CL_TILE - 1) / CL_TILE;
+  const int64_t nscan = small ? 0 : hits_scan_scratch(nt);
   // one scratch block, no other allocation and one synchronization (at the
   // end, for the count and the error flag):
   //   parent[n], cid[n] (int); counts[nt] + scan scratch, flags[2],
-  //   colstart[ncol + 1] (dj > 0), the matrix table (nb > 1) (int64 words)
+  //   colstart[ncol + 1] (dj > 0, general path), the matrix table (nb > 1)
+  //   (int64 words)
   auto up = [](int64_t bytes) { return (bytes + 255) & ~int64_t(255); };
   const int64_t o_cid = up(4 * n), o_w = o_cid + up(4 * n);
   const int64_t o_flags = o_w + 8 * (nt + nscan), o_col = o_flags + 16;
-  const int64_t o_tab = o_col + (dj > 0 ? 8 * (ncol + 1) : 0);
+  const int64_t o_tab = o_col + (dj > 0 && !small ? 8 * (ncol + 1) : 0);
   const int64_t bytes = o_tab + (nb > 1 ? 8 * static_cast<int64_t>(nb) : 0);
   char* buf = nullptr;
   ZDT_TRY_HIP(hipMallocAsync(reinterpret_cast<void**>(&buf), static_cast<size_t>(bytes), st));
@@ -409,36 +599,43 @@ int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* f
   int* cid = reinterpret_cast<int*>(buf + o_cid);
   int64_t* counts = reinterpret_cast<int64_t*>(buf + o_w);
   int64_t* flags_dev = reinterpret_cast<int64_t*>(buf + o_flags);
-  int64_t* colstart = dj > 0 ? reinterpret_cast<int64_t*>(buf + o_col) : nullptr;
+  int64_t* colstart = dj > 0 && !small ? reinterpret_cast<int64_t*>(buf + o_col) : nullptr;
   Mats mats{fdrs[0], nb > 1 ? reinterpret_cast<const float* const*>(buf + o_tab) : nullptr};
   int64_t flags[2] = {0, 0};
   auto run = [&]() -> int {
     if (nb > 1)
       ZDT_TRY_HIP(hipMemcpyAsync(buf + o_tab, fdrs, sizeof(float*) * nb, hipMemcpyHostToDevice, st));
-    {
-      ProfScope ps(st, "cluster_link", 20.0 * n);
-      InitArgs ia{hits, n, Nf, ncol, parent, colstart, flags_dev};
-      cl_init<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(ia);
-      ZDT_CHECK_LAUNCH("cl_init");
-      LinkArgs a{hits, n, Nf, ncol, di, dj, parent, colstart, flags_dev};
-      cl_link<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
-      ZDT_CHECK_LAUNCH("cl_link");
-    }
-    {
-      ProfScope ps(st, "cluster_roots", 8.0 * n);
-      cl_flatten_count<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(parent, n, counts);
-      ZDT_CHECK_LAUNCH("cl_flatten_count");
-    }
-    ZDT_TRY(launch_hits_scan(counts, nt, flags_dev, counts + nt, st));
-    {
-      ProfScope ps(st, "cluster_number", 12.0 * n);
-      NumberArgs a{hits, n, Nf, parent, counts, cid, cap, peak, size, box};
-      cl_number_roots<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(a);
-      ZDT_CHECK_LAUNCH("cl_number_roots");
+    if (small) {
+      ProfScope ps(st, "cluster_small", 16.0 * n);
+      SmallArgs sa{hits, static_cast<int>(n), Nf, ncol, di, dj, cid, flags_dev, cap, peak, size, box};
+      cl_small<<<1, CL_SMALL_THREADS, 0, st>>>(sa);
+      ZDT_CHECK_LAUNCH("cl_small");
+    } else {
+      {
+        ProfScope ps(st, "cluster_link", 20.0 * n);
+        InitArgs ia{hits, n, Nf, ncol, parent, colstart, flags_dev};
+        cl_init<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(ia);
+        ZDT_CHECK_LAUNCH("cl_init");
+        LinkArgs a{hits, n, Nf, ncol, di, dj, parent, colstart, flags_dev};
+        cl_link<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
+        ZDT_CHECK_LAUNCH("cl_link");
+      }
+      {
+        ProfScope ps(st, "cluster_roots", 8.0 * n);
+        cl_flatten_count<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(parent, n, counts);
+        ZDT_CHECK_LAUNCH("cl_flatten_count");
+      }
+      ZDT_TRY(launch_hits_scan(counts, nt, flags_dev, counts + nt, st));
+      {
+        ProfScope ps(st, "cluster_number", 12.0 * n);
+        NumberArgs a{hits, n, Nf, parent, counts, cid, cap, peak, size, box};
+        cl_number_roots<<<static_cast<unsigned>(nt), CL_THREADS, 0, st>>>(a);
+        ZDT_CHECK_LAUNCH("cl_number_roots");
+      }
     }
     if (cap > 0 || label != nullptr) {
       ProfScope ps(st, "cluster_reduce", 28.0 * n);
-      AccumArgs a{hits, n, Nf, Nr, ld, ncol, mats, parent, cid, cap, peak, size, box, label};
+      AccumArgs a{hits, n, Nf, Nr, ld, ncol, mats, small ? nullptr : parent, cid, cap, peak, size, box, label};
       cl_accumulate<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
       ZDT_CHECK_LAUNCH("cl_accumulate");
     }

@@ -125,7 +125,8 @@ def test_abi_errors(F):
     val = torch.zeros(cap, dtype=torch.float32, device="cuda")
     size = torch.zeros(cap, dtype=torch.int64, device="cuda")
     box = torch.zeros(4 * cap, dtype=torch.int64, device="cuda")
-    lab = torch.zeros(len(lin), dtype=torch.int64, device="cuda")
+    big = 20000  # above the one-workgroup path's 16384 proto-hits
+    lab = torch.zeros(max(len(lin), big), dtype=torch.int64, device="cuda")
     n = C.c_int64()
 
     def call(hp, cnt, capn, di=1, dj=1):
@@ -136,7 +137,7 @@ def test_abi_errors(F):
     assert call(lin.data_ptr(), len(lin), cap) == _lib.ZDT_ETRUNC and n.value == Hn
     assert np.array_equal(size.cpu().numpy(), hits.size[:cap])
     assert np.array_equal(box.cpu().numpy().reshape(cap, 4) + 1, hits.box[:cap])
-    assert np.array_equal(lab.cpu().numpy() + 1, hits.labels)
+    assert np.array_equal(lab[:len(lin)].cpu().numpy() + 1, hits.labels)
     # count only
     assert call(lin.data_ptr(), len(lin), 0) == _lib.ZDT_ETRUNC and n.value == Hn
     assert call(lin.data_ptr(), 0, cap) == _lib.ZDT_OK and n.value == 0
@@ -146,15 +147,21 @@ def test_abi_errors(F):
     bad = lin.clone()
     bad[-1] = 4096 * 33
     assert call(bad.data_ptr(), len(bad), cap) == _lib.ZDT_EINVAL
-    # out of range first: that proto-hit is cluster 0, within cap, with no
-    # value to take a peak from; its slot reads peak -1
-    bad[-1] = lin[-1]
-    bad[0] = -1
-    assert call(bad.data_ptr(), len(bad), cap) == _lib.ZDT_EINVAL
-    assert peak[0].item() == -1 and size[0].item() == 0
-    # past the end and out of order
-    bad[0] = 4096 * 33 + 5
-    assert call(bad.data_ptr(), len(bad), cap) == _lib.ZDT_EINVAL and peak[0].item() == -1
+    # out of range first, on the one-workgroup path (this list) and on the
+    # general kernels (a list of 20000); on the general kernels that
+    # proto-hit is cluster 0, within cap, with no value to take a peak from,
+    # and its slot reads peak -1
+    wide = torch.arange(0, 3 * big, 3, dtype=torch.int64, device="cuda")
+    for b in (bad, wide):
+        b[-1] = b[-2] + 1
+        for first in (-1, 4096 * 33 + 5):  # negative; past the end and out of order
+            b[0] = first
+            assert call(b.data_ptr(), len(b), cap) == _lib.ZDT_EINVAL
+            if b is wide:
+                assert peak[0].item() == -1 and size[0].item() == 0
+    # the general kernels on valid input
+    wide[0] = 0
+    assert call(wide.data_ptr(), big, 0) == _lib.ZDT_ETRUNC and n.value > 0
     assert call(lin.data_ptr(), len(lin), cap, di=-1) == _lib.ZDT_EINVAL
     torch.cuda.synchronize()
 
