@@ -277,6 +277,7 @@ struct AccumArgs {
   int64_t* size;
   int64_t* box;
   int64_t* label;  // per proto-hit cluster ordinal, or null
+  const int64_t* done;  // non-zero: cl_small has reduced the hits already (null on the general path)
 };
 
 // Order-preserving map of a float onto an unsigned integer.
@@ -297,6 +298,7 @@ __device__ __forceinline__ float unordered_bits(unsigned o) {
 // same either way.  The peak is reduced as a key (value bits, then the
 // reversed ordinal) in the peak array itself until cl_finish decodes it.
 __global__ __launch_bounds__(CL_THREADS) void cl_accumulate(AccumArgs a) {
+  if (a.done != nullptr && *a.done != 0) return;  // uniform
   const int64_t k = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(a.peak);
   bool live = false;
@@ -366,15 +368,20 @@ __global__ __launch_bounds__(CL_THREADS) void cl_accumulate(AccumArgs a) {
 // neighbour rule and numbering as cl_init .. cl_number_roots, in one launch,
 // with every root walk and search on LDS instead of device-scope memory.
 // The neighbours in column J+d are found by one binary search over the rest
-// of the sorted list (no column table).  It writes cid[k] = the hit ordinal
-// of every proto-hit (cl_accumulate then reads it without the parent array),
-// clears the accumulators of the hits below cap and writes the count, so that
+// of the sorted list (no column table).  With at most CL_SMALL_ACC hits, the
+// parents' LDS then takes the hits' accumulators: the same keyed peak, size
+// and box reductions as cl_accumulate, on LDS atomics, and the workgroup
+// writes the hits and labels itself and sets flags[2], on which cl_accumulate
+// and cl_finish return at once.  With more hits it writes cid[k] = the hit
+// ordinal of every proto-hit (cl_accumulate then reads it without the parent
+// array) and clears the accumulators of the hits below cap, so that
 // cl_accumulate and cl_finish complete it as they complete the general path.
 // Input that cl_link would flag sets the error flag and gives every proto-hit
 // the ordinal INT_MAX and a count of 0, so that the kernels after it touch
 // nothing within cap.
 constexpr int CL_SMALL_THREADS = 1024;
 constexpr int CL_SMALL_MAX = 16384;
+constexpr int CL_SMALL_ACC = CL_SMALL_MAX * 4 / 24;  // hits whose accumulators fit in the parents' LDS
 
 __device__ __forceinline__ int lds_ld(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -417,17 +424,25 @@ struct SmallArgs {
   int64_t Nf;
   int64_t ncol;
   int di, dj;
-  int* cid;  // out: hit ordinal of each proto-hit
-  int64_t* flags;
+  int* cid;        // out: hit ordinal of each proto-hit (when the reductions do not fold)
+  int64_t* flags;  // [0] count, [1] input error, [2] 1 when the reductions folded
   int64_t cap;
   int64_t* peak;
+  float* value;
   int64_t* size;
   int64_t* box;
+  int64_t* label;
+  int Nr;
+  int64_t ld;
+  Mats mats;
+  int want;  // cap > 0 or labels asked for
 };
 
 __global__ __launch_bounds__(CL_SMALL_THREADS) void cl_small(SmallArgs a) {
   __shared__ int hv[CL_SMALL_MAX];   // the proto-hits, as int32 linear indices
-  __shared__ int par[CL_SMALL_MAX];  // the forest; after numbering, -(ordinal + 1) at the roots
+  // the forest; after numbering, -(ordinal + 1) at the roots; at the end,
+  // the hits' accumulators
+  __shared__ __align__(8) int par[CL_SMALL_MAX];
   __shared__ int bad;
   __shared__ int wsum[CL_SMALL_THREADS / 64 + 1];
   const int n = a.n, tid = threadIdx.x;
@@ -465,6 +480,7 @@ __global__ __launch_bounds__(CL_SMALL_THREADS) void cl_small(SmallArgs a) {
     if (tid == 0) {
       a.flags[0] = 0;
       a.flags[1] = 1;
+      a.flags[2] = 0;
     }
     return;
   }
@@ -520,28 +536,101 @@ __global__ __launch_bounds__(CL_SMALL_THREADS) void cl_small(SmallArgs a) {
     wsum[CL_SMALL_THREADS / 64] = t;
   }
   __syncthreads();
+  const int C = wsum[CL_SMALL_THREADS / 64];
+  // the hits' reductions fold into this workgroup when their accumulators
+  // (24 bytes a hit) fit where the parents were; otherwise cl_accumulate and
+  // cl_finish run them
+  const bool fold = a.want && C <= CL_SMALL_ACC;
   int c = wsum[wv] + incl - s;
   for (int k = k0; k < k1; ++k) {
     if (par[k] != k) continue;
     if (c < a.cap) {
-      a.peak[c] = 0;
-      a.size[c] = 0;
-      a.box[4 * c + 0] = INT64_MAX;
-      a.box[4 * c + 1] = 0;
-      a.box[4 * c + 2] = hv[k] / Nf;
-      a.box[4 * c + 3] = 0;
+      a.box[4 * c + 2] = hv[k] / Nf;  // a root is its hit's first proto-hit: the column minimum
+      if (!fold) {
+        a.peak[c] = 0;
+        a.size[c] = 0;
+        a.box[4 * c + 0] = INT64_MAX;
+        a.box[4 * c + 1] = 0;
+        a.box[4 * c + 3] = 0;
+      }
     }
     par[k] = -(c + 1);  // read only after the barrier below
     ++c;
   }
   if (tid == 0) {
-    a.flags[0] = wsum[CL_SMALL_THREADS / 64];
+    a.flags[0] = C;
     a.flags[1] = 0;
+    a.flags[2] = fold ? 1 : 0;
   }
   __syncthreads();
-  for (int k = tid; k < n; k += CL_SMALL_THREADS) {
-    const int p = par[k];
-    a.cid[k] = p < 0 ? -p - 1 : -par[p] - 1;
+  constexpr int PT = CL_SMALL_MAX / CL_SMALL_THREADS;
+  int cc[PT];
+#pragma unroll
+  for (int q = 0; q < PT; ++q) {
+    const int k = tid + q * CL_SMALL_THREADS;
+    cc[q] = 0;
+    if (k < n) {
+      const int p = par[k];
+      cc[q] = p < 0 ? -p - 1 : -par[p] - 1;
+      if (!fold) a.cid[k] = cc[q];
+    }
+  }
+  if (!fold) return;  // uniform
+  __syncthreads();    // every read of the parents is done: their LDS takes the accumulators
+  unsigned long long* akey = reinterpret_cast<unsigned long long*>(par);
+  unsigned* asz = reinterpret_cast<unsigned*>(akey + C);
+  int* aimn = reinterpret_cast<int*>(asz + C);
+  int* aimx = aimn + C;
+  int* ajmx = aimx + C;
+  for (int q = tid; q < C; q += CL_SMALL_THREADS) {
+    akey[q] = 0ull;
+    asz[q] = 0u;
+    aimn[q] = INT_MAX;
+    aimx[q] = 0;
+    ajmx[q] = 0;
+  }
+  // every FDR value of this thread in flight at once
+  float vv[PT];
+#pragma unroll
+  for (int q = 0; q < PT; ++q) {
+    const int k = tid + q * CL_SMALL_THREADS;
+    vv[q] = 0.0f;
+    if (k < n) {
+      const int h = hv[k];
+      const int J = h / Nf, i = h - J * Nf;
+      const int b = J / a.Nr, j = J - b * a.Nr;
+      vv[q] = a.mats.get(b)[i + a.ld * j];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PT; ++q) {
+    const int k = tid + q * CL_SMALL_THREADS;
+    if (k < n) {
+      const int h = hv[k];
+      const int J = h / Nf, i = h - J * Nf;
+      const int cl = cc[q];
+      // peak: the largest value, ties to the smaller ordinal (the earlier proto-hit)
+      atomicMax(akey + cl, (static_cast<unsigned long long>(ordered_bits(vv[q])) << 32) |
+                               (0xffffffffull - static_cast<unsigned long long>(k)));
+      atomicAdd(asz + cl, 1u);
+      atomicMin(aimn + cl, i);
+      atomicMax(aimx + cl, i);
+      atomicMax(ajmx + cl, J);
+      if (a.label) a.label[k] = cl;
+    }
+  }
+  __syncthreads();
+  const int nout = a.cap < C ? static_cast<int>(a.cap) : C;
+  for (int q = tid; q < nout; q += CL_SMALL_THREADS) {
+    const unsigned long long key = akey[q];
+    const int kk = static_cast<int>(0xffffffffull - (key & 0xffffffffull));  // every hit has a member
+    a.peak[q] = hv[kk];
+    a.value[q] = unordered_bits(static_cast<unsigned>(key >> 32));
+    a.size[q] = asz[q];
+    a.box[4 * q + 0] = aimn[q];
+    a.box[4 * q + 1] = aimx[q];
+    a.box[4 * q + 3] = ajmx[q];
   }
 }
 
@@ -551,7 +640,9 @@ __global__ __launch_bounds__(CL_SMALL_THREADS) void cl_small(SmallArgs a) {
 // and a NaN value (the call reports the input error).
 __global__ __launch_bounds__(CL_THREADS) void cl_finish(const int64_t* __restrict__ hits, int64_t n,
                                                        const int64_t* __restrict__ flags, int64_t cap,
-                                                       int64_t* __restrict__ peak, float* __restrict__ value) {
+                                                       int64_t* __restrict__ peak, float* __restrict__ value,
+                                                       const int64_t* __restrict__ done) {
+  if (done != nullptr && *done != 0) return;  // uniform: cl_small wrote the hits
   const int64_t c = static_cast<int64_t>(blockIdx.x) * CL_THREADS + threadIdx.x;
   const int64_t ncl = flags[0] < cap ? flags[0] : cap;
   if (c >= ncl) return;
@@ -585,12 +676,12 @@ int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* f
   const int64_t nscan = small ? 0 : hits_scan_scratch(nt);
   // one scratch block, no other allocation and one synchronization (at the
   // end, for the count and the error flag):
-  //   parent[n], cid[n] (int); counts[nt] + scan scratch, flags[2],
+  //   parent[n], cid[n] (int); counts[nt] + scan scratch, flags[3],
   //   colstart[ncol + 1] (dj > 0, general path), the matrix table (nb > 1)
   //   (int64 words)
   auto up = [](int64_t bytes) { return (bytes + 255) & ~int64_t(255); };
   const int64_t o_cid = up(4 * n), o_w = o_cid + up(4 * n);
-  const int64_t o_flags = o_w + 8 * (nt + nscan), o_col = o_flags + 16;
+  const int64_t o_flags = o_w + 8 * (nt + nscan), o_col = o_flags + 24;
   const int64_t o_tab = o_col + (dj > 0 && !small ? 8 * (ncol + 1) : 0);
   const int64_t bytes = o_tab + (nb > 1 ? 8 * static_cast<int64_t>(nb) : 0);
   char* buf = nullptr;
@@ -607,7 +698,8 @@ int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* f
       ZDT_TRY_HIP(hipMemcpyAsync(buf + o_tab, fdrs, sizeof(float*) * nb, hipMemcpyHostToDevice, st));
     if (small) {
       ProfScope ps(st, "cluster_small", 16.0 * n);
-      SmallArgs sa{hits, static_cast<int>(n), Nf, ncol, di, dj, cid, flags_dev, cap, peak, size, box};
+      SmallArgs sa{hits, static_cast<int>(n), Nf, ncol, di, dj, cid, flags_dev, cap, peak, value, size, box, label,
+                   static_cast<int>(Nr), ld, mats, cap > 0 || label != nullptr ? 1 : 0};
       cl_small<<<1, CL_SMALL_THREADS, 0, st>>>(sa);
       ZDT_CHECK_LAUNCH("cl_small");
     } else {
@@ -635,13 +727,14 @@ int cluster_protohits_impl(const int64_t* hits, int64_t n, const float* const* f
     }
     if (cap > 0 || label != nullptr) {
       ProfScope ps(st, "cluster_reduce", 28.0 * n);
-      AccumArgs a{hits, n, Nf, Nr, ld, ncol, mats, small ? nullptr : parent, cid, cap, peak, size, box, label};
+      AccumArgs a{hits, n, Nf, Nr, ld, ncol, mats, small ? nullptr : parent, cid, cap, peak, size, box,
+                  label, small ? flags_dev + 2 : nullptr};
       cl_accumulate<<<blocks_for(n, CL_THREADS), CL_THREADS, 0, st>>>(a);
       ZDT_CHECK_LAUNCH("cl_accumulate");
     }
     if (cap > 0) {
-      cl_finish<<<blocks_for(cap < n ? cap : n, CL_THREADS), CL_THREADS, 0, st>>>(hits, n, flags_dev, cap, peak,
-                                                                                  value);
+      cl_finish<<<blocks_for(cap < n ? cap : n, CL_THREADS), CL_THREADS, 0, st>>>(
+          hits, n, flags_dev, cap, peak, value, small ? flags_dev + 2 : nullptr);
       ZDT_CHECK_LAUNCH("cl_finish");
     }
     ZDT_TRY_HIP(hipMemcpyAsync(flags, flags_dev, 16, hipMemcpyDeviceToHost, st));
