@@ -271,7 +271,9 @@ int zdt_findprotohits_noalloc(const float* const* fdrs, int nb, int64_t Nf, int6
  * label (device, n entries, or NULL): each proto-hit's hit number.
  * *nclusters receives the number of hits; ZDT_ETRUNC when it exceeds cap
  * (the first cap hits and every label are still written).  ZDT_EINVAL for an
- * index out of range or out of order, ZDT_ENOTSUP when n, Nf or nb*Nr reaches 2^31.
+ * index out of range or out of order (the outputs' contents are then
+ * unspecified; nothing is written outside them), ZDT_ENOTSUP when n, Nf or
+ * nb*Nr reaches 2^31.
  * Synchronizes the stream once (to size the outputs). */
 int zdt_clusterprotohits(const int64_t* hits, int64_t n, const float* const* fdrs, int nb,
                          int64_t Nf, int64_t Nr, int64_t ld, int di, int dj, int64_t* peak,
