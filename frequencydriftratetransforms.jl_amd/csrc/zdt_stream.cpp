@@ -49,6 +49,10 @@ struct zdt_stream {
   int64_t* counts = nullptr;  // one batch's tile counts, scan scratch, batch total, running total
   double* dstats = nullptr;   // colmean, colvar (nb*Nrb each), thr, then generic-path scratch
   void* hsmall = nullptr;     // pinned: per slot {thr (double), total (int64)}
+  // zdt_stream_findhits: the hits of one spectrogram (peak, value, size, box
+  // for ccap hits), staged on the device
+  char* dclus = nullptr;
+  int64_t ccap = 0;
 };
 namespace zdt {
 namespace {
@@ -118,6 +122,7 @@ int zdt_stream_destroy(zdt_stream* s) {
   }
   if (s->counts) (void)hipFree(s->counts);
   if (s->dstats) (void)hipFree(s->dstats);
+  if (s->dclus) (void)hipFree(s->dclus);
   if (s->hsmall) (void)hipHostFree(s->hsmall);
   if (s->readback) (void)hipStreamDestroy(s->readback);
   if (s->copy) (void)hipStreamDestroy(s->copy);
@@ -252,13 +257,28 @@ int zdt_stream_info(const zdt_stream* s, int64_t* Nrb, int64_t* nbatch, int* res
   return ZDT_OK;
 }
 
-int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, double snr, int64_t* hits,
-                      int64_t cap, int64_t* nhits, double* thr) {
-  if (s == nullptr) return fail(ZDT_EINVAL, "stream is NULL");
-  if (host_specs == nullptr || nspec < 1 || nhits == nullptr || cap < 0 || (cap > 0 && hits == nullptr))
-    return fail(ZDT_EINVAL, "stream_search: need host spectrograms, nhits and (for cap > 0) hits");
-  for (int i = 0; i < nspec; ++i)
-    if (host_specs[i] == nullptr) return fail(ZDT_EINVAL, "stream_search: spectrogram %d is NULL", i);
+}  // extern "C"
+
+namespace {
+
+// zdt_stream_findhits' outputs: per spectrogram i, hits at [i*cap ...]
+struct StreamHits {
+  int di, dj;
+  int64_t* peak;
+  float* value;
+  int64_t* size;
+  int64_t* box;
+  int64_t cap;    // hits per spectrogram in the host arrays
+  int64_t stage;  // hits the device staging holds (>= cap, >= 1)
+  int64_t* nclusters;
+};
+
+// The search pipeline of zdt_stream_search.  With cl, each spectrogram's
+// proto-hits (the first `cap` of them staged on the device) are clustered
+// right after their extraction, while every batch's FDR is still resident,
+// and hits (the host proto-hit array) may be NULL.
+int stream_run(zdt_stream* s, const float* const* host_specs, int nspec, double snr, int64_t* hits, int64_t cap,
+               int64_t* nhits, double* thr, const StreamHits* cl) {
   ZDT_TRY_HIP(hipSetDevice(s->device));
   const size_t spec_bytes = static_cast<size_t>(s->Nf * s->Nt) * 4;
   const int64_t Nf = s->Nf, Nrb = s->Nrb, nb = s->nb;
@@ -345,15 +365,48 @@ int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, 
     std::memcpy(&total, hs + 8, sizeof(int64_t));
     if (thr) thr[i] = t;
     nhits[i] = total;
-    const int64_t got = std::min(total, cap);
+    const int64_t got = hits ? std::min(total, cap) : 0;
     if (got > 0) {
       ZDT_TRY_HIP(hipMemcpyAsync(hits + static_cast<int64_t>(i) * cap, s->dhits[slot], static_cast<size_t>(got) * 8,
                                  hipMemcpyDeviceToHost, s->readback));
       ZDT_TRY_HIP(hipStreamSynchronize(s->readback));
     }
     if (total > cap)
-      rc_trunc = fail(ZDT_ETRUNC, "stream_search: spectrogram %d has %lld hits, capacity %lld", i, (long long)total,
-                      (long long)cap);
+      rc_trunc = fail(ZDT_ETRUNC, "stream_search: spectrogram %d has %lld proto-hits, capacity %lld", i,
+                      (long long)total, (long long)cap);
+    return ZDT_OK;
+  };
+  // the hits of spectrogram i, before the next spectrogram's search reuses
+  // the FDR buffers: its proto-hit count is read (one synchronization), the
+  // clustering runs on the compute stream and its results are copied out
+  auto cluster = [&](int i) -> int {
+    const int slot = i & 1;
+    ZDT_TRY_HIP(hipEventSynchronize(s->done[slot]));
+    int64_t total;
+    std::memcpy(&total, static_cast<const char*>(s->hsmall) + 16 * slot + 8, sizeof(int64_t));
+    if (total > cap) {  // not every proto-hit is staged: no hits for this spectrogram
+      cl->nclusters[i] = -1;
+      return ZDT_OK;
+    }
+    int64_t* dpeak = reinterpret_cast<int64_t*>(s->dclus);
+    int64_t* dsize = dpeak + cl->stage;
+    int64_t* dbox = dsize + cl->stage;
+    float* dvalue = reinterpret_cast<float*>(dbox + 4 * cl->stage);
+    int64_t ncl = 0;
+    ProfUse prof_(ws_profiler(s->ws));  // the clustering joins the workspace's kernel profile
+    const int rc = cluster_protohits_impl(s->dhits[slot], total, s->fdr.data(), static_cast<int>(nb), Nf, Nrb, Nf,
+                                          cl->di, cl->dj, dpeak, dvalue, dsize, dbox, cl->cap, &ncl, nullptr,
+                                          s->compute);
+    if (rc != ZDT_OK && rc != ZDT_ETRUNC) return rc;  // ETRUNC: reported once the stream is done
+    cl->nclusters[i] = ncl;
+    const int64_t got = std::min(ncl, cl->cap);
+    if (got > 0) {
+      const int64_t o = static_cast<int64_t>(i) * cl->cap;
+      ZDT_TRY_HIP(hipMemcpy(cl->peak + o, dpeak, static_cast<size_t>(got) * 8, hipMemcpyDeviceToHost));
+      ZDT_TRY_HIP(hipMemcpy(cl->value + o, dvalue, static_cast<size_t>(got) * 4, hipMemcpyDeviceToHost));
+      ZDT_TRY_HIP(hipMemcpy(cl->size + o, dsize, static_cast<size_t>(got) * 8, hipMemcpyDeviceToHost));
+      ZDT_TRY_HIP(hipMemcpy(cl->box + 4 * o, dbox, static_cast<size_t>(got) * 32, hipMemcpyDeviceToHost));
+    }
     return ZDT_OK;
   };
   // the events start out "recorded" so the first waits pass
@@ -362,11 +415,61 @@ int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, 
   for (int i = 0; i < nspec; ++i) {
     if (i + 1 < nspec) ZDT_TRY(enqueue_copy(i + 1));  // flies while spectrogram i is searched
     ZDT_TRY(enqueue_search(i));
+    if (cl) ZDT_TRY(cluster(i));
     if (i > 0) ZDT_TRY(finish(i - 1));
   }
   ZDT_TRY(finish(nspec - 1));
   ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
+  if (cl && rc_trunc == ZDT_OK)
+    for (int i = 0; i < nspec; ++i)
+      if (cl->nclusters[i] > cl->cap)
+        rc_trunc = fail(ZDT_ETRUNC, "stream_findhits: spectrogram %d has %lld hits, capacity %lld", i,
+                        (long long)cl->nclusters[i], (long long)cl->cap);
   return rc_trunc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, double snr, int64_t* hits,
+                      int64_t cap, int64_t* nhits, double* thr) {
+  if (s == nullptr) return fail(ZDT_EINVAL, "stream is NULL");
+  if (host_specs == nullptr || nspec < 1 || nhits == nullptr || cap < 0 || (cap > 0 && hits == nullptr))
+    return fail(ZDT_EINVAL, "stream_search: need host spectrograms, nhits and (for cap > 0) hits");
+  for (int i = 0; i < nspec; ++i)
+    if (host_specs[i] == nullptr) return fail(ZDT_EINVAL, "stream_search: spectrogram %d is NULL", i);
+  return stream_run(s, host_specs, nspec, snr, hits, cap, nhits, thr, nullptr);
+}
+
+int zdt_stream_findhits(zdt_stream* s, const float* const* host_specs, int nspec, double snr, int di, int dj,
+                        int64_t pcap, int64_t* peak, float* value, int64_t* size, int64_t* box, int64_t cap,
+                        int64_t* nclusters, int64_t* nprotohits, double* thr) {
+  if (s == nullptr) return fail(ZDT_EINVAL, "stream is NULL");
+  if (host_specs == nullptr || nspec < 1 || nclusters == nullptr || nprotohits == nullptr || pcap < 1 || cap < 0 ||
+      (cap > 0 && (peak == nullptr || value == nullptr || size == nullptr || box == nullptr)) || di < 0 || dj < 0)
+    return fail(ZDT_EINVAL, "stream_findhits: need host spectrograms, nclusters, nprotohits, pcap >= 1, "
+                            "di, dj >= 0 and (for cap > 0) the hit arrays");
+  for (int i = 0; i < nspec; ++i)
+    if (host_specs[i] == nullptr) return fail(ZDT_EINVAL, "stream_findhits: spectrogram %d is NULL", i);
+  if (!s->resident)
+    return fail(ZDT_ENOTSUP, "stream_findhits: the batches' FDRs are not resident together (a hit's peak needs "
+                             "every batch); use a larger hbm_bytes or zdt_stream_search");
+  if (s->Nf * s->nb * s->Nrb >= (int64_t(1) << 31) || pcap >= (int64_t(1) << 31))
+    return fail(ZDT_ENOTSUP, "stream_findhits: 2^31 cells or proto-hits and more");
+  ZDT_TRY_HIP(hipSetDevice(s->device));
+  const int64_t ccap = std::max<int64_t>(cap, 1);
+  if (ccap > s->ccap) {
+    ZDT_TRY_HIP(hipStreamSynchronize(s->compute));
+    if (s->dclus) (void)hipFree(s->dclus);
+    s->dclus = nullptr;
+    s->bytes -= 52 * s->ccap;
+    s->ccap = 0;
+    ZDT_TRY(salloc(s, reinterpret_cast<void**>(&s->dclus), static_cast<size_t>(52 * ccap)));
+    s->ccap = ccap;
+  }
+  StreamHits cl{di, dj, peak, value, size, box, cap, s->ccap, nclusters};
+  return stream_run(s, host_specs, nspec, snr, nullptr, pcap, nprotohits, thr, &cl);
 }
 
 }  // extern "C"

@@ -112,6 +112,8 @@ SIGNATURES = {
     "zdt_stream_destroy": (C.c_int, [_vp]),
     "zdt_stream_info": (C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), _i64p]),
     "zdt_stream_search": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_double, _vp, _i64, _vp, _vp]),
+    "zdt_stream_findhits": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_double, C.c_int, C.c_int, _i64, _vp, _vp,
+                                      _vp, _vp, _i64, _vp, _vp, _vp]),
 }
 
 

@@ -103,6 +103,56 @@ class StreamSearch:
             pass
 
     def search(self, specs: Sequence, snr: float = 10.0, cap: int = 1 << 20) -> Tuple[List[np.ndarray], np.ndarray]:
+        keep, ptrs = self._pointers(specs)
+        n = len(ptrs)
+        hits = np.zeros(max(1, n * cap), dtype=np.int64)
+        nh = np.zeros(n, dtype=np.int64)
+        thr = np.zeros(n, dtype=np.float64)
+        rc = _lib.lib().zdt_stream_search(self._h, (C.c_void_p * n)(*ptrs), n, float(snr),
+                                          hits.ctypes.data_as(C.c_void_p), int(cap), nh.ctypes.data_as(C.c_void_p),
+                                          thr.ctypes.data_as(C.c_void_p))
+        _lib.check(rc)
+        out = []
+        for i in range(n):
+            lin = hits[i * cap: i * cap + nh[i]]
+            out.append(np.stack([lin % self.Nf + 1, lin // self.Nf + 1], axis=1).astype(np.int64)
+                       if nh[i] else np.zeros((0, 2), np.int64))
+        return out, thr
+
+    def findhits(self, specs: Sequence, snr: float = 10.0, di: int = 1, dj: int = 1, cap: int = 1 << 16,
+                 pcap: int = 1 << 22) -> Tuple[List["Hits"], np.ndarray, np.ndarray]:
+        """The search, ending in hits: each spectrogram's proto-hits are
+        clustered on the device (as ``fdrt_hip.clusterprotohits`` clusters
+        them) while its batches' FDRs are resident.  Returns (one ``Hits`` per
+        spectrogram, with empty ``labels``; the proto-hit counts; the
+        thresholds).  Needs a resident stream (``self.resident``)."""
+        from .fdrutils import Hits
+        keep, ptrs = self._pointers(specs)
+        n = len(ptrs)
+        c1 = max(int(cap), 1)
+        peak = np.zeros(n * c1, np.int64)
+        value = np.zeros(n * c1, np.float32)
+        size = np.zeros(n * c1, np.int64)
+        box = np.zeros(4 * n * c1, np.int64)
+        ncl = np.zeros(n, np.int64)
+        nph = np.zeros(n, np.int64)
+        thr = np.zeros(n, np.float64)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        rc = _lib.lib().zdt_stream_findhits(self._h, (C.c_void_p * n)(*ptrs), n, float(snr), int(di), int(dj),
+                                            int(pcap), vp(peak), vp(value), vp(size), vp(box), int(cap), vp(ncl),
+                                            vp(nph), vp(thr))
+        _lib.check(rc)
+        out = []
+        for i in range(n):
+            h = int(ncl[i])
+            o = i * int(cap)
+            pk = peak[o:o + h]
+            out.append(Hits(peak=np.stack([pk % self.Nf + 1, pk // self.Nf + 1], axis=1).astype(np.int64),
+                            value=value[o:o + h].copy(), size=size[o:o + h].copy(),
+                            box=box[4 * o:4 * (o + h)].reshape(h, 4) + 1, labels=np.zeros(0, np.int64)))
+        return out, nph, thr
+
+    def _pointers(self, specs: Sequence):
         keep = []
         ptrs = []
         for s in specs:
@@ -119,20 +169,7 @@ class StreamSearch:
                     raise ValueError(f"expected ({self.Nf}, {self.Nt}) spectrograms")
                 keep.append(a)
                 ptrs.append(a.ctypes.data)
-        n = len(ptrs)
-        hits = np.zeros(max(1, n * cap), dtype=np.int64)
-        nh = np.zeros(n, dtype=np.int64)
-        thr = np.zeros(n, dtype=np.float64)
-        rc = _lib.lib().zdt_stream_search(self._h, (C.c_void_p * n)(*ptrs), n, float(snr),
-                                          hits.ctypes.data_as(C.c_void_p), int(cap), nh.ctypes.data_as(C.c_void_p),
-                                          thr.ctypes.data_as(C.c_void_p))
-        _lib.check(rc)
-        out = []
-        for i in range(n):
-            lin = hits[i * cap: i * cap + nh[i]]
-            out.append(np.stack([lin % self.Nf + 1, lin // self.Nf + 1], axis=1).astype(np.int64)
-                       if nh[i] else np.zeros((0, 2), np.int64))
-        return out, thr
+        return keep, ptrs
 
     def search_files(self, paths: Sequence, snr: float = 10.0, cap: int = 1 << 20, channels=None, ifnum: int = 0,
                      times=None, group: int = 2, dataset: str = "data") -> Tuple[List[np.ndarray], np.ndarray]:

@@ -500,4 +500,46 @@ function search(S::StreamSearch, specs::Vector{Matrix{Float32}}, snr; cap::Integ
     out, thr
 end
 
+"""
+    findhits(S, specs::Vector{Matrix{Float32}}, snr; di=1, dj=1, cap=2^16, pcap=2^22)
+        -> (hits, nprotohits, thresholds)
+
+The search ending in hits (`zdt_stream_findhits`): spectrogram i's proto-hits
+clustered on the device while its batches' FDRs are resident, as
+`clusterprotohits` clusters them.  `hits[i]` has the fields of
+`clusterprotohits`' result, with empty `labels`.  Needs a resident stream.
+"""
+function findhits(S::StreamSearch, specs::Vector{Matrix{Float32}}, snr; di::Integer=1, dj::Integer=1,
+                  cap::Integer=2^16, pcap::Integer=2^22)
+    n = length(specs)
+    all(size(s) == (S.Nf, S.Nt) for s in specs) || error("spectrograms must be $(S.Nf) x $(S.Nt)")
+    c1 = max(cap, 1)
+    peak = zeros(Int64, n * c1)
+    value = zeros(Float32, n * c1)
+    sz = zeros(Int64, n * c1)
+    box = zeros(Int64, 4 * n * c1)
+    ncl = zeros(Int64, n)
+    nph = zeros(Int64, n)
+    thr = zeros(Float64, n)
+    GC.@preserve specs begin
+        rc = ccall((:zdt_stream_findhits, LIBZDT), Cint,
+                   (Ptr{Cvoid}, Ptr{Ptr{Float32}}, Cint, Cdouble, Cint, Cint, Int64, Ptr{Int64}, Ptr{Float32},
+                    Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cdouble}),
+                   S.handle, [pointer(s) for s in specs], n, snr, di, dj, pcap, peak, value, sz, box, cap, ncl,
+                   nph, thr)
+    end
+    check(rc)
+    hits = map(1:n) do i
+        h = Int(ncl[i])
+        o = (i - 1) * cap
+        bx = reshape(box[4o+1:4(o+h)], 4, h)
+        (; peak = _cartesian(peak[o+1:o+h], S.Nf),
+           value = value[o+1:o+h],
+           size = Int.(sz[o+1:o+h]),
+           box = [Tuple(Int.(bx[:, q]) .+ 1) for q in 1:h],
+           labels = Int[])
+    end
+    hits, nph, thr
+end
+
 end # module ZDTHip

@@ -406,6 +406,28 @@ int zdt_stream_workspace(const zdt_stream* s, zdt_ws** ws);
 int zdt_stream_search(zdt_stream* s, const float* const* host_specs, int nspec, double snr,
                       int64_t* hits, int64_t cap, int64_t* nhits, double* thr);
 
+/* The same search, ending in hits instead of proto-hits: each spectrogram's
+ * proto-hits are clustered as zdt_clusterprotohits clusters them (radii di,
+ * dj over the hcat'd batches), on the device, while every batch's FDR is
+ * still resident, before the next spectrogram's search reuses the buffers.
+ * No counterpart in the reference, which leaves the clustering to the next
+ * stage of a search (src/fdrutils.jl:141-144, README.md:48-53).
+ *
+ * pcap: proto-hits staged per spectrogram; a spectrogram with more gets
+ * nclusters[i] = -1 and no hits.  For spectrogram i (HOST arrays):
+ * nprotohits[i], thr[i] (may be NULL) as zdt_stream_search gives them,
+ * nclusters[i], and its first min(nclusters[i], cap) hits at peak[i*cap ...],
+ * value[i*cap ...], size[i*cap ...] and box[4*i*cap ...] (0-based linear
+ * peak index i + Nf*(b*Nrb + j); box: first and last channel, first and last
+ * hcat'd column).  Returns ZDT_ETRUNC when some spectrogram has more than pcap
+ * proto-hits or more than cap hits, ZDT_ENOTSUP for a stream whose batches
+ * are not resident together (a hit's peak needs every batch).  Synchronizes
+ * the host with each spectrogram once, after its proto-hits. */
+int zdt_stream_findhits(zdt_stream* s, const float* const* host_specs, int nspec, double snr,
+                        int di, int dj, int64_t pcap, int64_t* peak, float* value, int64_t* size,
+                        int64_t* box, int64_t cap, int64_t* nclusters, int64_t* nprotohits,
+                        double* thr);
+
 #ifdef __cplusplus
 }
 #endif

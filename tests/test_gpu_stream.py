@@ -111,6 +111,51 @@ def test_stream_search_vs_fp64(F, mode):
     st.close()
 
 
+@pytest.mark.parametrize("di,dj", [(1, 1), (2, 0)])
+def test_stream_findhits(F, di, dj):
+    """zdt_stream_findhits: the search ending in hits.  Nr = 33 in batches of
+    10, resident: the hits of each spectrogram equal, exactly, the clustering
+    of the stream's own proto-hits over the hcat of the batch FDRs (the
+    search's own batches, rebuilt by workspaces), on the device and by the
+    oracle on the host.  A non-resident stream is refused, and too small a
+    proto-hit staging is reported."""
+    from fdrt_hip import _lib
+    from oracle import cluster_oracle as CO
+    specs = specs3()
+    st = F.StreamSearch(NF, NT, (R0, DR, NR), Nrb=10)
+    assert st.resident and st.nbatch == 4
+    snr = 6.0
+    got, nph, thr = st.findhits(specs, snr, di=di, dj=dj)
+    ph, thr2 = st.search(specs, snr)
+    assert np.array_equal(thr, thr2)
+    for i, spec in enumerate(specs):
+        say(f"stream findhits {i}: {nph[i]} proto-hits, {len(got[i].size)} hits")
+        assert nph[i] == len(ph[i]) > 0
+        fdrs = []
+        for b in range(4):
+            ws = F.ZDTWorkspace(spec, R0 + b * 10 * DR, DR, 10)
+            fdrs.append(F.zdtfdr(ws))
+            ws.close()
+        ref = F.clusterprotohits(fdrs, ph[i], di, dj)
+        for name in ("peak", "value", "size", "box"):
+            assert np.array_equal(getattr(got[i], name), getattr(ref, name)), name
+        if i == 0:
+            o = CO.clusterprotohits([f.cpu().numpy() for f in fdrs], ph[i], di, dj)
+            for name in ("peak", "value", "size", "box"):
+                assert np.array_equal(getattr(got[i], name), o[name]), name
+    # too few hits or proto-hits staged: reported, not wrong
+    with pytest.raises(_lib.ZDTError, match="capacity"):
+        st.findhits(specs[:1], snr, di=di, dj=dj, cap=1)
+    with pytest.raises(_lib.ZDTError, match="proto-hits"):
+        st.findhits(specs[:1], snr, di=di, dj=dj, pcap=3)
+    st.close()
+    rec = F.StreamSearch(NF, NT, (R0, DR, NR), Nrb=10, hbm_bytes=36 << 20)
+    assert not rec.resident
+    with pytest.raises(_lib.ZDTError, match="resident"):
+        rec.findhits(specs[:1], snr)
+    rec.close()
+
+
 def test_stream_auto_batch_and_truncation(F):
     specs = specs3()[:1]
     st = F.StreamSearch(NF, NT, (R0, DR, NR))
