@@ -8,7 +8,7 @@ One step = zdtfdr!(dest, ws, spectrogram) = input! + one ZDT batch writing the
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 100] [--no-cpu-baseline]
     python bench.py --ragged [--channels 64] [--steps 20] [--warmup 5]
-    python bench.py --long [--spectrograms 6] [--precision fp32|split-bf16|split-fp16]
+    python bench.py --long [--spectrograms 6] [--precision fp32|split-bf16|split-fp16] [--hits]
     python bench.py --cluster [--steps 20] [--warmup 5]
     python bench.py --copy-calibration | --valu-calibration | --lds-calibration
 
@@ -315,12 +315,20 @@ def long_integration(a) -> int:
     noise + 64 tones, amplitudes log-uniform 5..100) streamed from pinned host
     memory, rates -8:1/64:8 (Nr=1025) in HBM-sized batches, SNR-10 proto-hits.
     Prints one JSON line (FDR cells/s over the whole search, host copies
-    included)."""
+    included).  With --hits the search ends in hits (StreamSearch.findhits:
+    each spectrogram's proto-hits clustered, 8-neighbourhood, on the device)."""
     import torch
 
     import fdrt_hip as F
     torch.cuda.set_device(0)
     Nf, Nt, r0, dr, Nr = 1 << 22, 64, -8.0, 1.0 / 64, 1025
+
+    def run(stream, ss):
+        if a.hits:
+            h, nph, t = stream.findhits(ss, snr=10.0, cap=1 << 18)
+            return [len(x.size) for x in h], [int(v) for v in nph]
+        h, t = stream.search(ss, snr=10.0)
+        return [len(x) for x in h], None
     rng = np.random.default_rng(2024)
     specs = []
     for _ in range(a.spectrograms):
@@ -333,13 +341,13 @@ def long_integration(a) -> int:
         p.t().copy_(torch.from_numpy(rows))
         specs.append(p)
     st = F.StreamSearch(Nf, Nt, (r0, dr, Nr), precision=a.precision)
-    st.search(specs[:1], snr=10.0)  # warm-up
+    run(st, specs[:1])  # warm-up
     # one spectrogram alone: its host copy is not overlapped with anything
     t0 = time.perf_counter()
-    st.search(specs[:1], snr=10.0)
+    run(st, specs[:1])
     dt1 = time.perf_counter() - t0
     t0 = time.perf_counter()
-    hits, thr = st.search(specs, snr=10.0)
+    nhits, nproto = run(st, specs)
     dt = time.perf_counter() - t0
     n = a.spectrograms
     cells = n * Nf * Nr
@@ -351,10 +359,10 @@ def long_integration(a) -> int:
     # the kernels of the same search, timed one by one (HIP events around every
     # launch, hit extraction included) on a second search after the timed one
     sp = F.StreamSearch(Nf, Nt, (r0, dr, Nr), precision=a.precision, profile=True)
-    sp.search(specs[:1], snr=10.0)
+    run(sp, specs[:1])
     sp.reset_profile()
     nprof = min(2, n)
-    sp.search(specs[:nprof], snr=10.0)
+    run(sp, specs[:nprof])
     kt = sp.kernel_times()
     sp.close()
     ksum = sum(ms for _, ms, _ in kt) / nprof
@@ -368,9 +376,10 @@ def long_integration(a) -> int:
         "kernels": [{"name": k, "ms_per_spectrogram": ms / nprof, "launches_per_spectrogram": nl / nprof}
                     for k, ms, nl in kt],
         "data": "synthetic (seed 2024)",
-        "config": {"workload": "Nf=2^22, Nt=64, rates -8:1/64:8 (Nr=1025), SNR 10", "precision": a.precision,
-                   **info},
-        "hits": [int(len(h)) for h in hits],
+        "config": {"workload": "Nf=2^22, Nt=64, rates -8:1/64:8 (Nr=1025), SNR 10"
+                   + (", clustered into hits" if a.hits else ""), "precision": a.precision, **info},
+        "hits" if a.hits else "protohits": nhits,
+        **({"protohits": nproto} if a.hits else {}),
     }))
     return 0
 
@@ -455,6 +464,7 @@ def main(argv=None) -> int:
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--long", action="store_true", help="the long-integration streaming search instead")
+    ap.add_argument("--hits", action="store_true", help="--long: end the search in hits (proto-hits clustered)")
     ap.add_argument("--spectrograms", type=int, default=6)
     ap.add_argument("--precision", default="fp32", choices=("fp32", "split-bf16", "split-fp16"),
                     help="--long: the column DFTs' arithmetic")

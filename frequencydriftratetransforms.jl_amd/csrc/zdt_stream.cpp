@@ -455,8 +455,10 @@ int zdt_stream_findhits(zdt_stream* s, const float* const* host_specs, int nspec
   if (!s->resident)
     return fail(ZDT_ENOTSUP, "stream_findhits: the batches' FDRs are not resident together (a hit's peak needs "
                              "every batch); use a larger hbm_bytes or zdt_stream_search");
-  if (s->Nf * s->nb * s->Nrb >= (int64_t(1) << 31) || pcap >= (int64_t(1) << 31))
-    return fail(ZDT_ENOTSUP, "stream_findhits: 2^31 cells or proto-hits and more");
+  // the clustering's limits (zdt_clusterprotohits): channels, hcat'd columns
+  // and proto-hits each below 2^31
+  if (s->Nf >= (int64_t(1) << 31) || s->nb * s->Nrb >= (int64_t(1) << 31) || pcap >= (int64_t(1) << 31))
+    return fail(ZDT_ENOTSUP, "stream_findhits: 2^31 channels, columns or proto-hits and more");
   ZDT_TRY_HIP(hipSetDevice(s->device));
   const int64_t ccap = std::max<int64_t>(cap, 1);
   if (ccap > s->ccap) {
