@@ -303,6 +303,10 @@ def test_long_integration_shape_vs_fp64(F):
     say(f"long resident: Nrb={res.Nrb} nbatch={res.nbatch} resident={res.resident}")
     assert res.resident
     h1, t1 = res.search([p], snr=10.0)
+    # the same search ending in hits (4.3e9 cells: the general clustering
+    # kernels, 64-bit cell indices), checked against the FDR below
+    hh, nph, th = res.findhits([p], snr=10.0, cap=1 << 18)
+    assert nph[0] == len(h1[0]) and th[0] == t1[0]
     res.close()
     torch.cuda.empty_cache()
     # 16 GiB: room for the workspace (~8 GB at 205 rates), the two
@@ -337,6 +341,10 @@ def test_long_integration_shape_vs_fp64(F):
     full = F.findprotohits(fdr, t1[0])
     say(f"long: {len(full)} proto-hits by a full read, {len(h1[0])} by the search")
     assert np.array_equal(full, h1[0])
+    ref_hits = F.clusterprotohits(fdr, full, 1, 1, capacity=1 << 18)
+    say(f"long: {len(ref_hits.size)} hits")
+    for name in ("peak", "value", "size", "box"):
+        assert np.array_equal(getattr(hh[0], name), getattr(ref_hits, name)), name
     del fdr
     ws.close()
     torch.cuda.empty_cache()
